@@ -1,0 +1,211 @@
+"""Benchmark of the fused kernel summation on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A step is one summation u += K(XA, XB) w over the whole workload, enqueued
+through the C ABI (dgsks_device) on torch's current stream with every input
+already resident in HBM.  The default workload is the k = 64 Gaussian mixture
+(m = n = 131072, h = 0.3) on which the FP64-vs-mixed-precision metric is
+quoted.  One JSON line is printed with evals/s, the roofline position of the
+fused kernel (HIP events around each launch) and a CPU baseline: the
+reference's own dgsks (compiled from its sources into oracle/_ref) timed on
+this host's cores over a bounded sample of target rows of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "kernel evals/s (m·n/s), FP64 drop-in vs mixed precision at rel err ≤1e-6"
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector = FP64 matrix peak
+# flops per kernel evaluation in the reference's model, test/test_dgsks.c:254-283
+FLOP_C = {0: 37, 1: 6, 2: 60, 3: 35, 4: 89, 5: 8, 6: 6, 7: 7}
+TYPE_NAMES = ["gaussian", "polynomial", "laplace", "var_bandwidth", "tanh", "quartic",
+              "multiquadric", "epanechnikov"]
+
+
+def build_workload(name):
+    from ks_amd import workloads as W
+    if name == "mixture_k64":
+        d = W.mixture_k64()
+        d["kernel"] = dict(d["kernel"])
+        d["label"] = "mixture_k64 gaussian h=0.3 m=n=131072 k=64"
+    elif name == "mixture_k64_varbw":
+        d = W.mixture_k64()
+        d["kernel"] = dict(type=W.VAR_BANDWIDTH)
+        d["label"] = "mixture_k64 variable-bandwidth gaussian h~U[0.2,0.4) m=n=131072 k=64"
+    elif name == "uniform_gauss16":
+        d = W.uniform_gauss16()
+        d["label"] = "uniform_gauss16 gaussian h=0.5 m=n=8192 k=16"
+    elif name.startswith("nbody_k3"):
+        kind = name.split(":")[1] if ":" in name else "gaussian"
+        d = W.nbody_k3()
+        d["kernel"] = dict(W.NBODY_KERNELS[kind])
+        d["label"] = f"nbody_k3 {kind} m=n=262144 k=3 permutation maps"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return d
+
+
+def cpu_baseline(d, budget_s=8.0, max_threads=16):
+    """The reference dgsks (oracle/_ref) on this host, rows split over threads.
+
+    Each thread calls the reference's sequential dgsks on its own slice of the
+    target rows (the reference's own unit of parallel work, one dgsks per
+    task, frame/omp_dgsks_list.cpp:366-397)."""
+    from oracle import oracle as O
+    R = O.ref_lib()
+    kind = "reference"
+    if R is None:
+        return None
+    X, X2, w = d["X"], d["X2"], d["w"]
+    amap, bmap = d["amap"], d["bmap"]
+    kd = d["kernel"]
+    n, k = bmap.size, d["k"]
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    threads = max(1, min(max_threads, ncores))
+    hv = d.get("hinv")
+
+    def run(rows_lo, rows_hi, out):
+        am = np.ascontiguousarray(amap[rows_lo:rows_hi])
+        kt = O.RefKsT(int(kd["type"]), float(kd.get("powe", 0.0)), float(kd.get("scal", 0.0)),
+                      float(kd.get("cons", 0.0)), None, None, None)
+        if hv is not None:
+            kt.hi = kt.hj = hv.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        R.dgsks(ctypes.byref(kt), am.size, n, k, O._d(out), O._i(am), O._d(X), O._d(X2),
+                O._i(am), O._d(X), O._d(X2), O._i(bmap), O._d(w), O._i(bmap))
+
+    out = np.zeros(X.shape[0])
+    # calibrate on one thread
+    rows1 = max(1, min(amap.size, int(2e6 // max(n, 1)) + 1))
+    t0 = time.perf_counter()
+    run(0, rows1, out)
+    rate1 = rows1 * n / max(time.perf_counter() - t0, 1e-9)
+    rows = int(min(amap.size, max(threads, budget_s * rate1 * threads / n)))
+    rows -= rows % threads
+    rows = max(rows, threads)
+    per = rows // threads
+    ths = [threading.Thread(target=run, args=(i * per, (i + 1) * per, out)) for i in range(threads)]
+    t0 = time.perf_counter()
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    dt = time.perf_counter() - t0
+    return {"value": rows * n / dt, "unit": "evals/s", "cores": threads, "kind": kind,
+            "sample": f"reference dgsks (oracle/_ref, sandybridge AVX build) on {rows} of "
+                      f"{amap.size} target rows x all {n} sources, {threads} threads "
+                      f"x {per} rows, {dt:.2f} s",
+            "one_core_value": rate1}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="mixture_k64")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed_fp32", "mixed_bf16x3"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the fused summation is a single-GPU kernel "
+              "here (multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ks_amd import _lib
+
+    L = _lib.lib()
+    d = build_workload(args.workload)
+    kd = d["kernel"]
+    X = torch.from_numpy(d["X"]).cuda()
+    X2 = torch.from_numpy(d["X2"]).cuda()
+    w = torch.from_numpy(d["w"]).cuda()
+    amap = torch.from_numpy(d["amap"]).cuda()
+    bmap = torch.from_numpy(d["bmap"]).cuda()
+    u = torch.zeros(X.shape[0], dtype=torch.float64, device="cuda")
+    hdev = torch.from_numpy(d["hinv"]).cuda() if "hinv" in d else None
+    m, n, k = amap.numel(), bmap.numel(), d["k"]
+    kt = _lib.KsT()
+    kt.type = kd["type"]
+    kt.scal = kd.get("scal", 0.0)
+    kt.cons = kd.get("cons", 0.0)
+    kt.powe = kd.get("powe", 0.0)
+    if kd["type"] == 3:
+        kt.hi = kt.hj = hdev.data_ptr()
+    prec = {"fp64": 0, "mixed_fp32": 1, "mixed_bf16x3": 2}[args.precision]
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    argv = (ctypes.byref(kt), m, n, k, P(u), P(amap), P(X), P(X2), P(amap), P(X), P(X2), P(bmap),
+            P(w), P(bmap), prec, sp)
+
+    def step():
+        rc = L.dgsks_device(*argv)
+        if rc != 0:
+            raise RuntimeError(f"dgsks_device failed: {_lib.error_string(rc)}")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [a.elapsed_time(b) for a, b in ev]
+    ms_per_step = wall * 1e3 / args.steps
+    evals = float(m) * float(n)
+    value = evals / (ms_per_step / 1e3)
+    kernel_ms = float(np.mean(kms))
+    flops_per_eval = 2 * k + FLOP_C[kd["type"]]
+    achieved = evals * flops_per_eval / (kernel_ms / 1e3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == 0 else ("f32+f64" if prec == 1 else "bf16x3+f32+f64"),
+        "data": "synthetic (seeded numpy, regenerated each run)",
+        "config": {"workload": d["label"], "precision": args.precision, "m": m, "n": n, "k": k,
+                   "kernel": TYPE_NAMES[kd["type"]]},
+        "roofline": {"bound": "mfma", "pipe": "fp64 (VALU and matrix peaks are equal)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_eval": flops_per_eval, "kernel_ms": kernel_ms,
+                     "kernel": "gsks::fused_f64_kernel"},
+        "kernel_evals_per_s": evals / (kernel_ms / 1e3),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(d, budget_s=args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

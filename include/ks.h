@@ -1,0 +1,191 @@
+/*
+ * ks.h -- public C API of the MI355X fused kernel summation library
+ * (ks_amd/libgsks.so).
+ *
+ * The first half is ABI-identical to the reference header (include/ks.h:46-111
+ * of ChenhanYu/ks): the same KS_RHS / KS_NUM_THREAD constants, the same
+ * ks_type enumerator order, the same struct kernel_s field order and size
+ * (56 bytes on LP64), and the same dgsks / dgsks_ref / ks_malloc_aligned
+ * prototypes.  A program written against the reference compiles and links
+ * against this header and library unchanged.
+ *
+ * Semantics kept from the reference:
+ *   u[umap[i]] += sum_j K(XA[amap[i]], XB[bmap[j]]) * w[wmap[j]]
+ *   - XA / XB are row-major [npoints][k] (k leading), XA2 / XB2 the caller's
+ *     squared row norms, all maps are int32 global point ids;
+ *   - u is accumulated, never overwritten;
+ *   - KS_LAPLACE writes kernel->powe = (2-k)/2 and
+ *     kernel->scal = Gamma(k/2+1) / (k (k-2) pi^(k/2))  (frame/dgsks.c:445-447);
+ *   - KS_GAUSSIAN_VAR_BANDWIDTH reads kernel->hi[amap[i]] and kernel->hj[bmap[j]]
+ *     (global ids) and evaluates exp(-0.5 d^2 hi hj);
+ *   - KS_MULTIQUADRATIC is d^2 + cons (no square root);
+ *   - m, n or k == 0 returns early.
+ * Pointers may be host memory (the library stages them through HBM) or
+ * device memory (used in place, no copies); see INTEGRATION.md.
+ *
+ * Everything after the "extensions" marker has no counterpart in the reference.
+ */
+#ifndef __KS_H__
+#define __KS_H__
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <math.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KS_RHS 1          /* reference include/ks.h:46 */
+#define KS_NUM_THREAD 68  /* reference include/ks.h:47 (kept for source compatibility) */
+
+/* reference include/ks.h:49-58 -- the enumerator order is ABI */
+typedef enum {
+  KS_GAUSSIAN,
+  KS_POLYNOMIAL,
+  KS_LAPLACE,
+  KS_GAUSSIAN_VAR_BANDWIDTH,
+  KS_TANH,
+  KS_QUARTIC,
+  KS_MULTIQUADRATIC,
+  KS_EPANECHNIKOV
+} ks_type;
+
+/* reference include/ks.h:60-69 */
+struct kernel_s {
+  ks_type type;
+  double powe;
+  double scal;
+  double cons;
+  double *hi; /* variable-bandwidth factors, indexed by global target id */
+  double *hj; /* variable-bandwidth factors, indexed by global source id */
+  double *h;  /* unused by dgsks (as in the reference) */
+};
+
+typedef struct kernel_s ks_t;
+
+/* Fused FP64 kernel summation on the GPU.  Replaces reference include/ks.h:73-88
+ * (frame/dgsks.c:367-776).  One fused HIP kernel per call; never forms the
+ * m x n kernel matrix.  Returns when u holds the result. */
+void dgsks(
+    ks_t   *kernel,
+    int    m,
+    int    n,
+    int    k,
+    double *u,
+    int    *umap,
+    double *XA,
+    double *XA2,
+    int    *amap,
+    double *XB,
+    double *XB2,
+    int    *bmap,
+    double *w,
+    int    *wmap
+    );
+
+/* The reference algorithm on the GPU: gather, FP64 GEMM of the cross term,
+ * elementwise kernel function, GEMV, scatter -- unfused, in column panels.
+ * Replaces reference include/ks.h:90-105 (frame/dgsks_ref.c:94-535). */
+void dgsks_ref(
+    ks_t   *kernel,
+    int    m,
+    int    n,
+    int    k,
+    double *u,
+    int    *umap,
+    double *XA,
+    double *XA2,
+    int    *amap,
+    double *XB,
+    double *XB2,
+    int    *bmap,
+    double *w,
+    int    *wmap
+    );
+
+/* 64-byte aligned host allocation of m*n elements of `size` bytes.  Replaces
+ * reference include/ks.h:107-111 (frame/ks_util.c:34-55). */
+double *ks_malloc_aligned(
+    int    m,
+    int    n,
+    int    size
+    );
+
+/* Flat ctypes entry points: replace frame/dgsks.c:783-826 and
+ * frame/dgsks_ref.c:537-580 (called by python/gsks.py:85,179).  The descriptor
+ * is built with hi = hj = h and, unlike the reference, with cons set. */
+void dgsks_wrapper(int m, int n, int k, double *u, int *umap, double *XA, double *XA2,
+                   int *alpha, double *XB, double *XB2, int *beta, double *w, int *omega,
+                   int type, double scal, double cons, double powe, double *h);
+void dgsks_ref_wrapper(int m, int n, int k, double *u, int *umap, double *XA, double *XA2,
+                       int *alpha, double *XB, double *XB2, int *beta, double *w, int *omega,
+                       int type, double scal, double cons, double powe, double *h);
+
+/* ======================= extensions (no reference counterpart) ============ */
+
+/* Error reporting.  The reference prints and calls exit(1) on an illegal kernel
+ * type (frame/dgsks.c:465-467); that stays the default (KS_ERRMODE_EXIT).  With
+ * KS_ERRMODE_RETURN the call prints nothing, leaves u untouched and records the
+ * code, which ks_last_error() returns (per thread) and then clears. */
+enum {
+  KS_OK = 0,
+  KS_ERR_INVALID_TYPE = 1,   /* kernel->type outside ks_type */
+  KS_ERR_INVALID_ARG = 2,    /* negative size, NULL array, k too large */
+  KS_ERR_NO_BANDWIDTH = 3,   /* VAR_BANDWIDTH with hi or hj NULL */
+  KS_ERR_LAPLACE_K = 4,      /* LAPLACE with k < 3 (reference prints and continues) */
+  KS_ERR_DEVICE = 5,         /* no usable GPU, or a HIP call failed */
+  KS_ERR_ALLOC = 6,          /* device or pinned allocation failed */
+  KS_ERR_MIXED_POINTERS = 7, /* some arrays on the device, others on the host */
+  KS_ERR_LIST = 8            /* inconsistent list description */
+};
+enum { KS_ERRMODE_EXIT = 0, KS_ERRMODE_RETURN = 1 };
+
+int ks_last_error(void);
+const char *ks_error_string(int code);
+void ks_set_error_mode(int mode);
+int ks_get_error_mode(void);
+
+/* Precision of a summation. */
+enum {
+  KS_PREC_FP64 = 0,       /* everything FP64 (the drop-in) */
+  KS_PREC_MIXED_FP32 = 1, /* FP32 cross term, FP32 K(.), FP64 norms/near pairs/u */
+  KS_PREC_MIXED_BF16X3 = 2 /* split-BF16 (hi/lo, 3 products) cross term on MFMA */
+};
+
+/* Stream-ordered device entry point: every array is a device pointer, the work
+ * is enqueued on `stream` (a hipStream_t; NULL = the per-thread default
+ * stream), the call does not synchronise and does not allocate after the first
+ * call on a device.  Returns a KS_* code.  kernel is read, not written (the
+ * Laplace parameters are derived internally).  hi / hj must be device
+ * pointers for VAR_BANDWIDTH. */
+int dgsks_device(const ks_t *kernel, int m, int n, int k,
+                 double *u, const int *umap,
+                 const double *XA, const double *XA2, const int *amap,
+                 const double *XB, const double *XB2, const int *bmap,
+                 const double *w, const int *wmap, int precision, void *stream);
+
+/* Squared row norms X2[i] = sum_p X[i*k+p]^2 on the device (the norm
+ * precompute of frame/omp_dgsks_list.cpp:75-81). */
+int ks_sqnorm_device(const double *X, long nx, int k, double *X2, void *stream);
+
+/* Batched summation: ntask independent summations over shared point pools,
+ * one persistent launch.  Task t owns the ranges [a_off[t], a_off[t+1]) of
+ * amap/umap and [b_off[t], b_off[t+1]) of bmap/wmap and uses kernels[t]
+ * (one descriptor per task; the reference's omp_dgsks_list takes a single
+ * descriptor, frame/omp_dgsks_list.cpp:297-420).  u[x] accumulates the sum of
+ * every task's contribution to point x, so tasks may share target points.
+ * Host pointers (staged) or device pointers.  kernels and the offset arrays
+ * are always host memory. */
+int dgsks_list(int ntask, ks_t *kernels, int k, double *u, long nu,
+               const long *a_off, const int *umap, const int *amap,
+               const long *b_off, const int *bmap, const int *wmap,
+               const double *XA, const double *XA2,
+               const double *XB, const double *XB2, const double *w,
+               int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* __KS_H__ */

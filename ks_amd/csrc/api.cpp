@@ -1,0 +1,350 @@
+// C-ABI of libgsks.so: the reference's dgsks()/dgsks_ref()/wrappers/ks_malloc_aligned
+// (include/ks.h:73-111, frame/dgsks.c:783-826, frame/dgsks_ref.c:537-580) plus the
+// stream-ordered device entry points.  Host-side only: validation, the Laplace
+// parameter side effect, staging of host arrays through HBM, scratch management
+// and kernel launches.  Every summation runs on the GPU; there is no CPU path.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "../../include/ks.h"
+#include "gsks_launch.h"
+#include "host_util.h"
+
+using namespace gsks;
+
+namespace gsks {
+
+// ------------------------------------------------------------------ errors
+
+static std::atomic<int> g_errmode{KS_ERRMODE_EXIT};
+static thread_local int t_last_error = KS_OK;
+
+const char *err_string(int code) {
+  switch (code) {
+    case KS_OK: return "ok";
+    case KS_ERR_INVALID_TYPE: return "illegal kernel type";
+    case KS_ERR_INVALID_ARG: return "invalid argument";
+    case KS_ERR_NO_BANDWIDTH: return "bandwidth vector has not been initialized";
+    case KS_ERR_LAPLACE_K: return "laplace kernel only supports k > 2";
+    case KS_ERR_DEVICE: return "GPU unavailable or HIP call failed";
+    case KS_ERR_ALLOC: return "allocation failed";
+    case KS_ERR_MIXED_POINTERS: return "inconsistent host/device pointers";
+    case KS_ERR_LIST: return "inconsistent list description";
+    default: return "unknown error";
+  }
+}
+
+// Record an error.  In KS_ERRMODE_EXIT a fatal error prints the reference's style
+// of message and exits(1) (frame/dgsks.c:465-467); warnings only print.
+int raise_error(int code, bool fatal, const char *fmt, ...) {
+  t_last_error = code;
+  if (g_errmode.load() == KS_ERRMODE_EXIT) {
+    va_list ap;
+    va_start(ap, fmt);
+    vprintf(fmt, ap);
+    va_end(ap);
+    fflush(stdout);
+    if (fatal) exit(1);
+  }
+  return code;
+}
+
+void set_last_error(int code) { t_last_error = code; }
+
+// ------------------------------------------------------------------ device context
+
+static std::mutex g_mu;
+
+struct Scratch {
+  double *partial = nullptr;
+  size_t partial_elems = 0;
+  unsigned *counters = nullptr;
+  size_t counter_elems = 0;
+};
+static std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
+
+int ensure_device() {
+  static int ndev = -1;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    (void)hipGetLastError();
+    ndev = n;
+  });
+  return ndev > 0 ? KS_OK : KS_ERR_DEVICE;
+}
+
+hipStream_t thread_stream() {
+  static thread_local hipStream_t s = nullptr;
+  static thread_local int dev = -1;
+  int d = 0;
+  (void)hipGetDevice(&d);
+  if (!s || dev != d) {
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    dev = d;
+  }
+  return s;
+}
+
+// Grow-only scratch for one (device, stream): partial sums and the zeroed
+// per-row-block counters of the split-source reduction.
+int get_scratch(hipStream_t s, size_t partial_elems, size_t counter_elems, double **partial,
+                unsigned **counters) {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  std::lock_guard<std::mutex> lk(g_mu);
+  Scratch &sc = g_scratch[{d, s}];
+  if (sc.partial_elems < partial_elems || sc.counter_elems < counter_elems) {
+    if (hipStreamSynchronize(s) != hipSuccess) return KS_ERR_DEVICE;
+    if (sc.partial_elems < partial_elems) {
+      if (sc.partial) (void)hipFree(sc.partial);
+      sc.partial = nullptr;
+      size_t want = partial_elems + partial_elems / 4;
+      if (hipMalloc(&sc.partial, want * sizeof(double)) != hipSuccess) {
+        sc.partial_elems = 0;
+        return KS_ERR_ALLOC;
+      }
+      sc.partial_elems = want;
+    }
+    if (sc.counter_elems < counter_elems) {
+      if (sc.counters) (void)hipFree(sc.counters);
+      sc.counters = nullptr;
+      size_t want = counter_elems + counter_elems / 4 + 64;
+      if (hipMalloc(&sc.counters, want * sizeof(unsigned)) != hipSuccess) {
+        sc.counter_elems = 0;
+        return KS_ERR_ALLOC;
+      }
+      if (hipMemset(sc.counters, 0, want * sizeof(unsigned)) != hipSuccess) return KS_ERR_DEVICE;
+      sc.counter_elems = want;
+    }
+  }
+  *partial = sc.partial;
+  *counters = sc.counters;
+  return KS_OK;
+}
+
+// ------------------------------------------------------------------ kernel parameters
+
+// Polynomial power evaluation mode: the reference's powe == 2 / powe == 4 fast
+// paths (frame/dgsks_ref.c:297-319), small integer powers by repeated squaring,
+// anything else through pow().
+static void poly_mode(double powe, KerParams &P) {
+  P.powe = powe;
+  P.ipow = 0;
+  if (powe == 2.0) {
+    P.pmode = POW_SQUARE;
+  } else if (powe == 4.0) {
+    P.pmode = POW_FOURTH;
+  } else if (powe == std::floor(powe) && std::fabs(powe) <= 64.0) {
+    P.pmode = POW_INT;
+    P.ipow = (int)powe;
+  } else {
+    P.pmode = POW_GENERAL;
+  }
+}
+
+void laplace_params(int k, double *powe, double *scal) {
+  // frame/dgsks.c:445-447
+  *powe = 0.5 * (2.0 - (double)k);
+  *scal = std::tgamma(0.5 * k + 1.0) / ((double)k * (double)(k - 2) * std::pow(M_PI, 0.5 * k));
+}
+
+int make_params(const ks_t *ker, int k, KerParams &P, bool warn) {
+  P = KerParams{ker->scal, ker->cons, ker->powe, 0, POW_GENERAL};
+  switch ((int)ker->type) {
+    case KS_GAUSSIAN:
+    case KS_TANH:
+    case KS_QUARTIC:
+    case KS_MULTIQUADRATIC:
+    case KS_EPANECHNIKOV:
+      return KS_OK;
+    case KS_GAUSSIAN_VAR_BANDWIDTH:
+      if (!ker->hi || !ker->hj)
+        return raise_error(KS_ERR_NO_BANDWIDTH, false,
+                           "Error dgsks(): bandwidth vector has been initialized yet.\n");
+      return KS_OK;
+    case KS_POLYNOMIAL:
+      poly_mode(ker->powe, P);
+      return KS_OK;
+    case KS_LAPLACE:
+      if (k < 3 && warn) raise_error(KS_ERR_LAPLACE_K, false,
+                                     "Error dgsks(): laplace kernel only supports k > 2.\n");
+      laplace_params(k, &P.powe, &P.scal);
+      P.ipow = k - 2;  // scal * rsqrt(d2)^(k-2)
+      P.pmode = POW_INT;
+      return KS_OK;
+    default:
+      return raise_error(KS_ERR_INVALID_TYPE, true, "Error dgsks(): illegal kernel type\n");
+  }
+}
+
+// ------------------------------------------------------------------ launch helpers
+
+constexpr int kTargetWGs = 2048;  // ~8 workgroups (32 waves) per CU on 256 CUs
+constexpr int kMinCols = 128;     // keep each workgroup's source slice >= 128 points
+
+int enqueue_fused(const Problem &p, hipStream_t s) {
+  FusedPlan plan = plan_fused(p.m, p.n, p.k, kTargetWGs, kMinCols);
+  double *partial = nullptr;
+  unsigned *counters = nullptr;
+  if (plan.nsplit > 1) {
+    int rc = get_scratch(s, (size_t)plan.nsplit * p.m, (size_t)plan.row_blocks, &partial,
+                         &counters);
+    if (rc != KS_OK) return rc;
+  }
+  hipError_t e = launch_fused_f64(p, plan, partial, counters, s);
+  return e == hipSuccess ? KS_OK : KS_ERR_DEVICE;
+}
+
+}  // namespace gsks
+
+// ================================================================== C ABI
+
+extern "C" {
+
+int ks_last_error(void) {
+  int e = t_last_error;
+  t_last_error = KS_OK;
+  return e;
+}
+
+const char *ks_error_string(int code) { return err_string(code); }
+
+void ks_set_error_mode(int mode) {
+  g_errmode.store(mode == KS_ERRMODE_RETURN ? KS_ERRMODE_RETURN : KS_ERRMODE_EXIT);
+}
+
+int ks_get_error_mode(void) { return g_errmode.load(); }
+
+double *ks_malloc_aligned(int m, int n, int size) {
+  // frame/ks_util.c:34-55: posix_memalign, exit(1) on failure
+  void *p = nullptr;
+  size_t bytes = (size_t)(m > 0 ? m : 0) * (size_t)(n > 0 ? n : 0) * (size_t)(size > 0 ? size : 0);
+  if (bytes == 0) bytes = 64;
+  if (posix_memalign(&p, 64, bytes) != 0 || !p) {
+    raise_error(KS_ERR_ALLOC, true, "ks_malloc_aligned(): posix_memalign() failures\n");
+    return nullptr;
+  }
+  return (double *)p;
+}
+
+int dgsks_device(const ks_t *kernel, int m, int n, int k, double *u, const int *umap,
+                 const double *XA, const double *XA2, const int *amap, const double *XB,
+                 const double *XB2, const int *bmap, const double *w, const int *wmap,
+                 int precision, void *stream) {
+  if (!kernel || m < 0 || n < 0 || k < 0)
+    return raise_error(KS_ERR_INVALID_ARG, false, "dgsks_device(): invalid argument\n");
+  if (m == 0 || n == 0 || k == 0) return KS_OK;
+  if (!u || !umap || !XA || !XA2 || !amap || !XB || !XB2 || !bmap || !w || !wmap)
+    return raise_error(KS_ERR_INVALID_ARG, false, "dgsks_device(): NULL array\n");
+  int rc = ensure_device();
+  if (rc != KS_OK) return raise_error(rc, false, "dgsks_device(): no GPU available\n");
+  Problem p;
+  rc = make_params(kernel, k, p.P, true);
+  if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return rc;
+  p.m = m;
+  p.n = n;
+  p.k = k;
+  p.type = (int)kernel->type;
+  p.u = u;
+  p.umap = umap;
+  p.XA = XA;
+  p.XA2 = XA2;
+  p.amap = amap;
+  p.XB = XB;
+  p.XB2 = XB2;
+  p.bmap = bmap;
+  p.w = w;
+  p.wmap = wmap;
+  p.hi = kernel->hi;
+  p.hj = kernel->hj;
+  hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+  int r;
+  if (precision == KS_PREC_FP64) {
+    r = enqueue_fused(p, s);
+  } else {
+    r = enqueue_mixed(p, precision, s);
+  }
+  if (r != KS_OK) return raise_error(r, false, "dgsks_device(): launch failed (%s)\n", err_string(r));
+  return KS_OK;
+}
+
+void dgsks(ks_t *kernel, int m, int n, int k, double *u, int *umap, double *XA, double *XA2,
+           int *amap, double *XB, double *XB2, int *bmap, double *w, int *wmap) {
+  // frame/dgsks.c:394-397
+  if (m == 0 || n == 0 || k == 0) {
+    if (g_errmode.load() == KS_ERRMODE_EXIT) printf("dgsks(): early return\n");
+    return;
+  }
+  if (!kernel || m < 0 || n < 0 || k < 0) {
+    raise_error(KS_ERR_INVALID_ARG, false, "dgsks(): invalid argument\n");
+    return;
+  }
+  KerParams P;
+  int rc = make_params(kernel, k, P, true);
+  if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return;
+  if (kernel->type == KS_LAPLACE) {  // the reference's side effect, frame/dgsks.c:445-447
+    kernel->powe = P.powe;
+    kernel->scal = P.scal;
+  }
+  run_staged(kernel, m, n, k, u, umap, XA, XA2, amap, XB, XB2, bmap, w, wmap,
+             /*reference_algorithm=*/false);
+}
+
+void dgsks_ref(ks_t *kernel, int m, int n, int k, double *u, int *umap, double *XA,
+               double *XA2, int *amap, double *XB, double *XB2, int *bmap, double *w,
+               int *wmap) {
+  if (!kernel || m <= 0 || n <= 0 || k <= 0) return;  // frame/dgsks_ref.c has no early-return print
+  KerParams P;
+  int rc = make_params(kernel, k, P, true);
+  if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return;
+  if (kernel->type == KS_LAPLACE) {  // frame/dgsks_ref.c:146-148
+    kernel->powe = P.powe;
+    kernel->scal = P.scal;
+  }
+  run_staged(kernel, m, n, k, u, umap, XA, XA2, amap, XB, XB2, bmap, w, wmap,
+             /*reference_algorithm=*/true);
+}
+
+// frame/dgsks.c:783-826 / frame/dgsks_ref.c:537-580.  The descriptor is zero-initialised
+// and cons is set (the reference leaves kernel.cons uninitialised).
+void dgsks_wrapper(int m, int n, int k, double *u, int *umap, double *XA, double *XA2,
+                   int *alpha, double *XB, double *XB2, int *beta, double *w, int *omega,
+                   int type, double scal, double cons, double powe, double *h) {
+  ks_t kernel;
+  memset(&kernel, 0, sizeof(kernel));
+  kernel.type = (ks_type)type;
+  kernel.scal = scal;
+  kernel.cons = cons;
+  kernel.powe = powe;
+  kernel.hi = h;
+  kernel.hj = h;
+  dgsks(&kernel, m, n, k, u, umap, XA, XA2, alpha, XB, XB2, beta, w, omega);
+}
+
+void dgsks_ref_wrapper(int m, int n, int k, double *u, int *umap, double *XA, double *XA2,
+                       int *alpha, double *XB, double *XB2, int *beta, double *w, int *omega,
+                       int type, double scal, double cons, double powe, double *h) {
+  ks_t kernel;
+  memset(&kernel, 0, sizeof(kernel));
+  kernel.type = (ks_type)type;
+  kernel.scal = scal;
+  kernel.cons = cons;
+  kernel.powe = powe;
+  kernel.hi = h;
+  kernel.hj = h;
+  dgsks_ref(&kernel, m, n, k, u, umap, XA, XA2, alpha, XB, XB2, beta, w, omega);
+}
+
+}  // extern "C"

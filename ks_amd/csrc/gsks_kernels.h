@@ -1,0 +1,121 @@
+// Device-side definitions shared by the fused summation kernels: the problem
+// descriptor handed to a launch and the per-type kernel functions in FP64.
+//
+// Kernel-function semantics follow the reference's fused micro-kernels
+// (micro_kernel/x86_64/sandybridge/ks_*_int_d8x4.c): the squared distance is
+// assembled as |a|^2 - 2 a.b + |b|^2 and clamped at 0 before the kernel
+// function; Laplace replaces d^2 <= 1e-15 by 1.79e308 so that coincident
+// points contribute ~0; quartic and Epanechnikov clamp d^2 into [0, 1].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gsks {
+
+enum KType : int {
+  GAUSSIAN = 0,
+  POLYNOMIAL = 1,
+  LAPLACE = 2,
+  VAR_BANDWIDTH = 3,
+  TANH = 4,
+  QUARTIC = 5,
+  MULTIQUADRATIC = 6,
+  EPANECHNIKOV = 7
+};
+
+// How the polynomial / Laplace power is evaluated.
+enum PowMode : int {
+  POW_GENERAL = 0,  // pow(x, powe)
+  POW_SQUARE = 1,   // x^2   (reference fast path, frame/dgsks_ref.c:297-305)
+  POW_FOURTH = 2,   // (x^2)^2 (frame/dgsks_ref.c:306-319)
+  POW_INT = 3       // x^ipow by repeated squaring, |ipow| <= 64
+};
+
+struct KerParams {
+  double scal;
+  double cons;
+  double powe;
+  int ipow;   // integer exponent (POW_INT), or k-2 for Laplace
+  int pmode;  // PowMode
+};
+
+// One summation, all pointers on the device.
+struct Problem {
+  int m, n, k;
+  int type;
+  KerParams P;
+  double *u;
+  const int *umap;
+  const double *XA, *XA2;
+  const int *amap;
+  const double *XB, *XB2;
+  const int *bmap;
+  const double *w;
+  const int *wmap;
+  const double *hi, *hj;
+};
+
+// ---------------------------------------------------------------- FP64 math
+
+// x^e for integer e (|e| <= 64) by binary powering; a negative e inverts at the end.
+__device__ __forceinline__ double ipow_f64(double x, int e) {
+  int ae = e < 0 ? -e : e;
+  double r = 1.0, b = x;
+#pragma unroll 1
+  while (ae) {
+    if (ae & 1) r *= b;
+    b *= b;
+    ae >>= 1;
+  }
+  return e < 0 ? 1.0 / r : r;
+}
+
+__device__ __forceinline__ double powe_f64(double x, const KerParams &P) {
+  switch (P.pmode) {
+    case POW_SQUARE: return x * x;
+    case POW_FOURTH: { double t = x * x; return t * t; }
+    case POW_INT: return ipow_f64(x, P.ipow);
+    default: return pow(x, P.powe);
+  }
+}
+
+// Laplace: scal * d2^((2-k)/2) = scal * rsqrt(d2)^(k-2).  P.ipow holds k-2.
+__device__ __forceinline__ double laplace_f64(double d2, const KerParams &P) {
+  d2 = fmax(d2, 1e-15);
+  d2 = (d2 == 1e-15) ? 1.79e308 : d2;
+  double s = rsqrt(d2);
+  double r = (P.ipow == 1) ? s : ipow_f64(s, P.ipow);
+  return P.scal * r;
+}
+
+// Kernel value from the rank-k sum c = a.b, the norms aa = |a|^2, bb = |b|^2 and
+// (variable bandwidth) the per-point factors hi, hj.
+template <int TYPE>
+__device__ __forceinline__ double kval_f64(double c, double aa, double bb, double hi,
+                                           double hj, const KerParams &P) {
+  if constexpr (TYPE == POLYNOMIAL) {
+    return powe_f64(c * P.scal + P.cons, P);
+  } else if constexpr (TYPE == TANH) {
+    return tanh(c * P.scal + P.cons);
+  } else {
+    double d2 = fma(-2.0, c, aa) + bb;  // |a|^2 - 2 a.b + |b|^2, reference order
+    if constexpr (TYPE == GAUSSIAN) {
+      return exp(fmax(d2, 0.0) * P.scal);
+    } else if constexpr (TYPE == VAR_BANDWIDTH) {
+      // exp(-0.5 d2 hi hj); hi already carries the -0.5 (exact power-of-two scaling)
+      return exp((fmax(d2, 0.0) * hi) * hj);
+    } else if constexpr (TYPE == LAPLACE) {
+      return laplace_f64(d2, P);
+    } else if constexpr (TYPE == QUARTIC) {
+      double t = 1.0 - fmin(fmax(d2, 0.0), 1.0);
+      return (15.0 / 16.0) * t * t;
+    } else if constexpr (TYPE == MULTIQUADRATIC) {
+      return fmax(d2, 0.0) + P.cons;
+    } else {  // EPANECHNIKOV
+      return 0.75 * (1.0 - fmin(fmax(d2, 0.0), 1.0));
+    }
+  }
+}
+
+}  // namespace gsks

@@ -1,0 +1,38 @@
+// Host-side helpers shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/ks.h"
+#include "gsks_kernels.h"
+
+namespace gsks {
+
+const char *err_string(int code);
+int raise_error(int code, bool fatal, const char *fmt, ...);
+void set_last_error(int code);
+int ensure_device();
+hipStream_t thread_stream();
+int get_scratch(hipStream_t s, size_t partial_elems, size_t counter_elems, double **partial,
+                unsigned **counters);
+void laplace_params(int k, double *powe, double *scal);
+int make_params(const ks_t *ker, int k, KerParams &P, bool warn);
+
+// Fused FP64 summation of one problem (all device pointers) on stream s.
+int enqueue_fused(const Problem &p, hipStream_t s);
+// The reference algorithm (unfused: cross-term panel, elementwise kernel, GEMV).
+int enqueue_reference(const Problem &p, hipStream_t s);
+// Mixed-precision summation (KS_PREC_MIXED_*).
+int enqueue_mixed(const Problem &p, int precision, hipStream_t s);
+
+// Host entry behind dgsks()/dgsks_ref(): all-device pointers run in place,
+// all-host pointers are staged through HBM, then the call synchronises.
+void run_staged(const ks_t *kernel, int m, int n, int k, double *u, const int *umap,
+                const double *XA, const double *XA2, const int *amap, const double *XB,
+                const double *XB2, const int *bmap, const double *w, const int *wmap,
+                bool reference_algorithm);
+
+// True when p is device (or managed) memory that kernels can dereference.
+bool is_device_ptr(const void *p);
+
+}  // namespace gsks

@@ -1,0 +1,144 @@
+"""CPU checks of the boundary: libgsks.so builds/loads, exports every entry
+point include/ks.h and include/omp_dgsks_list.hpp declare, keeps the
+reference's ks_t layout and enum order, and fails loudly (error codes, no
+CPU fallback) where there is no GPU.  No GPU compute is called here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, gpu_available
+
+LIB = os.path.join(ROOT, "ks_amd", "libgsks.so")
+
+
+@pytest.fixture(scope="module")
+def L():
+    if not os.path.exists(LIB):
+        from ks_amd import build
+        build.build()
+    from ks_amd import _lib
+    return _lib.lib()
+
+
+def _declared_c_functions(header):
+    src = open(header).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_ ]*[\s\*]+([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                       src, flags=re.M)
+    return sorted({n for n in names if n not in ("if", "while", "for", "return", "sizeof")})
+
+
+def test_exports_every_declared_c_symbol(L):
+    names = _declared_c_functions(os.path.join(ROOT, "include", "ks.h"))
+    assert {"dgsks", "dgsks_ref", "ks_malloc_aligned", "dgsks_wrapper",
+            "dgsks_ref_wrapper", "dgsks_device", "dgsks_list"} <= set(names)
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True,
+                         text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+
+
+def test_exports_cpp_list_overloads():
+    out = subprocess.run(["nm", "-DC", "--defined-only", LIB], capture_output=True,
+                         text=True).stdout
+    for fn in ("omp_dgsks_list_unsymmetric", "omp_dgsks_list_symmetric",
+               "omp_dgsks_list_separated_u_unsymmetric", "omp_dgsks_list_separated_u_symmetric",
+               "omp_dgsks_list(kernel_s*"):
+        assert fn in out, fn
+
+
+def test_header_compiles_as_c_and_cpp(tmp_path):
+    """The reference's callers include ks.h from C and (inside extern "C") from C++."""
+    c = tmp_path / "t.c"
+    c.write_text('#include "ks.h"\n#include <stddef.h>\n'
+                 '_Static_assert(sizeof(ks_t) == 56, "ks_t size");\n'
+                 '_Static_assert(offsetof(ks_t, hi) == 32, "hi offset");\n'
+                 '_Static_assert(KS_EPANECHNIKOV == 7 && KS_LAPLACE == 2, "enum order");\n'
+                 'int main(void){ ks_t k; k.type = KS_GAUSSIAN; (void)k; return 0; }\n')
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", inc, "-c", str(c), "-o",
+                    str(tmp_path / "t.o")], check=True)
+    cc = tmp_path / "t.cpp"
+    cc.write_text('extern "C" {\n#include "ks.h"\n}\n#include "omp_dgsks_list.hpp"\n'
+                  'static_assert(sizeof(ks_t) == 56, "size");\nint main(){ return 0; }\n')
+    subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", "-I", inc, "-c", str(cc), "-o",
+                    str(tmp_path / "tc.o")], check=True)
+
+
+def test_kst_layout_matches_header(L):
+    from ks_amd._lib import KsT
+    assert ctypes.sizeof(KsT) == 56
+    assert KsT.hi.offset == 32 and KsT.powe.offset == 8
+
+
+def test_error_strings_and_modes(L):
+    assert L.ks_error_string(0) == b"ok"
+    assert b"illegal kernel type" in L.ks_error_string(1)
+    old = L.ks_get_error_mode()
+    L.ks_set_error_mode(1)
+    assert L.ks_get_error_mode() == 1
+    L.ks_set_error_mode(old)
+
+
+def test_malloc_aligned(L):
+    p = L.ks_malloc_aligned(10, 7, 8)
+    assert p and p % 64 == 0
+    ctypes.CDLL(None).free(ctypes.c_void_p(p))
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-GPU failure path")
+def test_no_gpu_fails_loudly(L):
+    """Without a GPU every summation entry point reports KS_ERR_DEVICE; nothing
+    is computed on the CPU."""
+    from ks_amd import _lib
+    from ks_amd._lib import KsT
+    L.ks_set_error_mode(_lib.KS_ERRMODE_RETURN)
+    kt = KsT()
+    kt.type = 0
+    kt.scal = -0.5
+    X = np.random.default_rng(0).random((8, 3))
+    X2 = (X * X).sum(1)
+    idx = np.arange(8, dtype=np.int32)
+    u = np.zeros(8)
+    w = np.ones(8)
+    ptr = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    rc = L.dgsks_device(ctypes.byref(kt), 8, 8, 3, ptr(u), ptr(idx), ptr(X), ptr(X2), ptr(idx),
+                        ptr(X), ptr(X2), ptr(idx), ptr(w), ptr(idx), 0, None)
+    assert rc == _lib.KS_ERR_DEVICE
+    L.dgsks(ctypes.byref(kt), 8, 8, 3, ptr(u), ptr(idx), ptr(X), ptr(X2), ptr(idx), ptr(X),
+            ptr(X2), ptr(idx), ptr(w), ptr(idx))
+    assert L.ks_last_error() == _lib.KS_ERR_DEVICE
+    assert np.all(u == 0)
+    from ks_amd import gsks
+    with pytest.raises(_lib.KsError):
+        gsks.dgsks(3, u, idx, X, X2, idx, X, X2, idx, w, idx, type="Gaussian", scal=-0.5)
+
+
+def test_python_argument_checks(L):
+    """python/gsks.py:19-36 asserts, raised before any library call."""
+    from ks_amd import gsks
+    X = np.zeros((10, 3))
+    X2 = np.zeros(10)
+    idx = np.arange(10, dtype=np.int32)
+    u, w = np.zeros(10), np.ones(10)
+    with pytest.raises(AssertionError):
+        gsks.dgsks(3, u, idx, X, X2, idx.astype(np.int64), X, X2, idx, w, idx, type="Gaussian")
+    with pytest.raises(AssertionError):
+        gsks.dgsks(3, u, idx, X, X2[:5], idx, X, X2, idx, w, idx, type="Gaussian")
+    with pytest.raises(AssertionError):
+        gsks.dgsks(3, u, idx, X, X2, idx, np.zeros((10, 4)), X2, idx, w, idx, type="Gaussian")
+
+
+def test_unsupported_type_prints_and_returns(L, capsys):
+    from ks_amd import gsks
+    X = np.zeros((4, 2))
+    X2 = np.zeros(4)
+    idx = np.arange(4, dtype=np.int32)
+    u = np.zeros(4)
+    gsks.dgsks(2, u, idx, X, X2, idx, X, X2, idx, np.ones(4), idx, type="Cauchy")
+    assert "not suppurted" in capsys.readouterr().out

@@ -1,0 +1,348 @@
+"""GPU parity of the FP64 fused summation (dgsks through the C ABI) against the
+oracle and the reference's golden outputs.  Tolerance: relative L2 <= 1e-12
+over the updated rows (the reference's error metric, test/test_dgsks.c:43-79;
+the bar BASELINE.json sets for the FP64 drop-in)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import golden_cases
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+@pytest.fixture(scope="module")
+def G():
+    from ks_amd import gsks
+    gsks._lib.lib()
+    return gsks
+
+
+def _opts(t, scal=0.0, cons=0.0, powe=0.0, h=None):
+    from ks_amd.gsks import TYPES
+    name = [k for k, v in TYPES.items() if v == t][0]
+    o = dict(type=name, scal=scal, cons=cons, powe=powe)
+    if h is not None:
+        o["h"] = h
+    return o
+
+
+def _dev(torch, *arrs):
+    return [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in arrs]
+
+
+# ------------------------------------------------------------------ golden fixtures
+
+@pytest.mark.parametrize("path", ["host", "device", "reference_algorithm"])
+def test_golden(G, golden, torch_cuda, path):
+    torch = torch_cuda
+    for key, d in golden_cases(golden):
+        t, m, n, k = (int(x) for x in d["meta"])
+        scal, cons, powe = d["par"][:3]
+        h = d.get("h")
+        X, X2 = d["X"], d["X2"]
+        amap, umap, bmap, wmap = d["amap"], d["umap"], d["bmap"], d["wmap"]
+        if path == "host":
+            u = d["u0"].copy()
+            G.dgsks(k, u, umap, X, X2, amap, X, X2, bmap, d["w"], wmap, **_opts(t, scal, cons, powe, h))
+        elif path == "reference_algorithm":
+            u = d["u0"].copy()
+            G.dgsks_ref(k, u, umap, X, X2, amap, X, X2, bmap, d["w"], wmap,
+                        **_opts(t, scal, cons, powe, h))
+        else:
+            du, dumap, dX, dX2, damap, dbmap, dw, dwmap = _dev(
+                torch, d["u0"], umap, X, X2, amap, bmap, d["w"], wmap)
+            dh = _dev(torch, h)[0] if h is not None else None
+            G.dgsks(k, du, dumap, dX, dX2, damap, dX, dX2, dbmap, dw, dwmap,
+                    **_opts(t, scal, cons, powe, dh))
+            torch.cuda.synchronize()
+            u = du.cpu().numpy()
+        ref = d["ref_u"]
+        rows = umap[:m]
+        err = np.linalg.norm(u[rows] - ref[rows]) / np.linalg.norm(ref[rows])
+        tol = 1e-14 if path == "reference_algorithm" else TOL
+        assert err <= tol, (key, t, path, err)
+        untouched = np.setdiff1d(np.arange(u.size), rows)
+        assert np.array_equal(u[untouched], d["u0"][untouched]), key
+
+
+# ------------------------------------------------------------------ all types x k
+
+PARAMS = {0: dict(scal=-1.3), 1: dict(scal=0.45, cons=0.25, powe=3.0), 2: dict(), 3: dict(),
+          4: dict(scal=0.3, cons=0.1), 5: dict(), 6: dict(cons=0.5), 7: dict()}
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 8, 16, 32, 64])
+def test_all_types_random_maps(G, oracle, torch_cuda, k):
+    torch = torch_cuda
+    rng = np.random.default_rng(1000 + k)
+    nx = 3000
+    m, n = 1237, 2011  # ragged: not multiples of any tile
+    X = rng.random((nx, k)) * (1.2 / np.sqrt(k))
+    X2 = oracle.sqnorm(X)
+    amap = rng.permutation(nx)[:m].astype(np.int32)
+    umap = rng.permutation(nx)[:m].astype(np.int32)
+    bmap = rng.permutation(nx)[:n].astype(np.int32)
+    wmap = rng.permutation(nx)[:n].astype(np.int32)
+    w = rng.standard_normal(nx)
+    u0 = rng.standard_normal(nx)
+    hv = rng.uniform(1.0, 3.0, nx)
+    dX, dX2, damap, dumap, dbmap, dwmap, dw, dh = _dev(torch, X, X2, amap, umap, bmap, wmap, w, hv)
+    for t in range(8):
+        if t == 2 and k < 3:
+            continue
+        prm = PARAMS[t]
+        h = hv if t == 3 else None
+        du = _dev(torch, u0)[0]
+        G.dgsks(k, du, dumap, dX, dX2, damap, dX, dX2, dbmap, dw, dwmap,
+                **_opts(t, h=dh if t == 3 else None, **prm))
+        torch.cuda.synchronize()
+        got = du.cpu().numpy()[umap]
+        ref = oracle.dgsks_rows(t, X, X2, amap, X, X2, bmap, w, wmap, u0, umap, hi=h, hj=h, **prm)
+        err = oracle.rel_l2(got, ref)
+        assert err <= TOL, (t, k, err)
+
+
+def test_distinct_tables_and_bandwidths(G, oracle, torch_cuda):
+    """XA != XB, hi != hj, umap != amap."""
+    torch = torch_cuda
+    rng = np.random.default_rng(77)
+    k, na, nb, m, n = 16, 900, 1500, 700, 1400
+    A, B = rng.random((na, k)) * 0.4, rng.random((nb, k)) * 0.4
+    A2, B2 = oracle.sqnorm(A), oracle.sqnorm(B)
+    amap = rng.integers(0, na, m).astype(np.int32)
+    umap = rng.permutation(2000)[:m].astype(np.int32)
+    bmap = rng.integers(0, nb, n).astype(np.int32)
+    wmap = rng.integers(0, 1800, n).astype(np.int32)
+    w = rng.standard_normal(1800)
+    u0 = rng.standard_normal(2000)
+    hi, hj = rng.uniform(0.5, 2, na), rng.uniform(0.5, 2, nb)
+    from ks_amd import _lib
+    kt = _lib.KsT()
+    kt.type = 3
+    dA, dA2, dB, dB2, damap, dumap, dbmap, dwmap, dw, du, dhi, dhj = _dev(
+        torch, A, A2, B, B2, amap, umap, bmap, wmap, w, u0, hi, hj)
+    kt.hi, kt.hj = dhi.data_ptr(), dhj.data_ptr()
+    p = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+    rc = _lib.lib().dgsks_device(ctypes.byref(kt), m, n, k, p(du), p(dumap), p(dA), p(dA2),
+                                 p(damap), p(dB), p(dB2), p(dbmap), p(dw), p(dwmap), 0,
+                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    ref = oracle.dgsks_rows(3, A, A2, amap, B, B2, bmap, w, wmap, u0, umap, hi=hi, hj=hj)
+    assert oracle.rel_l2(du.cpu().numpy()[umap], ref) <= TOL
+
+
+# ------------------------------------------------------------------ benchmark shapes
+
+def test_uniform_gauss16_full(G, oracle, torch_cuda):
+    """m = n = 8192, k = 16, Gaussian h = 0.5: full oracle, every row."""
+    from ks_amd import workloads as W
+    torch = torch_cuda
+    d = W.uniform_gauss16()
+    X, X2, w, idx = d["X"], d["X2"], d["w"], d["amap"]
+    u = np.zeros(X.shape[0])
+    dX, dX2, dw, didx, du = _dev(torch, X, X2, w, idx, u)
+    G.dgsks(16, du, didx, dX, dX2, didx, dX, dX2, didx, dw, didx, type="Gaussian",
+            scal=d["kernel"]["scal"])
+    torch.cuda.synchronize()
+    ref = oracle.dgsks_rows(0, X, X2, idx, X, X2, idx, w, idx, u, idx, scal=d["kernel"]["scal"])
+    err = oracle.rel_l2(du.cpu().numpy(), ref)
+    print(f"uniform_gauss16 rel L2 = {err:.3e}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("name", ["gaussian", "laplace", "polynomial", "tanh"])
+def test_nbody_k3_sampled_rows(G, oracle, torch_cuda, name):
+    """m = n = 262144, k = 3, permutation maps; oracle on 256 sampled rows."""
+    from ks_amd import workloads as W
+    torch = torch_cuda
+    d = W.nbody_k3()
+    kd = W.NBODY_KERNELS[name]
+    X, X2, w, perm = d["X"], d["X2"], d["w"], d["amap"]
+    N = X.shape[0]
+    dX, dX2, dw, dperm = _dev(torch, X, X2, w, perm)
+    du = torch.zeros(N, dtype=torch.float64, device="cuda")
+    opts = _opts(kd["type"], kd.get("scal", 0.0), kd.get("cons", 0.0), kd.get("powe", 0.0))
+    G.dgsks(3, du, dperm, dX, dX2, dperm, dX, dX2, dperm, dw, dperm, **opts)
+    torch.cuda.synchronize()
+    rows = np.random.default_rng(12).choice(N, 256, replace=False).astype(np.int32)
+    ref = oracle.dgsks_rows(kd["type"], X, X2, perm, X, X2, perm, w, perm, np.zeros(N), perm,
+                            rows=rows, scal=kd.get("scal", 0.0), cons=kd.get("cons", 0.0),
+                            powe=kd.get("powe", 0.0))
+    got = du.cpu().numpy()[perm[rows]]
+    err = oracle.rel_l2(got, ref)
+    print(f"nbody_k3 {name} rel L2 = {err:.3e}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("variable", [False, True])
+def test_mixture_k64_fp64_sampled_rows(G, oracle, torch_cuda, variable):
+    from ks_amd import workloads as W
+    torch = torch_cuda
+    d = W.mixture_k64()
+    X, X2, w, idx = d["X"], d["X2"], d["w"], d["amap"]
+    N = X.shape[0]
+    dX, dX2, dw, didx, dh = _dev(torch, X, X2, w, idx, d["hinv"])
+    du = torch.zeros(N, dtype=torch.float64, device="cuda")
+    if variable:
+        G.dgsks(64, du, didx, dX, dX2, didx, dX, dX2, didx, dw, didx, type="Variable_Bandwidth",
+                h=dh)
+    else:
+        G.dgsks(64, du, didx, dX, dX2, didx, dX, dX2, didx, dw, didx, type="Gaussian",
+                scal=d["kernel"]["scal"])
+    torch.cuda.synchronize()
+    rows = np.random.default_rng(13).choice(N, 128, replace=False).astype(np.int32)
+    t = 3 if variable else 0
+    ref = oracle.dgsks_rows(t, X, X2, idx, X, X2, idx, w, idx, np.zeros(N), idx, rows=rows,
+                            scal=d["kernel"]["scal"], hi=d["hinv"] if variable else None,
+                            hj=d["hinv"] if variable else None)
+    err = oracle.rel_l2(du.cpu().numpy()[rows], ref)
+    print(f"mixture_k64 variable={variable} rel L2 = {err:.3e}")
+    assert err <= TOL
+
+
+# ------------------------------------------------------------------ edge cases
+
+def test_edges(G, oracle, torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(5)
+    X = rng.random((50, 4))
+    X2 = oracle.sqnorm(X)
+    w = rng.standard_normal(50)
+    idx = np.arange(50, dtype=np.int32)
+    # m = 1, n = 1
+    u = np.zeros(50)
+    G.dgsks(4, u, idx[:1], X, X2, idx[3:4], X, X2, idx[7:8], w, idx[9:10], type="Gaussian",
+            scal=-0.5)
+    assert u[0] == pytest.approx(np.exp(-0.5 * np.sum((X[3] - X[7]) ** 2)) * w[9], rel=1e-14)
+    assert np.all(u[1:] == 0)
+    # empty call is a no-op
+    u2 = u.copy()
+    G.dgsks(4, u2, idx[:0], X, X2, idx[:0], X, X2, idx[:5], w, idx[:5], type="Gaussian", scal=-0.5)
+    assert np.array_equal(u, u2)
+    # accumulation: two calls add twice
+    dX, dX2, dw, didx = _dev(torch, X, X2, w, idx)
+    du = torch.zeros(50, dtype=torch.float64, device="cuda")
+    for _ in range(2):
+        G.dgsks(4, du, didx, dX, dX2, didx, dX, dX2, didx, dw, didx, type="Gaussian", scal=-0.5)
+    torch.cuda.synchronize()
+    one = oracle.dgsks_rows(0, X, X2, idx, X, X2, idx, w, idx, np.zeros(50), idx, scal=-0.5)
+    assert oracle.rel_l2(du.cpu().numpy(), 2 * one) <= TOL
+
+
+def test_deterministic(G, torch_cuda):
+    """Same inputs, same bits: the split-source reduction has a fixed order."""
+    from ks_amd import workloads as W
+    torch = torch_cuda
+    d = W.uniform_gauss16(m=4096, n=8192)
+    X, X2, w = _dev(torch, d["X"], d["X2"], d["w"])
+    ia, ib = _dev(torch, d["amap"], d["bmap"])
+    outs = []
+    for _ in range(3):
+        du = torch.zeros(X.shape[0], dtype=torch.float64, device="cuda")
+        G.dgsks(16, du, ia, X, X2, ia, X, X2, ib, w, ib, type="Gaussian", scal=-2.0)
+        outs.append(du.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+
+
+def test_error_paths_leave_u_untouched(G, torch_cuda):
+    from ks_amd import _lib
+    L = _lib.lib()
+    L.ks_set_error_mode(_lib.KS_ERRMODE_RETURN)
+    X = np.random.default_rng(1).random((10, 3))
+    X2 = (X * X).sum(1)
+    idx = np.arange(10, dtype=np.int32)
+    u = np.ones(10)
+    w = np.ones(10)
+    kt = _lib.KsT()
+    kt.type = 11  # illegal
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    L.dgsks(ctypes.byref(kt), 10, 10, 3, p(u), p(idx), p(X), p(X2), p(idx), p(X), p(X2), p(idx),
+            p(w), p(idx))
+    assert L.ks_last_error() == _lib.KS_ERR_INVALID_TYPE and np.all(u == 1)
+    kt.type = 3  # variable bandwidth without hi / hj
+    L.dgsks(ctypes.byref(kt), 10, 10, 3, p(u), p(idx), p(X), p(X2), p(idx), p(X), p(X2), p(idx),
+            p(w), p(idx))
+    assert L.ks_last_error() == _lib.KS_ERR_NO_BANDWIDTH and np.all(u == 1)
+
+
+def test_laplace_descriptor_side_effect(G):
+    from ks_amd import _lib
+    L = _lib.lib()
+    X = np.random.default_rng(2).random((20, 5))
+    X2 = (X * X).sum(1)
+    idx = np.arange(20, dtype=np.int32)
+    u, w = np.zeros(20), np.ones(20)
+    kt = _lib.KsT()
+    kt.type = 2
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    L.dgsks(ctypes.byref(kt), 20, 20, 5, p(u), p(idx), p(X), p(X2), p(idx), p(X), p(X2), p(idx),
+            p(w), p(idx))
+    assert kt.powe == -1.5
+    import math
+    assert kt.scal == pytest.approx(math.gamma(3.5) / (5 * 3 * math.pi ** 2.5), rel=1e-15)
+    assert np.all(np.isfinite(u)) and np.all(u > 0)
+
+
+def test_sqnorm_device(G, oracle, torch_cuda):
+    torch = torch_cuda
+    X = np.random.default_rng(3).random((1000, 37))
+    got = G.sqnorm(torch.from_numpy(X).cuda()).cpu().numpy()
+    assert np.array_equal(got, oracle.sqnorm(X))
+
+
+# ------------------------------------------------------------------ batched list
+
+def test_list_golden(G, golden, torch_cuda):
+    from ks_amd import _lib
+    L = _lib.lib()
+    g = golden
+    ntask = len(g["L_aoff"]) - 1
+    kernels = (_lib.KsT * ntask)()
+    for t in range(ntask):
+        kernels[t].type = 0
+        kernels[t].scal = -0.5
+    u = np.zeros_like(g["L_ref_u"])
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    X, X2, w = g["L_X"], g["L_X2"], g["L_w"]
+    rc = L.dgsks_list(ntask, kernels, X.shape[1], p(u), u.size, p(g["L_aoff"]), p(g["L_amap"]),
+                      p(g["L_amap"]), p(g["L_boff"]), p(g["L_bmap"]), p(g["L_bmap"]), p(X), p(X2),
+                      p(X), p(X2), p(w), 0)
+    assert rc == 0
+    ref = g["L_ref_u"]
+    assert np.linalg.norm(u - ref) / np.linalg.norm(ref) <= TOL
+
+
+def test_list_mixed_kernels(G, oracle, torch_cuda):
+    from ks_amd import _lib
+    from ks_amd import workloads as W
+    L = _lib.lib()
+    d = W.list_pool(k=8, ntask=64, pool=20000, lo=8, hi=600, seed=9)
+    ntask = len(d["kernels"])
+    kernels = (_lib.KsT * ntask)()
+    for t, kd in enumerate(d["kernels"]):
+        kernels[t].type = kd["type"]
+        kernels[t].scal = kd.get("scal", 0.0) or 0.0
+        kernels[t].cons = kd.get("cons", 0.0)
+        kernels[t].powe = kd.get("powe", 0.0)
+    X, X2, w = d["X"], d["X2"], d["w"]
+    u = np.zeros(X.shape[0])
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    rc = L.dgsks_list(ntask, kernels, 8, p(u), u.size, p(d["a_off"]), p(d["amap"]), p(d["amap"]),
+                      p(d["b_off"]), p(d["bmap"]), p(d["bmap"]), p(X), p(X2), p(X), p(X2), p(w), 0)
+    assert rc == 0
+    ref = np.zeros_like(u)
+    oracle.dgsks_list(d["kernels"], 8, ref, d["a_off"], d["amap"], d["amap"], d["b_off"],
+                      d["bmap"], d["bmap"], X, X2, X, X2, w)
+    assert oracle.rel_l2(u, ref) <= TOL
