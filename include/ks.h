@@ -154,9 +154,9 @@ enum {
 };
 
 /* Stream-ordered device entry point: every array is a device pointer, the work
- * is enqueued on `stream` (a hipStream_t; NULL = the per-thread default
- * stream), the call does not synchronise and does not allocate after the first
- * call on a device.  Returns a KS_* code.  kernel is read, not written (the
+ * is enqueued on `stream` (a hipStream_t; NULL = the HIP null stream), the call
+ * does not synchronise and does not allocate once its scratch has grown to the
+ * problem size (the first call of a given size on a stream).  Returns a KS_* code.  kernel is read, not written (the
  * Laplace parameters are derived internally).  hi / hj must be device
  * pointers for VAR_BANDWIDTH. */
 int dgsks_device(const ks_t *kernel, int m, int n, int k,
@@ -166,7 +166,8 @@ int dgsks_device(const ks_t *kernel, int m, int n, int k,
                  const double *w, const int *wmap, int precision, void *stream);
 
 /* Squared row norms X2[i] = sum_p X[i*k+p]^2 on the device (the norm
- * precompute of frame/omp_dgsks_list.cpp:75-81). */
+ * precompute of frame/omp_dgsks_list.cpp:75-81), enqueued on `stream`
+ * (NULL = the HIP null stream). */
 int ks_sqnorm_device(const double *X, long nx, int k, double *X2, void *stream);
 
 /* Batched summation: ntask independent summations over shared point pools,

@@ -134,6 +134,18 @@ int get_scratch(hipStream_t s, size_t partial_elems, size_t counter_elems, doubl
   return KS_OK;
 }
 
+// Device-pointer check with a small per-thread cache of recently verified buffers.
+bool device_ptr_cached(const void *p) {
+  static thread_local const void *cache[64];
+  static thread_local int next = 0;
+  for (const void *c : cache)
+    if (c == p) return true;
+  if (!is_device_ptr(p)) return false;
+  cache[next] = p;
+  next = (next + 1) & 63;
+  return true;
+}
+
 // ------------------------------------------------------------------ kernel parameters
 
 // Polynomial power evaluation mode: the reference's powe == 2 / powe == 4 fast
@@ -195,7 +207,9 @@ constexpr int kTargetWGs = 2048;  // ~8 workgroups (32 waves) per CU on 256 CUs
 constexpr int kMinCols = 128;     // keep each workgroup's source slice >= 128 points
 
 int enqueue_fused(const Problem &p, hipStream_t s) {
-  FusedPlan plan = plan_fused(p.m, p.n, p.k, kTargetWGs, kMinCols);
+  const bool mfma = mfma_f64_eligible(p.type, p.k);
+  FusedPlan plan = mfma ? plan_split(p.m, p.n, mfma_rows_per_wg(), kTargetWGs, kMinCols)
+                        : plan_fused(p.m, p.n, p.k, kTargetWGs, kMinCols);
   double *partial = nullptr;
   unsigned *counters = nullptr;
   if (plan.nsplit > 1) {
@@ -203,7 +217,8 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
                          &counters);
     if (rc != KS_OK) return rc;
   }
-  hipError_t e = launch_fused_f64(p, plan, partial, counters, s);
+  hipError_t e = mfma ? launch_fused_f64_mfma(p, plan, partial, counters, s)
+                      : launch_fused_f64(p, plan, partial, counters, s);
   return e == hipSuccess ? KS_OK : KS_ERR_DEVICE;
 }
 
@@ -269,7 +284,19 @@ int dgsks_device(const ks_t *kernel, int m, int n, int k, double *u, const int *
   p.wmap = wmap;
   p.hi = kernel->hi;
   p.hj = kernel->hj;
-  hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+  // A host pointer handed to a kernel would fault the GPU: check every array once
+  // (results are cached per thread for repeated calls with the same buffers).
+  const void *arrays[] = {u, umap, XA, XA2, amap, XB, XB2, bmap, w, wmap,
+                          p.type == KS_GAUSSIAN_VAR_BANDWIDTH ? p.hi : nullptr,
+                          p.type == KS_GAUSSIAN_VAR_BANDWIDTH ? p.hj : nullptr};
+  for (const void *a : arrays)
+    if (a && !device_ptr_cached(a))
+      return raise_error(KS_ERR_MIXED_POINTERS, false,
+                         "dgsks_device(): every array must be device memory\n");
+  if (p.type == KS_GAUSSIAN_VAR_BANDWIDTH && (!p.hi || !p.hj))
+    return raise_error(KS_ERR_NO_BANDWIDTH, false, "dgsks_device(): %s\n",
+                       err_string(KS_ERR_NO_BANDWIDTH));
+  hipStream_t s = (hipStream_t)stream;  // NULL: the HIP null stream (torch's default stream)
   int r;
   if (precision == KS_PREC_FP64) {
     r = enqueue_fused(p, s);

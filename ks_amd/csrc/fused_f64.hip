@@ -36,6 +36,27 @@ __device__ __forceinline__ cptr<T> to_const(const T *p) {
 
 constexpr int WG = 256;
 
+// Rank-k sum taken exactly as the reference's loops do (a rounded product added
+// to a rounded running sum, p = 0..k-1, no FMA).  Used where a threshold on d^2
+// makes the result discontinuous (Laplace: d^2 < 1e-15 drops the pair,
+// frame/dgsks_ref.c:343-345): with the same sum and the reference's assembly
+// order, d^2 is bit-identical to the reference's, so the same pairs are dropped.
+template <int K>
+__device__ __forceinline__ double dot_exact(const double *a, cptr<double> b) {
+#pragma clang fp contract(off)
+  double c = a[0] * b[0];
+#pragma unroll
+  for (int p = 1; p < K; ++p) c = c + a[p] * b[p];
+  return c;
+}
+
+__device__ __forceinline__ double dot_exact_rt(const double *a, cptr<double> b, int k) {
+#pragma clang fp contract(off)
+  double c = a[0] * b[0];
+  for (int p = 1; p < k; ++p) c = c + a[p] * b[p];
+  return c;
+}
+
 template <int TYPE, int K, int R>
 __global__ __launch_bounds__(WG) void fused_f64_kernel(FusedArgs A) {
   const Problem &p = A.p;
@@ -90,16 +111,26 @@ __global__ __launch_bounds__(WG) void fused_f64_kernel(FusedArgs A) {
       for (int pp = 0; pp < K; ++pp) bv[pp] = b[pp];
 #pragma unroll
       for (int q = 0; q < R; ++q) {
-        double c = a[q][0] * bv[0];
+        double c;
+        if constexpr (TYPE == LAPLACE) {
+          c = dot_exact<K>(a[q], b);
+        } else {
+          c = a[q][0] * bv[0];
 #pragma unroll
-        for (int pp = 1; pp < K; ++pp) c = fma(a[q][pp], bv[pp], c);
+          for (int pp = 1; pp < K; ++pp) c = fma(a[q][pp], bv[pp], c);
+        }
         acc[q] = fma(kval_f64<TYPE>(c, aa[q], bb, hi[q], hj, P), wj, acc[q]);
       }
     } else {
 #pragma unroll
       for (int q = 0; q < R; ++q) {
-        double c = arow[q][0] * b[0];
-        for (int pp = 1; pp < kdim; ++pp) c = fma(arow[q][pp], b[pp], c);
+        double c;
+        if constexpr (TYPE == LAPLACE) {
+          c = dot_exact_rt(arow[q], b, kdim);
+        } else {
+          c = arow[q][0] * b[0];
+          for (int pp = 1; pp < kdim; ++pp) c = fma(arow[q][pp], b[pp], c);
+        }
         acc[q] = fma(kval_f64<TYPE>(c, aa[q], bb, hi[q], hj, P), wj, acc[q]);
       }
     }
@@ -183,18 +214,24 @@ int fused_kspec(int k) {
   }
 }
 
-FusedPlan plan_fused(int m, int n, int k, int target_wgs, int min_cols) {
+FusedPlan plan_split(int m, int n, int rows_per_wg, int target_wgs, int min_cols) {
   FusedPlan pl;
-  pl.kspec = fused_kspec(k);
-  int R = fused_rows_per_lane(pl.kspec);
-  pl.rows_per_wg = WG * R;
-  pl.row_blocks = (m + pl.rows_per_wg - 1) / pl.rows_per_wg;
+  pl.kspec = 0;
+  pl.rows_per_wg = rows_per_wg;
+  pl.row_blocks = (m + rows_per_wg - 1) / rows_per_wg;
   long want = (target_wgs + pl.row_blocks - 1) / pl.row_blocks;
   long maxs = (n + min_cols - 1) / min_cols;
   if (want > maxs) want = maxs;
   if (want < 1) want = 1;
   pl.cols_per_split = (int)((n + want - 1) / want);
   pl.nsplit = (n + pl.cols_per_split - 1) / pl.cols_per_split;
+  return pl;
+}
+
+FusedPlan plan_fused(int m, int n, int k, int target_wgs, int min_cols) {
+  int ks = fused_kspec(k);
+  FusedPlan pl = plan_split(m, n, WG * fused_rows_per_lane(ks), target_wgs, min_cols);
+  pl.kspec = ks;
   return pl;
 }
 

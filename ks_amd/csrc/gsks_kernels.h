@@ -4,8 +4,9 @@
 // Kernel-function semantics follow the reference's fused micro-kernels
 // (micro_kernel/x86_64/sandybridge/ks_*_int_d8x4.c): the squared distance is
 // assembled as |a|^2 - 2 a.b + |b|^2 and clamped at 0 before the kernel
-// function; Laplace replaces d^2 <= 1e-15 by 1.79e308 so that coincident
-// points contribute ~0; quartic and Epanechnikov clamp d^2 into [0, 1].
+// function; Laplace replaces d^2 < 1e-15 by 1.79e308 so that coincident
+// points contribute ~0 (its d^2 is formed bit-identically to the reference's);
+// quartic and Epanechnikov clamp d^2 into [0, 1].
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -82,8 +83,7 @@ __device__ __forceinline__ double powe_f64(double x, const KerParams &P) {
 
 // Laplace: scal * d2^((2-k)/2) = scal * rsqrt(d2)^(k-2).  P.ipow holds k-2.
 __device__ __forceinline__ double laplace_f64(double d2, const KerParams &P) {
-  d2 = fmax(d2, 1e-15);
-  d2 = (d2 == 1e-15) ? 1.79e308 : d2;
+  d2 = (d2 < 1e-15) ? 1.79e308 : d2;  // frame/dgsks_ref.c:343-345
   double s = rsqrt(d2);
   double r = (P.ipow == 1) ? s : ipow_f64(s, P.ipow);
   return P.scal * r;

@@ -26,8 +26,17 @@ struct FusedPlan {
 
 int fused_rows_per_lane(int k);
 int fused_kspec(int k);
+// Split the sources of an m x n problem so that about target_wgs workgroups of
+// rows_per_wg target rows each run, with at least min_cols sources per split.
+FusedPlan plan_split(int m, int n, int rows_per_wg, int target_wgs, int min_cols);
 FusedPlan plan_fused(int m, int n, int k, int target_wgs, int min_cols);
 hipError_t launch_fused_f64(const Problem &p, const FusedPlan &plan, double *partial,
                             unsigned *counters, hipStream_t s);
+
+// FP64 cross term on the matrix cores (fused_f64_mfma.hip)
+bool mfma_f64_eligible(int type, int k);
+int mfma_rows_per_wg();
+hipError_t launch_fused_f64_mfma(const Problem &p, const FusedPlan &plan, double *partial,
+                                 unsigned *counters, hipStream_t s);
 
 }  // namespace gsks

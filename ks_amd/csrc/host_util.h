@@ -34,5 +34,6 @@ void run_staged(const ks_t *kernel, int m, int n, int k, double *u, const int *u
 
 // True when p is device (or managed) memory that kernels can dereference.
 bool is_device_ptr(const void *p);
+bool device_ptr_cached(const void *p);
 
 }  // namespace gsks

@@ -135,19 +135,20 @@ int list_run(int ntask, const ks_t *kernels, int k, double *u, long nu, const lo
     L.w = (const double *)up(w, (size_t)nw * 8);
     if (XA2) L.XA2 = (const double *)up(XA2, (size_t)nxa * 8);
     if (XB2) L.XB2 = same_x ? L.XA2 : (const double *)up(XB2, (size_t)nxb * 8);
-    std::map<const double *, const double *> hmap;
+    // bandwidth vectors: hi is indexed by target ids (< nxa), hj by source ids
+    // (< nxb); a vector used in both roles covers both ranges
+    std::map<const double *, long> hext;
     for (auto &td : tasks) {
       if (td.type != KS_GAUSSIAN_VAR_BANDWIDTH) continue;
-      for (const double **hp : {&td.hi, &td.hj}) {
-        long ext = (hp == &td.hi) ? nxa : nxb;
-        auto it = hmap.find(*hp);
-        if (it == hmap.end()) {
-          const double *d = (const double *)up(*hp, (size_t)std::max(nxa, nxb) * 8);
-          (void)ext;
-          it = hmap.emplace(*hp, d).first;
-        }
-        *hp = it->second;
-      }
+      hext[td.hi] = std::max(hext[td.hi], nxa);
+      hext[td.hj] = std::max(hext[td.hj], nxb);
+    }
+    std::map<const double *, const double *> hmap;
+    for (auto &e : hext) hmap[e.first] = (const double *)up(e.first, (size_t)e.second * 8);
+    for (auto &td : tasks) {
+      if (td.type != KS_GAUSSIAN_VAR_BANDWIDTH) continue;
+      td.hi = hmap[td.hi];
+      td.hj = hmap[td.hj];
     }
     if (rc != KS_OK) {
       release();

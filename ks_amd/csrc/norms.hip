@@ -26,7 +26,7 @@ extern "C" int ks_sqnorm_device(const double *X, long nx, int k, double *X2, voi
   if (nx == 0) return KS_OK;
   if (ensure_device() != KS_OK)
     return raise_error(KS_ERR_DEVICE, false, "ks_sqnorm_device(): no GPU available\n");
-  hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+  hipStream_t s = (hipStream_t)stream;
   long blocks = (nx + 255) / 256;
   hipLaunchKernelGGL(sqnorm_kernel, dim3((unsigned)blocks), dim3(256), 0, s, X, nx, k, X2);
   return hipGetLastError() == hipSuccess ? KS_OK : KS_ERR_DEVICE;

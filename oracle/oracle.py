@@ -120,7 +120,13 @@ def _i(a):
 
 
 def _nthreads(n):
-    return int(n) if n else (os.cpu_count() or 1)
+    if n:
+        return int(n)
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(avail, 16))
 
 
 def _f64(a):
