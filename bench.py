@@ -60,6 +60,11 @@ def build_workload(name):
         d = W.nbody_k3()
         d["kernel"] = dict(W.NBODY_KERNELS[kind])
         d["label"] = f"nbody_k3 {kind} m=n=262144 k=3 permutation maps"
+    elif name.startswith("list_pool"):
+        k = int(name.split("_k")[1]) if "_k" in name else 3
+        d = W.list_pool(k=k)
+        d["label"] = (f"list_pool 4096 tasks m,n log-U[64,8192] k={k} pool=1048576 "
+                      "kernels {gaussian h=0.2, laplace, polynomial deg 2}")
     else:
         raise SystemExit(f"unknown workload {name}")
     return d
@@ -120,6 +125,66 @@ def cpu_baseline(d, budget_s=8.0, max_threads=16):
             "one_core_value": rate1}
 
 
+def bench_list(args, d, L):
+    """Batched workload: one dgsks_list call (device-resident pool) per step."""
+    import torch
+    from ks_amd import _lib
+    X = torch.from_numpy(d["X"]).cuda()
+    X2 = torch.from_numpy(d["X2"]).cuda()
+    w = torch.from_numpy(d["w"]).cuda()
+    amap = torch.from_numpy(d["amap"]).cuda()
+    bmap = torch.from_numpy(d["bmap"]).cuda()
+    u = torch.zeros(X.shape[0], dtype=torch.float64, device="cuda")
+    ntask = len(d["kernels"])
+    kernels = (_lib.KsT * ntask)()
+    for t, kd in enumerate(d["kernels"]):
+        kernels[t].type = kd["type"]
+        kernels[t].scal = kd.get("scal") or 0.0
+        kernels[t].cons = kd.get("cons", 0.0)
+        kernels[t].powe = kd.get("powe", 0.0)
+    a_off, b_off = d["a_off"], d["b_off"]
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    H = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    k = d["k"]
+    prec = {"auto": 0, "fp64": 0, "mixed_fp32": 1, "mixed_bf16x3": 2}[args.precision]
+
+    def step():
+        rc = L.dgsks_list(ntask, kernels, k, P(u), u.numel(), H(a_off), P(amap), P(amap),
+                          H(b_off), P(bmap), P(bmap), P(X), P(X2), P(X), P(X2), P(w), prec)
+        if rc != 0:
+            raise RuntimeError(f"dgsks_list failed: {_lib.error_string(rc)}")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        step()  # dgsks_list returns when u holds the result
+        times.append(time.perf_counter() - t0)
+    ms_per_step = float(np.mean(times)) * 1e3
+    evals = float(np.sum(np.diff(a_off) * np.diff(b_off)))
+    types = [kd["type"] for kd in d["kernels"]]
+    per = float(np.sum([(2 * k + FLOP_C[t]) * float(ma) * float(nb) for t, ma, nb in
+                        zip(types, np.diff(a_off), np.diff(b_off))]))
+    achieved = per / (ms_per_step / 1e3) / 1e12
+    out = {
+        "metric": METRIC, "value": evals / (ms_per_step / 1e3), "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == 0 else "mixed", "data": "synthetic (seeded numpy)",
+        "config": {"workload": d["label"], "ntask": ntask, "evals": evals, "k": k},
+        "roofline": {"bound": "mfma", "pipe": "fp64", "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "note": "wall time of the whole call (tile layout upload, persistent "
+                             "kernel, deterministic merge)"},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +221,8 @@ def main():
 
     L = _lib.lib()
     d = build_workload(args.workload)
+    if "kernels" in d:
+        return bench_list(args, d, L)
     kd = d["kernel"]
     X = torch.from_numpy(d["X"]).cuda()
     X2 = torch.from_numpy(d["X2"]).cuda()
@@ -172,7 +239,6 @@ def main():
     kt.powe = kd.get("powe", 0.0)
     if kd["type"] == 3:
         kt.hi = kt.hj = hdev.data_ptr()
-    prec = {"fp64": 0, "mixed_fp32": 1, "mixed_bf16x3": 2}[args.precision]
     stream = torch.cuda.current_stream()
     sp = ctypes.c_void_p(stream.cuda_stream)
     P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731

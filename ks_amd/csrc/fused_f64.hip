@@ -243,6 +243,7 @@ hipError_t launch_fused_f64(const Problem &p, const FusedPlan &plan, double *par
   A.nsplit = plan.nsplit;
   A.partial = partial;
   A.counters = counters;
+  A.vec2 = 0;
   switch (p.type) {
     case GAUSSIAN: return launch_type<GAUSSIAN>(A, plan, s);
     case POLYNOMIAL: return launch_type<POLYNOMIAL>(A, plan, s);

@@ -213,6 +213,7 @@ hipError_t launch_fused_f64_mfma(const Problem &p, const FusedPlan &plan, double
   A.nsplit = plan.nsplit;
   A.partial = partial;
   A.counters = counters;
+  A.vec2 = 0;
   switch (p.type) {
     case GAUSSIAN: return go_type<GAUSSIAN>(A, plan.row_blocks, s);
     case POLYNOMIAL: return go_type<POLYNOMIAL>(A, plan.row_blocks, s);

@@ -14,6 +14,7 @@ struct FusedArgs {
   int nsplit;
   double *partial;     // [nsplit][m] partial sums (nsplit > 1)
   unsigned *counters;  // [row_blocks], zero between launches
+  int vec2;            // point rows are 16-byte aligned (k even, aligned tables)
 };
 
 struct FusedPlan {

@@ -1,37 +1,456 @@
-// Batched summation: every task of a list on one stream.
+// Batched summation (dgsks_list): thousands of independent summations of
+// uneven m, n over shared point pools in ONE persistent launch.
+//
+// Work unit = tile: 64 target rows of one task x one slice of at most `bc`
+// of that task's sources.  The host lays the tiles out once per call, largest
+// first; every wave of the persistent grid pulls the next tile index from a
+// device-scope counter (atomicAdd) and runs it, so uneven tasks balance
+// across all 1024 SIMDs without host involvement.  Per tile:
+//   * k <= 8, Laplace at any k, k > 64: lane = target row, rows in VGPRs,
+//     sources as wave-uniform scalar loads (as fused_f64_kernel);
+//   * 9 <= k <= 64 (other types): 4 blocks of 16 rows on the FP64 matrix
+//     cores (as fused_f64_mfma_kernel).
+// Each tile writes 64 row partials; the last tile of a (task, row block) to
+// finish (device-scope counter) adds its partials in slice order into
+// R[a_off[t] + i].  The last wave to leave resets the tile counter, so the
+// launch leaves its scratch ready for the next one (graph replay friendly).
+//
+// Merging R into u: tasks may share target points, and the reference adds
+// every task's contribution (frame/omp_dgsks_list.cpp:407-414).  To keep the
+// result independent of scheduling, the target ids umap are stably
+// radix-sorted (hipCUB) together with their positions, and one thread per
+// distinct target adds its R entries in position order.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "gsks_kernels.h"
 #include "host_util.h"
 #include "list_launch.h"
 
 namespace gsks {
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+
+template <typename T>
+__device__ __forceinline__ cptr<T> to_const(const T *p) {
+  return (cptr<T>)(p);
+}
+
+struct DevTask {
+  int type;
+  int m, n;
+  int pad;
+  long a0, b0;
+  KerParams P;
+  const double *hi, *hj;
+};
+
+struct RowBlock {
+  int task;
+  int row0;       // first row of the block within the task
+  int nsplit;     // source slices of this block
+  int tile_base;  // index of its first tile
+};
+
+struct ListArgs {
+  int ntiles, k, bc;
+  const int2 *tiles;  // {row block, slice}
+  const RowBlock *rbs;
+  const DevTask *tasks;
+  const int *amap, *bmap, *wmap;
+  const double *XA, *XA2, *XB, *XB2, *w;
+  double *partial;     // [ntiles][64]
+  unsigned *rbcount;   // [nrowblocks], zero between launches
+  unsigned *head;      // tile counter
+  unsigned *done;      // waves that have left
+  double *R;           // [total_a] per-(task, row) totals
+};
+
+// ------------------------------------------------------------------ per-tile work
+
+template <int TYPE>
+__device__ __forceinline__ double dot_ref_order(const double *a, cptr<double> b, int k) {
+#pragma clang fp contract(off)
+  double c = a[0] * b[0];
+  for (int p = 1; p < k; ++p) c = c + a[p] * b[p];
+  return c;
+}
+
+// lane = row; sources [c0, c1) of task T through wave-uniform scalar loads
+template <int TYPE, int K>
+__device__ double tile_valu(const ListArgs &L, const DevTask &T, int pos, int c0, int c1) {
+  const int kdim = K > 0 ? K : L.k;
+  const int ai = L.amap[T.a0 + (pos < T.m ? pos : T.m - 1)];
+  const double *arow = L.XA + (long)ai * kdim;
+  constexpr int KR = K > 0 ? K : 1;
+  double a[KR];
+  if constexpr (K > 0) {
+#pragma unroll
+    for (int p = 0; p < K; ++p) a[p] = arow[p];
+  }
+  const double aa = L.XA2[ai];
+  const double hi = (TYPE == VAR_BANDWIDTH) ? -0.5 * T.hi[ai] : 0.0;
+  cptr<int> bmap = to_const(L.bmap + T.b0);
+  cptr<int> wmap = to_const(L.wmap + T.b0);
+  cptr<double> XB = to_const(L.XB);
+  cptr<double> XB2 = to_const(L.XB2);
+  cptr<double> W = to_const(L.w);
+  cptr<double> HJ = to_const(T.hj);
+  const KerParams P = T.P;
+  double acc = 0.0;
+  for (int j = c0; j < c1; ++j) {
+    const int bj = bmap[j];
+    const double wj = W[wmap[j]];
+    const double bb = XB2[bj];
+    const double hj = (TYPE == VAR_BANDWIDTH) ? HJ[bj] : 0.0;
+    cptr<double> b = XB + (long)bj * kdim;
+    double c;
+    if constexpr (TYPE == LAPLACE) {
+      if constexpr (K > 0) {
+#pragma clang fp contract(off)
+        c = a[0] * b[0];
+#pragma unroll
+        for (int p = 1; p < K; ++p) c = c + a[p] * b[p];
+      } else {
+        c = dot_ref_order<TYPE>(arow, b, kdim);
+      }
+    } else if constexpr (K > 0) {
+      c = a[0] * b[0];
+#pragma unroll
+      for (int p = 1; p < K; ++p) c = fma(a[p], b[p], c);
+    } else {
+      c = arow[0] * b[0];
+      for (int p = 1; p < kdim; ++p) c = fma(arow[p], b[p], c);
+    }
+    acc = fma(kval_f64<TYPE>(c, aa, bb, hi, hj, P), wj, acc);
+  }
+  return acc;
+}
+
+// 64 rows as 4 blocks of 16 on the FP64 matrix cores; returns the sum of the
+// lane's row (lane = row) through the wave's LDS slot `red` (64 doubles).
+template <int TYPE, int KC>
+__device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c0, int c1,
+                            double *red) {
+  const int k = L.k;
+  const int lane = threadIdx.x & 63, p16 = lane & 15, g = lane >> 4;
+  double af[4][KC][4], aa[4][4], hi[4][4], racc[4][4];
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb) {
+    int r = row0 + rb * 16 + p16;
+    int ai = L.amap[T.a0 + (r < T.m ? r : T.m - 1)];
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        int idx = c * 16 + 4 * g + t;
+        af[rb][c][t] = idx < k ? L.XA[(long)ai * k + idx] : 0.0;
+      }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      int ro = row0 + rb * 16 + g + 4 * v;
+      int ao = L.amap[T.a0 + (ro < T.m ? ro : T.m - 1)];
+      aa[rb][v] = L.XA2[ao];
+      hi[rb][v] = (TYPE == VAR_BANDWIDTH) ? -0.5 * T.hi[ao] : 0.0;
+      racc[rb][v] = 0.0;
+    }
+  }
+  const KerParams P = T.P;
+  for (int jb = c0; jb < c1; jb += 16) {
+    const int col = jb + p16;
+    const bool ok = col < c1;
+    const int bj = L.bmap[T.b0 + (ok ? col : c1 - 1)];
+    double bf[KC][4];
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        int idx = c * 16 + 4 * g + t;
+        bf[c][t] = idx < k ? L.XB[(long)bj * k + idx] : 0.0;
+      }
+    const double bb = L.XB2[bj];
+    const double wj = ok ? L.w[L.wmap[T.b0 + col]] : 0.0;
+    const double hj = (TYPE == VAR_BANDWIDTH) ? T.hj[bj] : 0.0;
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int c = 0; c < KC; ++c)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[rb][c][t], bf[c][t], acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        racc[rb][v] = fma(kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P), wj, racc[rb][v]);
+    }
+  }
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double s = racc[rb][v];
+      s += __shfl_xor(s, 8, 16);
+      s += __shfl_xor(s, 4, 16);
+      s += __shfl_xor(s, 2, 16);
+      s += __shfl_xor(s, 1, 16);
+      if (p16 == v) red[rb * 16 + g + 4 * v] = s;
+    }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  double mine = red[lane];
+  __builtin_amdgcn_wave_barrier();
+  return mine;
+}
+
+template <int TYPE, int K>
+__device__ double run_tile(const ListArgs &L, const DevTask &T, int row0, int c0, int c1,
+                           double *red) {
+  if constexpr (TYPE != LAPLACE && K == 0) {
+    if (L.k >= 9 && L.k <= 64) {
+      switch ((L.k + 15) / 16) {
+        case 1: return tile_mfma<TYPE, 1>(L, T, row0, c0, c1, red);
+        case 2: return tile_mfma<TYPE, 2>(L, T, row0, c0, c1, red);
+        case 3: return tile_mfma<TYPE, 3>(L, T, row0, c0, c1, red);
+        default: return tile_mfma<TYPE, 4>(L, T, row0, c0, c1, red);
+      }
+    }
+  }
+  return tile_valu<TYPE, K>(L, T, row0 + (threadIdx.x & 63), c0, c1);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void list_kernel(ListArgs L) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  __shared__ double s_red[4][64];
+  double *red = s_red[wave];
+  while (true) {
+    unsigned t = 0;
+    if (lane == 0) t = atomicAdd(L.head, 1u);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+    if (t >= (unsigned)L.ntiles) break;
+    const int2 tl = L.tiles[t];
+    const RowBlock rb = L.rbs[tl.x];
+    const DevTask T = L.tasks[rb.task];
+    const int c0 = tl.y * L.bc;
+    const int c1 = min(T.n, c0 + L.bc);
+    double s;
+    switch (T.type) {
+      case GAUSSIAN: s = run_tile<GAUSSIAN, K>(L, T, rb.row0, c0, c1, red); break;
+      case POLYNOMIAL: s = run_tile<POLYNOMIAL, K>(L, T, rb.row0, c0, c1, red); break;
+      case LAPLACE: s = run_tile<LAPLACE, K>(L, T, rb.row0, c0, c1, red); break;
+      case VAR_BANDWIDTH: s = run_tile<VAR_BANDWIDTH, K>(L, T, rb.row0, c0, c1, red); break;
+      case TANH: s = run_tile<TANH, K>(L, T, rb.row0, c0, c1, red); break;
+      case QUARTIC: s = run_tile<QUARTIC, K>(L, T, rb.row0, c0, c1, red); break;
+      case MULTIQUADRATIC: s = run_tile<MULTIQUADRATIC, K>(L, T, rb.row0, c0, c1, red); break;
+      default: s = run_tile<EPANECHNIKOV, K>(L, T, rb.row0, c0, c1, red); break;
+    }
+    const int pos = rb.row0 + lane;
+    const bool valid = pos < T.m;
+    if (rb.nsplit == 1) {
+      if (valid) L.R[T.a0 + pos] = s;
+      continue;
+    }
+    L.partial[(long)t * 64 + lane] = s;
+    __threadfence();
+    unsigned prev = 0;
+    if (lane == 0) prev = atomicAdd(&L.rbcount[tl.x], 1u);
+    prev = __builtin_amdgcn_readfirstlane(__shfl(prev, 0));
+    if (prev != (unsigned)(rb.nsplit - 1)) continue;
+    __threadfence();
+    double tot = 0.0;
+    for (int sp = 0; sp < rb.nsplit; ++sp) tot += L.partial[(long)(rb.tile_base + sp) * 64 + lane];
+    if (valid) L.R[T.a0 + pos] = tot;
+    if (lane == 0) L.rbcount[tl.x] = 0u;
+  }
+  // the last wave to leave resets the tile counter for the next launch
+  if (lane == 0) {
+    __threadfence();
+    unsigned d = atomicAdd(L.done, 1u);
+    if (d == gridDim.x * 4u - 1u) {
+      *L.head = 0u;
+      *L.done = 0u;
+    }
+  }
+}
+
+__global__ void iota_kernel(int *v, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (int)i;
+}
+
+// one thread per sorted position; the first position of each target adds the
+// target's entries in position order
+__global__ void merge_kernel(const int *keys, const int *pos, const double *R, long n, double *u) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int key = keys[s];
+  if (s > 0 && keys[s - 1] == key) return;
+  double acc = 0.0;
+  for (long q = s; q < n && keys[q] == key; ++q) acc += R[pos[q]];
+  u[key] += acc;
+}
+
+// ------------------------------------------------------------------ host side
+
+template <int K>
+static void launch_list(const ListArgs &A, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(list_kernel<K>, dim3(grid), dim3(256), 0, s, A);
+}
+
+struct ListScratch {
+  char *buf = nullptr;
+  size_t bytes = 0;
+  int dev = -1;
+  hipStream_t stream = nullptr;
+};
+
+static int list_tile_cols(int k) {
+  (void)k;
+  return 1024;
+}
+
 int list_enqueue(const std::vector<TaskDesc> &tasks, int k, const ListArrays &L, int precision,
                  hipStream_t s) {
-  for (const TaskDesc &t : tasks) {
-    if (t.m == 0 || t.n == 0) continue;
-    Problem p;
-    p.m = t.m;
-    p.n = t.n;
-    p.k = k;
-    p.type = t.type;
-    p.P = t.P;
-    p.u = L.u;
-    p.umap = L.umap + t.a0;
-    p.XA = L.XA;
-    p.XA2 = L.XA2;
-    p.amap = L.amap + t.a0;
-    p.XB = L.XB;
-    p.XB2 = L.XB2;
-    p.bmap = L.bmap + t.b0;
-    p.w = L.w;
-    p.wmap = L.wmap + t.b0;
-    p.hi = t.hi;
-    p.hj = t.hj;
-    int rc = precision == KS_PREC_FP64 ? enqueue_fused(p, s) : enqueue_mixed(p, precision, s);
-    if (rc != KS_OK) return rc;
+  if (precision != KS_PREC_FP64) {
+    // mixed-precision lists run task by task through the mixed kernels
+    for (const TaskDesc &t : tasks) {
+      if (t.m == 0 || t.n == 0) continue;
+      Problem p;
+      p.m = t.m; p.n = t.n; p.k = k; p.type = t.type; p.P = t.P;
+      p.u = L.u; p.umap = L.umap + t.a0; p.XA = L.XA; p.XA2 = L.XA2; p.amap = L.amap + t.a0;
+      p.XB = L.XB; p.XB2 = L.XB2; p.bmap = L.bmap + t.b0; p.w = L.w; p.wmap = L.wmap + t.b0;
+      p.hi = t.hi; p.hj = t.hj;
+      int rc = enqueue_mixed(p, precision, s);
+      if (rc != KS_OK) return rc;
+    }
+    return KS_OK;
   }
-  return KS_OK;
+  const int bc = list_tile_cols(k);
+  // ---- tile layout, largest tiles first
+  std::vector<DevTask> dt(tasks.size());
+  std::vector<RowBlock> rbs;
+  std::vector<int2> tiles;
+  long total_a = 0;
+  for (size_t t = 0; t < tasks.size(); ++t) {
+    const TaskDesc &td = tasks[t];
+    dt[t] = DevTask{td.type, td.m, td.n, 0, td.a0, td.b0, td.P, td.hi, td.hj};
+    total_a = std::max(total_a, td.a0 + td.m);
+    if (td.m == 0 || td.n == 0) continue;
+    const int nsplit = (td.n + bc - 1) / bc;
+    for (int r0 = 0; r0 < td.m; r0 += 64) rbs.push_back(RowBlock{(int)t, r0, nsplit, 0});
+  }
+  if (rbs.empty()) return KS_OK;
+  // order row blocks by work (columns per tile x rows) descending, then lay tiles out
+  std::vector<int> order(rbs.size());
+  std::iota(order.begin(), order.end(), 0);
+  auto work = [&](const RowBlock &r) {
+    const DevTask &T = dt[r.task];
+    long rows = std::min(64, T.m - r.row0);
+    long cols = std::min(bc, T.n);
+    return rows * cols;
+  };
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return work(rbs[x]) > work(rbs[y]); });
+  std::vector<RowBlock> rbs_sorted(rbs.size());
+  for (size_t i = 0; i < order.size(); ++i) rbs_sorted[i] = rbs[order[i]];
+  rbs.swap(rbs_sorted);
+  for (size_t i = 0; i < rbs.size(); ++i) {
+    rbs[i].tile_base = (int)tiles.size();
+    for (int sp = 0; sp < rbs[i].nsplit; ++sp) tiles.push_back(make_int2((int)i, sp));
+  }
+  const long ntiles = (long)tiles.size();
+  if (ntiles >= (1l << 31)) return KS_ERR_INVALID_ARG;
+
+  // ---- scratch: descriptors, partials, counters, R, sort buffers
+  size_t sort_bytes = 0;
+  const int nbits = 32;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int *)nullptr,
+                                           (int *)nullptr, (const int *)nullptr, (int *)nullptr,
+                                           (int)total_a, 0, nbits, s);
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t off_tiles = 0;
+  size_t off_rbs = off_tiles + al(ntiles * sizeof(int2));
+  size_t off_tasks = off_rbs + al(rbs.size() * sizeof(RowBlock));
+  size_t off_part = off_tasks + al(dt.size() * sizeof(DevTask));
+  size_t off_cnt = off_part + al((size_t)ntiles * 64 * sizeof(double));
+  size_t off_head = off_cnt + al(rbs.size() * sizeof(unsigned));
+  size_t off_R = off_head + 256;
+  size_t off_iota = off_R + al((size_t)total_a * sizeof(double));
+  size_t off_kout = off_iota + al((size_t)total_a * sizeof(int));
+  size_t off_vout = off_kout + al((size_t)total_a * sizeof(int));
+  size_t off_sort = off_vout + al((size_t)total_a * sizeof(int));
+  size_t bytes = off_sort + al(sort_bytes);
+  char *buf = nullptr;
+  if (hipMallocAsync((void **)&buf, bytes, s) != hipSuccess) {
+    (void)hipGetLastError();
+    return KS_ERR_ALLOC;
+  }
+  int rc = KS_OK;
+  if (hipMemcpyAsync(buf + off_tiles, tiles.data(), ntiles * sizeof(int2), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(buf + off_rbs, rbs.data(), rbs.size() * sizeof(RowBlock), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(buf + off_tasks, dt.data(), dt.size() * sizeof(DevTask), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemsetAsync(buf + off_cnt, 0, off_R - off_cnt, s) != hipSuccess ||
+      hipMemsetAsync(buf + off_R, 0, (size_t)total_a * sizeof(double), s) != hipSuccess)
+    rc = KS_ERR_DEVICE;
+  ListArgs A;
+  A.ntiles = (int)ntiles;
+  A.k = k;
+  A.bc = bc;
+  A.tiles = (const int2 *)(buf + off_tiles);
+  A.rbs = (const RowBlock *)(buf + off_rbs);
+  A.tasks = (const DevTask *)(buf + off_tasks);
+  A.amap = L.amap;
+  A.bmap = L.bmap;
+  A.wmap = L.wmap;
+  A.XA = L.XA;
+  A.XA2 = L.XA2;
+  A.XB = L.XB;
+  A.XB2 = L.XB2;
+  A.w = L.w;
+  A.partial = (double *)(buf + off_part);
+  A.rbcount = (unsigned *)(buf + off_cnt);
+  A.head = (unsigned *)(buf + off_head);
+  A.done = A.head + 1;
+  A.R = (double *)(buf + off_R);
+  if (rc == KS_OK) {
+    // persistent grid: enough 256-thread workgroups to hold ~4 waves per SIMD
+    int dev = 0, ncu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    long grid = std::min<long>((long)ncu * 4, (ntiles + 3) / 4);
+    if (grid < 1) grid = 1;
+    switch (k) {
+      case 3: launch_list<3>(A, (int)grid, s); break;
+      case 8: launch_list<8>(A, (int)grid, s); break;
+      default: launch_list<0>(A, (int)grid, s); break;
+    }
+    if (hipGetLastError() != hipSuccess) rc = KS_ERR_DEVICE;
+  }
+  if (rc == KS_OK) {
+    int *iota = (int *)(buf + off_iota);
+    int *kout = (int *)(buf + off_kout);
+    int *vout = (int *)(buf + off_vout);
+    long nb = (total_a + 255) / 256;
+    hipLaunchKernelGGL(iota_kernel, dim3((unsigned)nb), dim3(256), 0, s, iota, total_a);
+    size_t tb = sort_bytes;
+    if (hipcub::DeviceRadixSort::SortPairs(buf + off_sort, tb, L.umap, kout, (const int *)iota,
+                                           vout, (int)total_a, 0, nbits, s) != hipSuccess)
+      rc = KS_ERR_DEVICE;
+    hipLaunchKernelGGL(merge_kernel, dim3((unsigned)nb), dim3(256), 0, s, (const int *)kout,
+                       (const int *)vout, (const double *)A.R, total_a, L.u);
+    if (hipGetLastError() != hipSuccess) rc = KS_ERR_DEVICE;
+  }
+  (void)hipFreeAsync(buf, s);
+  return rc;
 }
 
 }  // namespace gsks

@@ -33,6 +33,8 @@ int list_run(int ntask, const ks_t *kernels, int k, double *u, long nu, const lo
     if (a_off[t + 1] < a_off[t] || b_off[t + 1] < b_off[t] || a_off[t] < 0 || b_off[t] < 0)
       return raise_error(KS_ERR_LIST, false, "%s(): offsets must be non-decreasing\n", name);
   }
+  if (a_off[0] != 0)
+    return raise_error(KS_ERR_LIST, false, "%s(): a_off[0] must be 0\n", name);
   const long total_a = a_off[ntask], total_b = b_off[ntask];
   if (total_a == 0 || total_b == 0) return KS_OK;
   if (!u || !umap || !amap || !bmap || !wmap || !XA || !XB || !w)

@@ -40,7 +40,7 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 constexpr float kFarT = 16.0f;          // far: estimated x < -16 (natural-log units)
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr int XWG = 256;                // 4 waves
-constexpr int kQueue = 1024;            // per-wave near-pair ring (power of two)
+constexpr int kQueue = 512;             // per-wave near-pair ring (power of two)
 
 // ------------------------------------------------------------------ pre-pass
 
@@ -144,22 +144,20 @@ __device__ __forceinline__ f4 cross_block(const Frag<PREC, NC> &a, const Frag<PR
   return acc;
 }
 
-// FP64 recompute of one near pair; returns K*w.
+// FP64 kernel value times weight of one near pair from its exact rank-k sum c.
 template <int TYPE>
-__device__ __forceinline__ double near_pair(const Problem &p, int row, int col) {
+__device__ __forceinline__ double near_value(const Problem &p, int row, int col, double c) {
   const int ai = p.amap[row], bj = p.bmap[col];
-  const double *a = p.XA + (long)ai * p.k;
-  const double *b = p.XB + (long)bj * p.k;
-  double c = 0.0;
-  for (int q = 0; q < p.k; ++q) c = fma(a[q], b[q], c);
   double d2 = fmax(fma(-2.0, c, p.XA2[ai]) + p.XB2[bj], 0.0);
   double x = (TYPE == GAUSSIAN) ? d2 * p.P.scal : (d2 * (-0.5 * p.hi[ai])) * p.hj[bj];
   return exp(x) * p.w[p.wmap[col]];
 }
 
-// Row of the 16 x 16 result block held in accumulator slot v by lane group g
-// (v_mfma_*_16x16x* on gfx950: rows g + 4v of column lane%16).
-__device__ __forceinline__ int out_row(int g, int v) { return g + 4 * v; }
+// Row of the 16 x 16 result block held in accumulator slot v by lane group g.
+// Measured on gfx950: v_mfma_f32_16x16x32_bf16 and v_mfma_f32_16x16x4f32 put
+// rows 4g + v of column lane%16 in lane g*16 + col (the FP64 16x16x4 shape
+// uses g + 4v instead).
+__device__ __forceinline__ int out_row(int g, int v) { return 4 * g + v; }
 
 constexpr int kFlush = 8;  // far-pair FP32 partials go to FP64 every 8 source blocks
 
@@ -179,14 +177,24 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
 
   __shared__ int s_queue[4][kQueue];
   __shared__ double s_near[4][WROWS];
+  // the wave's target rows in FP64 for the near-pair recompute: each is reused
+  // by every near pair of its row, so only the source side is gathered later
+  // (row stride 32*NC + 2 doubles keeps the 16 lanes' 32-byte reads apart)
+  constexpr int KS = 32 * NC + 2;
+  __shared__ double s_rows[4][WROWS * KS];
   int *queue = s_queue[wave];
   double *nacc = s_near[wave];
+  double *arows = s_rows[wave];
   for (int q = lane; q < WROWS; q += 64) nacc[q] = 0.0;
+  for (int e = lane; e < WROWS * 32 * NC; e += 64) {
+    const int rl = e / (32 * NC), q = e - rl * (32 * NC);
+    const int r = row0 + rl;
+    arows[rl * KS + q] = q < p.k ? p.XA[(long)p.amap[r < m ? r : m - 1] * p.k + q] : 0.0;
+  }
 
   Frag<PREC, NC> af[RB];
   float ra_s[RB][4], ra_h[RB][4], facc[RB][4];
   double racc[RB][4];
-  unsigned rowok = 0;  // bit rb*4+v: the row of slot (rb, v) exists (< m)
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) {
     int r = row0 + rb * 16 + p16;
@@ -194,8 +202,9 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       int ro = row0 + rb * 16 + out_row(g, v);
-      rowok |= (ro < m ? 1u : 0u) << (rb * 4 + v);
       float2 t = pk.ra[ro < m ? ro : m - 1];
+      // a row past m can never be near: its estimated exponent is -inf
+      if (ro >= m) t.x = (TYPE == GAUSSIAN) ? -INFINITY : INFINITY;
       ra_s[rb][v] = t.x;
       ra_h[rb][v] = t.y;
       racc[rb][v] = 0.0;
@@ -213,13 +222,68 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
     bool ok = col < j1;
     load_frag_packed<PREC, NC>(pk.Bp + (long)(ok ? col : j1 - 1) * NC * 8, g, f);
     c4 = pk.cb[ok ? col : j1 - 1];
+    if (!ok) {  // a column past the slice: exponent -inf, weight 0
+      c4.x = (TYPE == GAUSSIAN) ? -INFINITY : INFINITY;
+      c4.z = 0.0f;
+    }
   };
+  // Near pairs, up to 64 per call: 16 rounds of 4 pairs x 16 lanes.  In round r
+  // the 16 lanes of group s sum pair 4r + s, each over 2*NC consecutive
+  // coordinates (contiguous 16-byte loads of both rows), reduce across the 16
+  // lanes, and lane 16s + r keeps the sum; afterwards every lane holds one
+  // pair's exact rank-k sum and evaluates its kernel value in FP64.
   auto drain = [&](unsigned count) {
-    // one near pair per lane: FP64 recompute, added to the wave's row sums
-    if (lane < (int)count) {
-      int e = queue[(head + lane) & (kQueue - 1)];
-      int rl = e & 31, col = e >> 5;
-      atomicAdd(&nacc[rl], near_pair<TYPE>(p, row0 + rl, col));
+    const int grp = lane >> 4, sub = lane & 15;
+    constexpr int CPL = 2 * NC;  // coordinates per lane (k <= 32*NC)
+    double mine_c = 0.0;
+    int mine_e = -1;
+    // 4 rounds at a time: all their loads are issued before the first FMA
+#pragma unroll 1
+    for (int r0 = 0; r0 < 16; r0 += 4) {
+      if (4 * r0 >= (int)count) break;
+      double2 av[4][CPL / 2], bv[4][CPL / 2];
+      int ev[4];
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int idx = 4 * (r0 + rr) + grp;
+        const int e = idx < (int)count ? queue[(head + idx) & (kQueue - 1)] : -1;
+        ev[rr] = e;
+        const int rl = (e >= 0 ? e : 0) & 31, col = e >= 0 ? (e >> 5) : j0;
+        const double *b = p.XB + (long)p.bmap[col] * p.k + sub * CPL;
+        const double *a = arows + rl * KS + sub * CPL;  // zero-padded past k
+#pragma unroll
+        for (int t = 0; t < CPL / 2; ++t) {
+          const bool in = (e >= 0) && (sub * CPL + 2 * t < p.k);
+          av[rr][t] = *reinterpret_cast<const double2 *>(a + 2 * t);
+          if (A.vec2) {
+            bv[rr][t] = in ? reinterpret_cast<const double2 *>(b)[t] : make_double2(0.0, 0.0);
+          } else {
+            const bool in1 = in && (sub * CPL + 2 * t + 1 < p.k);
+            bv[rr][t] = make_double2(in ? b[2 * t] : 0.0, in1 ? b[2 * t + 1] : 0.0);
+          }
+        }
+      }
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        double c = 0.0;
+#pragma unroll
+        for (int t = 0; t < CPL / 2; ++t) {
+          c = fma(av[rr][t].x, bv[rr][t].x, c);
+          c = fma(av[rr][t].y, bv[rr][t].y, c);
+        }
+        c += __shfl_xor(c, 8, 16);
+        c += __shfl_xor(c, 4, 16);
+        c += __shfl_xor(c, 2, 16);
+        c += __shfl_xor(c, 1, 16);
+        if (sub == r0 + rr) {
+          mine_c = c;
+          mine_e = ev[rr];
+        }
+      }
+    }
+    if (mine_e >= 0) {
+      const int rl = mine_e & 31, col = mine_e >> 5;
+      atomicAdd(&nacc[rl], near_value<TYPE>(p, row0 + rl, col, mine_c));
     }
     head += count;
   };
@@ -229,41 +293,39 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
   for (int jb = j0; jb < j1; jb += 16) {
     const bool more = jb + 16 < j1;
     if (more) fetch(jb + 16, bn, cbn);
-    const bool colok = jb + p16 < j1;
-    unsigned nearbits = 0;
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
       f4 c = cross_block<PREC, NC>(af[rb], bf);
+      unsigned long long nmask[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        float x2;  // estimated exponent, log2 units
+        float x2;  // estimated exponent, log2 units (-inf for padding)
         if constexpr (TYPE == GAUSSIAN) {
           x2 = fmaf(-2.0f * xs, c[v], ra_s[rb][v] + cb.x);
         } else {
           float d2 = fmaxf(fmaf(-2.0f, c[v], ra_s[rb][v] + cb.x), 0.0f);
           x2 = d2 * (ra_h[rb][v] * cb.y);
         }
-        const bool nr = colok && ((rowok >> (rb * 4 + v)) & 1u) && (x2 > -kFarT * kLog2e);
+        const bool nr = x2 > -kFarT * kLog2e;
         const float K = __builtin_amdgcn_exp2f(x2);
-        facc[rb][v] = fmaf((nr || !colok) ? 0.0f : K, cb.z, facc[rb][v]);
-        nearbits |= (nr ? 1u : 0u) << (rb * 4 + v);
+        facc[rb][v] = fmaf(nr ? 0.0f : K, cb.z, facc[rb][v]);
+        nmask[v] = __ballot(nr);
       }
-    }
-    // queue the near pairs of this block (row-local index in the low 5 bits)
-    if (__ballot(nearbits != 0)) {
+      // queue this row block's near pairs (row-local index in the low 5 bits);
+      // the ring holds < 64 + 4*64 entries at any time
 #pragma unroll
-      for (int s = 0; s < RB * 4; ++s) {
-        const bool nr = (nearbits >> s) & 1u;
-        const unsigned long long mask = __ballot(nr);
+      for (int v = 0; v < 4; ++v) {
+        const unsigned long long mask = nmask[v];
         if (mask) {
-          const unsigned off = __popcll(mask & ((1ull << lane) - 1ull));
-          if (nr) queue[(tail + off) & (kQueue - 1)] =
-                      ((jb + p16) << 5) | ((s >> 2) * 16 + out_row(g, s & 3));
+          if ((mask >> lane) & 1ull) {
+            const unsigned off = __popcll(mask & ((1ull << lane) - 1ull));
+            queue[(tail + off) & (kQueue - 1)] = ((jb + p16) << 5) | (rb * 16 + out_row(g, v));
+          }
           tail += (unsigned)__popcll(mask);
         }
       }
+      while (tail - head >= 64) drain(64);
     }
-    while (tail - head >= 64) drain(64);
     if (++nblk == kFlush) {
       nblk = 0;
 #pragma unroll
@@ -279,7 +341,7 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
       cb = cbn;
     }
   }
-  if (tail != head) drain(tail - head);
+  while (tail != head) drain(tail - head < 64 ? tail - head : 64);
 
   // ---- row totals: far partials summed over the 16 column lanes + near sums
 #pragma unroll
@@ -369,6 +431,7 @@ static hipError_t mixed_launch(const Problem &p, const FusedPlan &plan, double *
   A.nsplit = plan.nsplit;
   A.partial = partial;
   A.counters = counters;
+  A.vec2 = (p.k % 2 == 0) && ((uintptr_t)p.XA % 16 == 0) && ((uintptr_t)p.XB % 16 == 0);
   const float xs = (float)(p.P.scal * (double)kLog2e);
   long tot = (long)p.m + p.n;
   dim3 pg((unsigned)((tot + 255) / 256));

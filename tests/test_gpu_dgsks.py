@@ -346,3 +346,47 @@ def test_list_mixed_kernels(G, oracle, torch_cuda):
     oracle.dgsks_list(d["kernels"], 8, ref, d["a_off"], d["amap"], d["amap"], d["b_off"],
                       d["bmap"], d["bmap"], X, X2, X, X2, w)
     assert oracle.rel_l2(u, ref) <= TOL
+
+
+@pytest.mark.parametrize("k", [3, 8, 32, 20])
+def test_list_persistent_shared_targets(G, oracle, torch_cuda, k):
+    """Persistent list launch: mixed kernel types (Gaussian, Laplace, polynomial),
+    tasks sharing target points, some tasks with more sources than one tile
+    slice, empty tasks; every k path (scalar-load, matrix-core, generic)."""
+    from ks_amd import _lib
+    L = _lib.lib()
+    rng = np.random.default_rng(300 + k)
+    pool, ntask = 6000, 40
+    X = rng.random((pool, k)) * (1.5 / np.sqrt(k))
+    X2 = oracle.sqnorm(X)
+    w = rng.standard_normal(pool)
+    ms = rng.integers(1, 300, ntask)
+    ns = rng.integers(1, 2600, ntask)
+    ms[3] = 0
+    ns[5] = 0
+    amaps = [rng.choice(pool, int(s), replace=False).astype(np.int32) for s in ms]
+    bmaps = [rng.choice(pool, int(s), replace=False).astype(np.int32) for s in ns]
+    kinds = [dict(type=0, scal=-1.5), dict(type=2), dict(type=1, scal=1.0 / k, cons=1.0, powe=2.0)]
+    kernels_py = [kinds[t % 3] for t in range(ntask)]
+    a_off = np.concatenate([[0], np.cumsum(ms)]).astype(np.int64)
+    b_off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    amap = np.concatenate(amaps)
+    bmap = np.concatenate(bmaps)
+    kernels = (_lib.KsT * ntask)()
+    for t, kd in enumerate(kernels_py):
+        kernels[t].type = kd["type"]
+        kernels[t].scal = kd.get("scal", 0.0)
+        kernels[t].cons = kd.get("cons", 0.0)
+        kernels[t].powe = kd.get("powe", 0.0)
+    u = rng.standard_normal(pool)
+    ref = u.copy()
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    rc = L.dgsks_list(ntask, kernels, k, p(u), u.size, p(a_off), p(amap), p(amap), p(b_off),
+                      p(bmap), p(bmap), p(X), p(X2), p(X), p(X2), p(w), 0)
+    assert rc == 0
+    oracle.dgsks_list(kernels_py, k, ref, a_off, amap, amap, b_off, bmap, bmap, X, X2, X, X2, w)
+    err = oracle.rel_l2(u, ref)
+    print(f"list k={k}: rel L2 = {err:.3e}")
+    assert err <= TOL
+    if k >= 3:  # Laplace descriptor side effect per task
+        assert kernels[1].powe == 0.5 * (2 - k)
