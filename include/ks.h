@@ -170,6 +170,24 @@ int dgsks_device(const ks_t *kernel, int m, int n, int k,
  * (NULL = the HIP null stream). */
 int ks_sqnorm_device(const double *X, long nx, int k, double *X2, void *stream);
 
+/* Several right-hand sides at once (the reference's KS_RHS > 1 layout,
+ * include/ks.h:46): U[umap[i]*nrhs + r] += sum_j K(a_i, b_j) W[wmap[j]*nrhs + r]
+ * for r < nrhs <= 72.  Device pointers, enqueued on `stream`. */
+int dgsks_multirhs_device(const ks_t *kernel, int m, int n, int k, int nrhs,
+                          double *U, const int *umap,
+                          const double *XA, const double *XA2, const int *amap,
+                          const double *XB, const double *XB2, const int *bmap,
+                          const double *W, const int *wmap, void *stream);
+
+/* Gradient of L = sum_i g[i] u[i] for the Gaussian summation u = K w,
+ * K_ij = exp(-|a_i - b_j|^2 / (2 h^2)), identity maps, XA [m][k], XB [n][k]:
+ * dXA [m][k], dXB [n][k], dw [n] (the transposed summation) and dh [1]
+ * (outputs may be NULL).  The kernel matrix is recomputed, never stored.
+ * Device pointers, enqueued on `stream`; k <= 70. */
+int ks_gauss_grad_device(int m, int n, int k, const double *XA, const double *XB,
+                         const double *w, const double *g, double h,
+                         double *dXA, double *dXB, double *dw, double *dh, void *stream);
+
 /* Batched summation: ntask independent summations over shared point pools,
  * one persistent launch.  Task t owns the ranges [a_off[t], a_off[t+1]) of
  * amap/umap and [b_off[t], b_off[t+1]) of bmap/wmap and uses kernels[t]

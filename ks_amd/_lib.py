@@ -51,11 +51,10 @@ _SIGS = {
     "dgsks_list": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(KsT), ctypes.c_int, _vp,
                                   ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                   _vp, _vp, ctypes.c_int]),
-    "dgsks_list_device": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp,
-                                         _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                         ctypes.c_int, _vp]),
-    "gsks_gauss_grad_device": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int] +
-                               [_vp] * 9 + [ctypes.c_double, ctypes.c_int, _vp]),
+    "dgsks_multirhs_device": (ctypes.c_int, [ctypes.POINTER(KsT), ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_int] + [_vp] * 10 + [_vp]),
+    "ks_gauss_grad_device": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp,
+                                            _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
 }
 
 

@@ -22,6 +22,8 @@ int make_params(const ks_t *ker, int k, KerParams &P, bool warn);
 int enqueue_fused(const Problem &p, hipStream_t s);
 // The reference algorithm (unfused: cross-term panel, elementwise kernel, GEMV).
 int enqueue_reference(const Problem &p, hipStream_t s);
+// Multi-right-hand-side summation: p.u / p.w are [npoints][nrhs] matrices.
+int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s);
 // Mixed-precision summation (KS_PREC_MIXED_*).
 int enqueue_mixed(const Problem &p, int precision, hipStream_t s);
 

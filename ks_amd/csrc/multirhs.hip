@@ -1,0 +1,370 @@
+// Multi-right-hand-side fused summation and the Gaussian gradient built on it.
+//
+//   U[umap[i]][r] += sum_j K(XA[amap[i]], XB[bmap[j]]) * W[wmap[j]][r],  r < nrhs
+//
+// This is the reference's KS_RHS > 1 layout (u and w row-major [npoints][rhs],
+// include/ks.h:46; rhs loops in micro_kernel/x86_64/sandybridge/
+// ks_kernel_summation_int_d8x4.h): the GEMV becomes a small GEMM epilogue.  The
+// kernel matrix is recomputed tile by tile, never stored.  Mapping as in
+// fused_f64_kernel: a lane owns one target row (coordinates in VGPRs), the
+// workgroup streams one source slice with wave-uniform scalar loads, and each
+// K value feeds nrhs FMAs against the source's weight row (also scalar loads);
+// split slices are reduced deterministically by the last workgroup.
+//
+// The gradient of L = g . u for u = K w, K_ij = exp(-|a_i - b_j|^2 / (2 h^2)),
+// uses two such passes (identity maps):
+//   rows = a, sources = b, RHS [w, w|b|^2, w b_1..w b_k] -> [u_i, T_i, V_i]
+//   rows = b, sources = a, RHS [g, g a_1..g a_k]         -> [dw_j, Q_j]
+// and then, per point,
+//   dXA_i = -(g_i / h^2) (u_i a_i - V_i)
+//   dXB_j =  (w_j / h^2) (Q_j - dw_j b_j)
+//   dh    = (1 / h^3) sum_i g_i (|a_i|^2 u_i + T_i - 2 a_i . V_i).
+#include <hip/hip_runtime.h>
+
+#include "gsks_kernels.h"
+#include "gsks_launch.h"
+#include "host_util.h"
+
+namespace gsks {
+
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+
+template <typename T>
+__device__ __forceinline__ cptr<T> to_const(const T *p) {
+  return (cptr<T>)(p);
+}
+
+struct MrhsArgs {
+  Problem p;  // p.u / p.w are the [npoints][nrhs] matrices U / W
+  int nrhs;
+  int cols_per_split, nsplit;
+  double *partial;     // [nsplit][m][nrhs]
+  unsigned *counters;  // [row blocks]
+};
+
+constexpr int MR_WG = 256;
+
+template <int TYPE, int K, int RMAX>
+__global__ __launch_bounds__(MR_WG) void multirhs_kernel(MrhsArgs A) {
+  const Problem &p = A.p;
+  const int m = p.m, nr = A.nrhs;
+  const int kdim = K > 0 ? K : p.k;
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * MR_WG + tid;
+  const int split = blockIdx.y;
+  const int ai = p.amap[i < m ? i : m - 1];
+  const double *arow = p.XA + (long)ai * kdim;
+  constexpr int KR = K > 0 ? K : 1;
+  double a[KR];
+  if constexpr (K > 0) {
+#pragma unroll
+    for (int q = 0; q < K; ++q) a[q] = arow[q];
+  }
+  const double aa = p.XA2[ai];
+  const double hi = (TYPE == VAR_BANDWIDTH) ? -0.5 * p.hi[ai] : 0.0;
+  double acc[RMAX];
+#pragma unroll
+  for (int r = 0; r < RMAX; ++r) acc[r] = 0.0;
+
+  const int j0 = split * A.cols_per_split;
+  const int j1 = min(p.n, j0 + A.cols_per_split);
+  cptr<int> bmap = to_const(p.bmap);
+  cptr<int> wmap = to_const(p.wmap);
+  cptr<double> XB = to_const(p.XB);
+  cptr<double> XB2 = to_const(p.XB2);
+  cptr<double> W = to_const(p.w);
+  cptr<double> HJ = to_const(p.hj);
+  const KerParams P = p.P;
+  for (int j = j0; j < j1; ++j) {
+    const int bj = bmap[j];
+    const double bb = XB2[bj];
+    const double hj = (TYPE == VAR_BANDWIDTH) ? HJ[bj] : 0.0;
+    cptr<double> b = XB + (long)bj * kdim;
+    double c;
+    if constexpr (TYPE == LAPLACE) {
+#pragma clang fp contract(off)
+      c = (K > 0 ? a[0] : arow[0]) * b[0];
+      for (int q = 1; q < kdim; ++q) c = c + (K > 0 ? a[q] : arow[q]) * b[q];
+    } else if constexpr (K > 0) {
+      c = a[0] * b[0];
+#pragma unroll
+      for (int q = 1; q < K; ++q) c = fma(a[q], b[q], c);
+    } else {
+      c = arow[0] * b[0];
+      for (int q = 1; q < kdim; ++q) c = fma(arow[q], b[q], c);
+    }
+    const double Kv = kval_f64<TYPE>(c, aa, bb, hi, hj, P);
+    cptr<double> wr = W + (long)wmap[j] * nr;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r)
+      if (r < nr) acc[r] = fma(Kv, wr[r], acc[r]);
+  }
+
+  if (A.nsplit == 1) {
+    if (i < m) {
+      double *ur = p.u + (long)p.umap[i] * nr;
+#pragma unroll
+      for (int r = 0; r < RMAX; ++r)
+        if (r < nr) ur[r] += acc[r];
+    }
+    return;
+  }
+  if (i < m) {
+    double *pr = A.partial + ((long)split * m + i) * nr;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r)
+      if (r < nr) pr[r] = acc[r];
+  }
+  __threadfence();
+  __syncthreads();
+  __shared__ int s_last;
+  if (tid == 0) {
+    unsigned prev = atomicAdd(&A.counters[blockIdx.x], 1u);
+    s_last = (prev == (unsigned)(A.nsplit - 1));
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  if (i < m) {
+    double *ur = p.u + (long)p.umap[i] * nr;
+    for (int r = 0; r < nr; ++r) {
+      double s = 0.0;
+      for (int sp = 0; sp < A.nsplit; ++sp) s += A.partial[((long)sp * m + i) * nr + r];
+      ur[r] += s;
+    }
+  }
+  if (tid == 0) A.counters[blockIdx.x] = 0u;
+}
+
+template <int TYPE, int RMAX>
+static hipError_t mr_k(const MrhsArgs &A, int rblocks, hipStream_t s) {
+  dim3 grid(rblocks, A.nsplit);
+  switch (A.p.k) {
+    case 3: hipLaunchKernelGGL((multirhs_kernel<TYPE, 3, RMAX>), grid, dim3(MR_WG), 0, s, A); break;
+    case 16: hipLaunchKernelGGL((multirhs_kernel<TYPE, 16, RMAX>), grid, dim3(MR_WG), 0, s, A); break;
+    default: hipLaunchKernelGGL((multirhs_kernel<TYPE, 0, RMAX>), grid, dim3(MR_WG), 0, s, A); break;
+  }
+  return hipGetLastError();
+}
+
+template <int TYPE>
+static hipError_t mr_r(const MrhsArgs &A, int rblocks, hipStream_t s) {
+  if (A.nrhs <= 8) return mr_k<TYPE, 8>(A, rblocks, s);
+  if (A.nrhs <= 24) return mr_k<TYPE, 24>(A, rblocks, s);
+  return mr_k<TYPE, 72>(A, rblocks, s);
+}
+
+int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s) {
+  if (nrhs < 1 || nrhs > 72) return KS_ERR_INVALID_ARG;
+  FusedPlan plan = plan_split(p.m, p.n, MR_WG, 2048, 128);
+  double *partial = nullptr;
+  unsigned *counters = nullptr;
+  if (plan.nsplit > 1) {
+    int rc = get_scratch(s, (size_t)plan.nsplit * p.m * nrhs, (size_t)plan.row_blocks, &partial,
+                         &counters);
+    if (rc != KS_OK) return rc;
+  }
+  MrhsArgs A;
+  A.p = p;
+  A.nrhs = nrhs;
+  A.cols_per_split = plan.cols_per_split;
+  A.nsplit = plan.nsplit;
+  A.partial = partial;
+  A.counters = counters;
+  hipError_t e;
+  switch (p.type) {
+    case GAUSSIAN: e = mr_r<GAUSSIAN>(A, plan.row_blocks, s); break;
+    case POLYNOMIAL: e = mr_r<POLYNOMIAL>(A, plan.row_blocks, s); break;
+    case LAPLACE: e = mr_r<LAPLACE>(A, plan.row_blocks, s); break;
+    case VAR_BANDWIDTH: e = mr_r<VAR_BANDWIDTH>(A, plan.row_blocks, s); break;
+    case TANH: e = mr_r<TANH>(A, plan.row_blocks, s); break;
+    case QUARTIC: e = mr_r<QUARTIC>(A, plan.row_blocks, s); break;
+    case MULTIQUADRATIC: e = mr_r<MULTIQUADRATIC>(A, plan.row_blocks, s); break;
+    default: e = mr_r<EPANECHNIKOV>(A, plan.row_blocks, s); break;
+  }
+  return e == hipSuccess ? KS_OK : KS_ERR_DEVICE;
+}
+
+// ------------------------------------------------------------------ Gaussian gradient
+
+// RHS of the first pass: [w, w|b|^2, w b]
+__global__ void grad_rhs_b(const double *XB, const double *XB2, const double *w, int n, int k,
+                           double *W1) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int R = k + 2;
+  const double wj = w[j];
+  W1[j * R] = wj;
+  W1[j * R + 1] = wj * XB2[j];
+  for (int q = 0; q < k; ++q) W1[j * R + 2 + q] = wj * XB[j * k + q];
+}
+
+// RHS of the second pass: [g, g a]
+__global__ void grad_rhs_a(const double *XA, const double *g, int m, int k, double *W2) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int R = k + 1;
+  const double gi = g[i];
+  W2[i * R] = gi;
+  for (int q = 0; q < k; ++q) W2[i * R + 1 + q] = gi * XA[i * k + q];
+}
+
+// dXA, and per-row dh terms (summed afterwards in a fixed order)
+__global__ void grad_fin_a(const double *XA, const double *XA2, const double *g, const double *U1,
+                           int m, int k, double h, double *dXA, double *dh_rows) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int R = k + 2;
+  const double *u = U1 + i * R;
+  const double ui = u[0], Ti = u[1];
+  const double f = -g[i] / (h * h);
+  double av = 0.0;
+  for (int q = 0; q < k; ++q) {
+    const double a = XA[i * k + q], V = u[2 + q];
+    dXA[i * k + q] = f * (ui * a - V);
+    av = fma(a, V, av);
+  }
+  dh_rows[i] = g[i] * (XA2[i] * ui + Ti - 2.0 * av);
+}
+
+__global__ void grad_fin_b(const double *XB, const double *w, const double *U2, int n, int k,
+                           double h, double *dXB, double *dw) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int R = k + 1;
+  const double *u = U2 + j * R;
+  const double f = w[j] / (h * h);
+  dw[j] = u[0];
+  for (int q = 0; q < k; ++q) dXB[j * k + q] = f * (u[1 + q] - u[0] * XB[j * k + q]);
+}
+
+// deterministic sum of the per-row dh terms: one workgroup, fixed tree
+__global__ void grad_fin_h(const double *dh_rows, int m, double h, double *dh) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < m; i += 256) acc += dh_rows[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *dh = s[0] / (h * h * h);
+}
+
+__global__ void zero_f64(double *v, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = 0.0;
+}
+
+__global__ void iota_i32(int *v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = i;
+}
+
+}  // namespace gsks
+
+using namespace gsks;
+
+extern "C" int dgsks_multirhs_device(const ks_t *kernel, int m, int n, int k, int nrhs,
+                                     double *U, const int *umap, const double *XA,
+                                     const double *XA2, const int *amap, const double *XB,
+                                     const double *XB2, const int *bmap, const double *W,
+                                     const int *wmap, void *stream) {
+  if (!kernel || m < 0 || n < 0 || k < 0 || nrhs < 1 || nrhs > 72)
+    return raise_error(KS_ERR_INVALID_ARG, false, "dgsks_multirhs_device(): invalid argument\n");
+  if (m == 0 || n == 0 || k == 0) return KS_OK;
+  if (ensure_device() != KS_OK)
+    return raise_error(KS_ERR_DEVICE, false, "dgsks_multirhs_device(): no GPU available\n");
+  Problem p;
+  int rc = make_params(kernel, k, p.P, true);
+  if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return rc;
+  const void *arrays[] = {U, umap, XA, XA2, amap, XB, XB2, bmap, W, wmap};
+  for (const void *a : arrays)
+    if (!a || !device_ptr_cached(a))
+      return raise_error(KS_ERR_MIXED_POINTERS, false,
+                         "dgsks_multirhs_device(): every array must be device memory\n");
+  p.m = m; p.n = n; p.k = k; p.type = (int)kernel->type;
+  p.u = U; p.umap = umap; p.XA = XA; p.XA2 = XA2; p.amap = amap;
+  p.XB = XB; p.XB2 = XB2; p.bmap = bmap; p.w = W; p.wmap = wmap;
+  p.hi = kernel->hi; p.hj = kernel->hj;
+  if (p.type == KS_GAUSSIAN_VAR_BANDWIDTH && (!p.hi || !p.hj)) return KS_ERR_NO_BANDWIDTH;
+  rc = enqueue_multirhs(p, nrhs, (hipStream_t)stream);
+  return rc == KS_OK ? KS_OK : raise_error(rc, false, "dgsks_multirhs_device(): %s\n", err_string(rc));
+}
+
+// Gradient of L = g . (K w) for the Gaussian kernel with bandwidth h and identity
+// maps; all arrays device memory, XA [m][k], XB [n][k].  Outputs dXA [m][k],
+// dXB [n][k], dw [n], dh [1] (any output may be NULL).  Temporary buffers come from
+// the stream-ordered pool.
+extern "C" int ks_gauss_grad_device(int m, int n, int k, const double *XA, const double *XB,
+                                    const double *w, const double *g, double h, double *dXA,
+                                    double *dXB, double *dw, double *dh, void *stream) {
+  if (m <= 0 || n <= 0 || k <= 0 || k > 70 || !(h > 0.0) || !XA || !XB || !w || !g)
+    return raise_error(KS_ERR_INVALID_ARG, false, "ks_gauss_grad_device(): invalid argument\n");
+  if (ensure_device() != KS_OK)
+    return raise_error(KS_ERR_DEVICE, false, "ks_gauss_grad_device(): no GPU available\n");
+  hipStream_t s = (hipStream_t)stream;
+  const int R1 = k + 2, R2 = k + 1;
+  size_t bytes = sizeof(double) * ((size_t)m + n + (size_t)n * R1 + (size_t)m * R1 +
+                                   (size_t)m * R2 + (size_t)n * R2 + m + (size_t)m * k +
+                                   (size_t)n * k + n + 1) +
+                 sizeof(int) * ((size_t)(m > n ? m : n)) + 16 * 256;
+  char *buf = nullptr;
+  if (hipMallocAsync((void **)&buf, bytes, s) != hipSuccess) {
+    (void)hipGetLastError();
+    return raise_error(KS_ERR_ALLOC, false, "ks_gauss_grad_device(): allocation failed\n");
+  }
+  char *q = buf;
+  auto take = [&](size_t b) {
+    char *r = q;
+    q += (b + 255) & ~(size_t)255;
+    return r;
+  };
+  double *XA2 = (double *)take(sizeof(double) * m);
+  double *XB2 = (double *)take(sizeof(double) * n);
+  double *W1 = (double *)take(sizeof(double) * (size_t)n * R1);
+  double *U1 = (double *)take(sizeof(double) * (size_t)m * R1);
+  double *W2 = (double *)take(sizeof(double) * (size_t)m * R2);
+  double *U2 = (double *)take(sizeof(double) * (size_t)n * R2);
+  double *hrows = (double *)take(sizeof(double) * m);
+  double *tXA = (double *)take(sizeof(double) * (size_t)m * k);
+  double *tXB = (double *)take(sizeof(double) * (size_t)n * k);
+  double *tdw = (double *)take(sizeof(double) * n);
+  double *tdh = (double *)take(sizeof(double));
+  int *ident = (int *)take(sizeof(int) * (size_t)(m > n ? m : n));
+  const int mx = m > n ? m : n;
+  int rc = KS_OK;
+  rc |= ks_sqnorm_device(XA, m, k, XA2, s);
+  rc |= ks_sqnorm_device(XB, n, k, XB2, s);
+  hipLaunchKernelGGL(iota_i32, dim3((mx + 255) / 256), dim3(256), 0, s, ident, mx);
+  hipLaunchKernelGGL(grad_rhs_b, dim3((n + 255) / 256), dim3(256), 0, s, XB, XB2, w, n, k, W1);
+  hipLaunchKernelGGL(grad_rhs_a, dim3((m + 255) / 256), dim3(256), 0, s, XA, g, m, k, W2);
+  hipLaunchKernelGGL(zero_f64, dim3((unsigned)(((long)m * R1 + 255) / 256)), dim3(256), 0, s, U1,
+                     (long)m * R1);
+  hipLaunchKernelGGL(zero_f64, dim3((unsigned)(((long)n * R2 + 255) / 256)), dim3(256), 0, s, U2,
+                     (long)n * R2);
+  Problem p1;
+  p1.m = m; p1.n = n; p1.k = k; p1.type = GAUSSIAN;
+  p1.P = KerParams{-1.0 / (2.0 * h * h), 0.0, 0.0, 0, POW_GENERAL};
+  p1.u = U1; p1.umap = ident; p1.XA = XA; p1.XA2 = XA2; p1.amap = ident;
+  p1.XB = XB; p1.XB2 = XB2; p1.bmap = ident; p1.w = W1; p1.wmap = ident;
+  p1.hi = p1.hj = nullptr;
+  if (rc == KS_OK) rc = enqueue_multirhs(p1, R1, s);
+  Problem p2 = p1;
+  p2.m = n; p2.n = m;
+  p2.u = U2; p2.XA = XB; p2.XA2 = XB2; p2.XB = XA; p2.XB2 = XA2; p2.w = W2;
+  if (rc == KS_OK) rc = enqueue_multirhs(p2, R2, s);
+  if (rc == KS_OK) {
+    hipLaunchKernelGGL(grad_fin_a, dim3((m + 255) / 256), dim3(256), 0, s, XA, XA2, g, U1, m, k, h,
+                       dXA ? dXA : tXA, hrows);
+    hipLaunchKernelGGL(grad_fin_b, dim3((n + 255) / 256), dim3(256), 0, s, XB, w, U2, n, k, h,
+                       dXB ? dXB : tXB, dw ? dw : tdw);
+    hipLaunchKernelGGL(grad_fin_h, dim3(1), dim3(256), 0, s, hrows, m, h, dh ? dh : tdh);
+    if (hipGetLastError() != hipSuccess) rc = KS_ERR_DEVICE;
+  }
+  (void)hipFreeAsync(buf, s);
+  if (rc != KS_OK) return raise_error(rc, false, "ks_gauss_grad_device(): %s\n", err_string(rc));
+  return KS_OK;
+}
