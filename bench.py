@@ -33,9 +33,12 @@ METRIC = "kernel evals/s (mÂ·n/s), FP64 drop-in vs mixed precision at rel err â‰
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector = FP64 matrix peak
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix peak
-# HBM bytes per launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE
-# (profiles/), keyed by (workload, precision); filled in from those summaries.
-TRAFFIC = {}
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE, each
+# scaled by what the counter reads for a 1 GiB device copy (FETCH_SIZE x 2,
+# WRITE_SIZE x 1), keyed by (workload, precision); the summaries and commands
+# are in profiles/mixture_k64_fp64_and_mixed.md.
+TRAFFIC = {("mixture_k64", 0): 8.867e9,   # fused_f64_mfma_kernel
+           ("mixture_k64", 2): 3.745e9}   # mixed_kernel + mixed_pack_kernel
 # flops per kernel evaluation in the reference's model, test/test_dgsks.c:254-283
 FLOP_C = {0: 37, 1: 6, 2: 60, 3: 35, 4: 89, 5: 8, 6: 6, 7: 7}
 TYPE_NAMES = ["gaussian", "polynomial", "laplace", "var_bandwidth", "tanh", "quartic",
