@@ -143,6 +143,8 @@ enum { KS_ERRMODE_EXIT = 0, KS_ERRMODE_RETURN = 1 };
 
 int ks_last_error(void);
 const char *ks_error_string(int code);
+/* Which HIP call failed last on this thread, with HIP's message. */
+const char *ks_error_detail(void);
 void ks_set_error_mode(int mode);
 int ks_get_error_mode(void);
 

@@ -43,6 +43,7 @@ _SIGS = {
                           [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp]),
     "ks_last_error": (ctypes.c_int, []),
     "ks_error_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "ks_error_detail": (ctypes.c_char_p, []),
     "ks_set_error_mode": (None, [ctypes.c_int]),
     "ks_get_error_mode": (ctypes.c_int, []),
     "dgsks_device": (ctypes.c_int, [ctypes.POINTER(KsT), ctypes.c_int, ctypes.c_int, ctypes.c_int] +
@@ -58,6 +59,19 @@ _SIGS = {
 }
 
 
+def _bind_process_hip_runtime():
+    """Load torch's HIP runtime before libgsks.so when torch is installed.
+
+    torch ships its own libamdhip64 with the same SONAME as ROCm's.  If
+    libgsks.so were loaded first, a later `import torch` would bring a second
+    HIP runtime into the process and the two would compete for the device;
+    loading torch first makes libgsks.so bind to the runtime already loaded."""
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def lib():
     """Load libgsks.so (once).  Raises when it is missing."""
     global _lib
@@ -67,6 +81,7 @@ def lib():
                 raise RuntimeError(
                     f"{LIB_PATH} is not built: run `python -m ks_amd.build` "
                     "(ks_amd has no CPU fallback)")
+            _bind_process_hip_runtime()
             L = ctypes.CDLL(LIB_PATH)
             for name, (res, args) in _SIGS.items():
                 fn = getattr(L, name, None)
@@ -87,7 +102,9 @@ def error_string(code: int) -> str:
 class KsError(RuntimeError):
     def __init__(self, code: int, where: str = ""):
         self.code = code
-        super().__init__(f"{where}: {error_string(code)} (code {code})")
+        detail = lib().ks_error_detail().decode() if code == KS_ERR_DEVICE else ""
+        super().__init__(f"{where}: {error_string(code)} (code {code})" +
+                         (f" [{detail}]" if detail else ""))
 
 
 def check(code: int, where: str):

@@ -28,6 +28,13 @@ namespace gsks {
 
 static std::atomic<int> g_errmode{KS_ERRMODE_EXIT};
 static thread_local int t_last_error = KS_OK;
+static thread_local char t_detail[256] = "";
+
+// Remember which HIP call failed and why (read back by ks_error_detail()).
+int hip_failure(hipError_t e, const char *where) {
+  snprintf(t_detail, sizeof(t_detail), "%s: %s", where, hipGetErrorString(e));
+  return KS_ERR_DEVICE;
+}
 
 const char *err_string(int code) {
   switch (code) {
@@ -91,7 +98,11 @@ hipStream_t thread_stream() {
   int d = 0;
   (void)hipGetDevice(&d);
   if (!s || dev != d) {
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      hip_failure(e, "hipStreamCreateWithFlags");
+      s = nullptr;
+    }
     dev = d;
   }
   return s;
@@ -106,7 +117,8 @@ int get_scratch(hipStream_t s, size_t partial_elems, size_t counter_elems, doubl
   std::lock_guard<std::mutex> lk(g_mu);
   Scratch &sc = g_scratch[{d, s}];
   if (sc.partial_elems < partial_elems || sc.counter_elems < counter_elems) {
-    if (hipStreamSynchronize(s) != hipSuccess) return KS_ERR_DEVICE;
+    hipError_t se = hipStreamSynchronize(s);
+    if (se != hipSuccess) return hip_failure(se, "get_scratch: hipStreamSynchronize");
     if (sc.partial_elems < partial_elems) {
       if (sc.partial) (void)hipFree(sc.partial);
       sc.partial = nullptr;
@@ -125,7 +137,8 @@ int get_scratch(hipStream_t s, size_t partial_elems, size_t counter_elems, doubl
         sc.counter_elems = 0;
         return KS_ERR_ALLOC;
       }
-      if (hipMemset(sc.counters, 0, want * sizeof(unsigned)) != hipSuccess) return KS_ERR_DEVICE;
+      hipError_t me = hipMemset(sc.counters, 0, want * sizeof(unsigned));
+      if (me != hipSuccess) return hip_failure(me, "get_scratch: hipMemset");
       sc.counter_elems = want;
     }
   }
@@ -219,7 +232,7 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
   }
   hipError_t e = mfma ? launch_fused_f64_mfma(p, plan, partial, counters, s)
                       : launch_fused_f64(p, plan, partial, counters, s);
-  return e == hipSuccess ? KS_OK : KS_ERR_DEVICE;
+  return e == hipSuccess ? KS_OK : hip_failure(e, mfma ? "fused_f64_mfma launch" : "fused_f64 launch");
 }
 
 }  // namespace gsks
@@ -235,6 +248,8 @@ int ks_last_error(void) {
 }
 
 const char *ks_error_string(int code) { return err_string(code); }
+
+const char *ks_error_detail(void) { return t_detail; }
 
 void ks_set_error_mode(int mode) {
   g_errmode.store(mode == KS_ERRMODE_RETURN ? KS_ERRMODE_RETURN : KS_ERRMODE_EXIT);

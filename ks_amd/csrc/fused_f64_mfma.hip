@@ -33,20 +33,31 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int MWG = 256;  // 4 waves
 
-template <int KC>
+// Operand fragment of one point: coordinates c*16 + 4g .. +3 for each 16-chunk c.
+// EXACT (k == 16*KC, 16-byte aligned rows): two unguarded 16-byte loads per chunk.
+template <int KC, bool EXACT>
 __device__ __forceinline__ void load_frag(const double *__restrict__ X, long base, int k, int g,
                                           double (&f)[KC][4]) {
 #pragma unroll
   for (int c = 0; c < KC; ++c) {
+    if constexpr (EXACT) {
+      const double2 *q = reinterpret_cast<const double2 *>(X + base + c * 16 + 4 * g);
+      const double2 x0 = q[0], x1 = q[1];
+      f[c][0] = x0.x;
+      f[c][1] = x0.y;
+      f[c][2] = x1.x;
+      f[c][3] = x1.y;
+    } else {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      int idx = c * 16 + 4 * g + t;
-      f[c][t] = (idx < k) ? X[base + idx] : 0.0;
+      for (int t = 0; t < 4; ++t) {
+        int idx = c * 16 + 4 * g + t;
+        f[c][t] = (idx < k) ? X[base + idx] : 0.0;
+      }
     }
   }
 }
 
-template <int TYPE, int KC, int RB>
+template <int TYPE, int KC, int RB, bool EXACT>
 __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
   const Problem &p = A.p;
   const int m = p.m, k = p.k;
@@ -66,7 +77,7 @@ __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
   for (int rb = 0; rb < RB; ++rb) {
     int r = row0 + rb * 16 + p16;
     int ai = p.amap[r < m ? r : m - 1];
-    load_frag<KC>(p.XA, (long)ai * k, k, g, af[rb]);
+    load_frag<KC, EXACT>(p.XA, (long)ai * k, k, g, af[rb]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       int ro = row0 + rb * 16 + g + 4 * v;
@@ -81,22 +92,19 @@ __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
   const int j1 = min(p.n, j0 + A.cols_per_split);
   const KerParams P = p.P;
 
-  // ---- stream 16-point source blocks, B fragment double-buffered in registers
-  double bf[KC][4], bn[KC][4];
-  double bb, wj, hj, bbn, wjn, hjn;
+  // ---- stream 16-point source blocks; two register buffers used in turn
+  double bf0[KC][4], bf1[KC][4];
+  double bb0 = 0.0, wj0 = 0.0, hj0 = 0.0, bb1 = 0.0, wj1 = 0.0, hj1 = 0.0;
   auto fetch = [&](int jb, double (&f)[KC][4], double &b2, double &ww, double &hh) {
     int col = jb + p16;
     bool ok = col < j1;
     int bj = p.bmap[ok ? col : j1 - 1];
-    load_frag<KC>(p.XB, (long)bj * k, k, g, f);
+    load_frag<KC, EXACT>(p.XB, (long)bj * k, k, g, f);
     b2 = p.XB2[bj];
     ww = ok ? p.w[p.wmap[col]] : 0.0;
     hh = (TYPE == VAR_BANDWIDTH) ? p.hj[bj] : 0.0;
   };
-  if (j0 < j1) fetch(j0, bf, bb, wj, hj);
-  for (int jb = j0; jb < j1; jb += 16) {
-    const bool more = jb + 16 < j1;
-    if (more) fetch(jb + 16, bn, bbn, wjn, hjn);
+  auto block = [&](const double (&f)[KC][4], double bb, double wj, double hj) {
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
       d4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -104,7 +112,7 @@ __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
       for (int c = 0; c < KC; ++c) {
 #pragma unroll
         for (int t = 0; t < 4; ++t)
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[rb][c][t], bf[c][t], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[rb][c][t], f[c][t], acc, 0, 0, 0);
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -112,15 +120,14 @@ __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
         racc[rb][v] = fma(K, wj, racc[rb][v]);
       }
     }
-    if (more) {
-#pragma unroll
-      for (int c = 0; c < KC; ++c)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) bf[c][t] = bn[c][t];
-      bb = bbn;
-      wj = wjn;
-      hj = hjn;
-    }
+  };
+  if (j0 < j1) fetch(j0, bf0, bb0, wj0, hj0);
+  for (int jb = j0; jb < j1; jb += 32) {
+    if (jb + 16 < j1) fetch(jb + 16, bf1, bb1, wj1, hj1);
+    block(bf0, bb0, wj0, hj0);
+    if (jb + 16 >= j1) break;
+    if (jb + 32 < j1) fetch(jb + 32, bf0, bb0, wj0, hj0);
+    block(bf1, bb1, wj1, hj1);
   }
 
   // ---- sum the 16 lanes (source columns) of each row group
@@ -191,7 +198,12 @@ int mfma_rows_per_wg() { return 4 * kMfmaRB * 16; }
 template <int TYPE, int KC>
 static hipError_t go(const FusedArgs &A, int row_blocks, hipStream_t s) {
   dim3 grid(row_blocks, A.nsplit);
-  hipLaunchKernelGGL((fused_f64_mfma_kernel<TYPE, KC, kMfmaRB>), grid, dim3(MWG), 0, s, A);
+  const bool exact = A.p.k == 16 * KC && (uintptr_t)A.p.XA % 16 == 0 &&
+                     (uintptr_t)A.p.XB % 16 == 0;
+  if (exact)
+    hipLaunchKernelGGL((fused_f64_mfma_kernel<TYPE, KC, kMfmaRB, true>), grid, dim3(MWG), 0, s, A);
+  else
+    hipLaunchKernelGGL((fused_f64_mfma_kernel<TYPE, KC, kMfmaRB, false>), grid, dim3(MWG), 0, s, A);
   return hipGetLastError();
 }
 

@@ -11,6 +11,7 @@ namespace gsks {
 const char *err_string(int code);
 int raise_error(int code, bool fatal, const char *fmt, ...);
 void set_last_error(int code);
+int hip_failure(hipError_t e, const char *where);
 int ensure_device();
 hipStream_t thread_stream();
 int get_scratch(hipStream_t s, size_t partial_elems, size_t counter_elems, double **partial,

@@ -105,7 +105,10 @@ void run_staged(const ks_t *kernel, int m, int n, int k, double *u, const int *u
     p.hi = kernel->hi;
     p.hj = kernel->hj;
     rc = reference_algorithm ? enqueue_reference(p, s) : enqueue_fused(p, s);
-    if (rc == KS_OK && hipStreamSynchronize(s) != hipSuccess) rc = KS_ERR_DEVICE;
+    if (rc == KS_OK) {
+      hipError_t e = hipStreamSynchronize(s);
+      if (e != hipSuccess) rc = hip_failure(e, "hipStreamSynchronize");
+    }
     finish(rc, name);
     return;
   }
@@ -160,8 +163,10 @@ void run_staged(const ks_t *kernel, int m, int n, int k, double *u, const int *u
   ar.cap = bytes;
   auto up = [&](const void *src, size_t nbytes) -> void * {
     void *d = ar.take(nbytes);
-    if (nbytes && hipMemcpyAsync(d, src, nbytes, hipMemcpyHostToDevice, s) != hipSuccess)
-      rc = KS_ERR_DEVICE;
+    if (nbytes) {
+      hipError_t e = hipMemcpyAsync(d, src, nbytes, hipMemcpyHostToDevice, s);
+      if (e != hipSuccess) rc = hip_failure(e, "hipMemcpyAsync H2D");
+    }
     return d;
   };
   rc = KS_OK;
@@ -192,11 +197,15 @@ void run_staged(const ks_t *kernel, int m, int n, int k, double *u, const int *u
     }
   }
   if (rc == KS_OK) rc = reference_algorithm ? enqueue_reference(p, s) : enqueue_fused(p, s);
-  if (rc == KS_OK &&
-      hipMemcpyAsync(upack.data(), du, (size_t)m * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
-    rc = KS_ERR_DEVICE;
+  if (rc == KS_OK) {
+    hipError_t e = hipMemcpyAsync(upack.data(), du, (size_t)m * 8, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) rc = hip_failure(e, "hipMemcpyAsync D2H");
+  }
   (void)hipFreeAsync(ar.base, s);
-  if (hipStreamSynchronize(s) != hipSuccess && rc == KS_OK) rc = KS_ERR_DEVICE;
+  {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess && rc == KS_OK) rc = hip_failure(e, "hipStreamSynchronize");
+  }
   if (rc == KS_OK)
     for (int i = 0; i < m; ++i) u[umap[i]] = upack[i];
   finish(rc, name);
