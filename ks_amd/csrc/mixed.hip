@@ -1,32 +1,44 @@
 // Mixed-precision fused summation (Gaussian and variable-bandwidth Gaussian):
 //   u[umap[i]] += sum_j exp(x_ij) w[wmap[j]],  x = scal*d^2  or  -0.5 d^2 hi hj.
 //
-// Three changes against the FP64 drop-in, each where it is safe:
-//   1. the cross term a.b runs on the BF16 matrix cores as a split-BF16
-//      three-product, a ~= ah + al and b ~= bh + bl (bf16 each),
-//      a.b ~= ah.bh + ah.bl + al.bh, accumulated in FP32 by
-//      v_mfma_f32_16x16x32_bf16 -- 3 instructions per 32 coordinates of a
-//      16 x 16 pair block (KS_PREC_MIXED_BF16X3);  or on the FP32 matrix cores,
-//      v_mfma_f32_16x16x4f32, 8 instructions per 32 coordinates
-//      (KS_PREC_MIXED_FP32);
-//   2. that estimate classifies every pair: a pair whose estimated exponent
-//      lies below -kFarT (K < ~1e-7) is "far" and its kernel value is taken in
-//      FP32 (v_exp_f32) from the estimate; its error is bounded by the
-//      estimate's error times K, far below 1e-6 of u;
-//   3. a "near" pair -- the pairs that carry u, and the ones where the
-//      |a|^2 + |b|^2 - 2 a.b expansion cancels -- is pushed to a per-wave LDS
-//      queue and recomputed in FP64 from the coordinates (FP64 FMA dot
-//      product, FP64 norms, FP64 exp) 64 pairs at a time, one pair per lane.
-// u accumulates in FP64 across source tiles (FP32 only inside one 16 x 16
-// block of far pairs).
+// Every pair is evaluated; what changes against the FP64 drop-in is where:
+//   1. an estimate of every cross term a.b runs on the BF16 matrix cores as a
+//      split-BF16 three-product (a ~= ah + al, b ~= bh + bl, bf16 each;
+//      a.b ~= ah.bh + ah.bl + al.bh accumulated in FP32 by
+//      v_mfma_f32_16x16x32_bf16, 3 instructions per 32 coordinates of a 16 x 16
+//      pair block: KS_PREC_MIXED_BF16X3), or on the FP32 matrix cores
+//      (v_mfma_f32_16x16x4f32, 8 per 32 coordinates: KS_PREC_MIXED_FP32);
+//   2. a 16 x 16 block whose estimated exponents all lie below -kFarT
+//      (K < ~1e-7 for every pair) is "far": its kernel values are taken in FP32
+//      (v_exp_f32) from the estimate, whose error is bounded by the estimate's
+//      error times K, far below 1e-6 of u;
+//   3. any other block is "near": its cross terms are recomputed exactly on the
+//      FP64 matrix cores (v_mfma_f64_16x16x4f64 from FP64 coordinates) and its
+//      kernel values evaluated in FP64 with FP64 norms, so the |a|^2 + |b|^2 -
+//      2 a.b cancellation of close pairs never meets an FP32 rounding.
+// u accumulates in FP64 across source blocks (FP32 only within kFlush far
+// blocks of one row).
 //
-// Packing: one pre-pass converts the target rows (through amap) and the source
-// points (through bmap) to the MFMA operand layout once per call -- for BF16x3
-// each point is nc chunks of 32 coordinates, each chunk 4 groups x (8 hi + 8 lo
-// bf16); for FP32 each point is nc chunks x 4 groups x 8 floats -- plus per-row
-// and per-column FP32 scale factors and weights.  The main kernel streams the
-// packed sources; A fragments stay in VGPRs.
+// Locality.  Near pairs only fill whole 16 x 16 blocks when close points sit
+// next to each other in the block order.  A pre-pass therefore buckets the
+// targets and the sources by their nearest of up to kPivots pivot points (FP32
+// distances; the pivots are targets spaced evenly through amap) and orders both
+// by bucket with a stable radix sort, so that the points of one cluster form
+// contiguous runs.  The order only decides which pairs share a block; each
+// target's sum is the same set of terms, accumulated in source-block order.
+//
+// Packing: the pre-pass writes the MFMA operands in sorted order, in blocks of
+// 16 points.  BF16 / FP32 operand: block b, chunk c (32 coordinates), part h is
+// 64 lanes x 16 B, lane 16 g + p holding point 16 b + p, coordinates 32 c + 8 g
+// .. + 7 (BF16x3: part 0 the 8 hi, part 1 the 8 lo bf16; FP32: part h the
+// floats 4h .. 4h + 3).  FP64 source operand: block b, chunk c (16
+// coordinates) is 64 lanes x 32 B, lane 16 g + p holding point 16 b + p,
+// coordinates 16 c + 4 g .. + 3.  One fragment load is then contiguous across
+// the wave.  Per-point FP32 and FP64 factors (norms, bandwidths, weights) go
+// beside them; points past m / n are zero with a -inf exponent and weight 0,
+// so the kernel never clamps an index.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include "gsks_kernels.h"
 #include "gsks_launch.h"
@@ -35,72 +47,167 @@
 namespace gsks {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr float kFarT = 16.0f;          // far: estimated x < -16 (natural-log units)
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr int XWG = 256;                // 4 waves
-constexpr int kQueue = 512;             // per-wave near-pair ring (power of two)
+constexpr int kPivots = 128;            // bucketing pivots (at most)
+constexpr int kSortMin = 2048;          // bucket only when m and n are both at least this
+constexpr int kMixRB = 2;               // 16-row blocks per wave
+constexpr int kMixRowsPerWG = 4 * kMixRB * 16;
 
 // ------------------------------------------------------------------ pre-pass
 
 struct MixPack {
-  // packed operands, 128 B per 32-coordinate chunk per point (both precisions)
-  uint4 *Ap;   // [m][nc][4 groups][32 B]
-  uint4 *Bp;   // [n][nc][4 groups][32 B]
-  float2 *ra;  // per target row: {scaled |a|^2 (Gaussian) or |a|^2 (VarBW), hi factor}
-  float4 *cb;  // per source: {scaled |b|^2 or |b|^2, hj factor, w, 0}
+  uint4 *Ap;        // BF16 / FP32 target operand [mp/16][nc][2 parts][64 lanes]
+  uint4 *Bp;        // BF16 / FP32 source operand [np/16][nc][2 parts][64 lanes]
+  double4 *Bd;      // FP64 source operand [np/16][2 nc][64 lanes]
+  float2 *ra;       // per target: {-|a|^2 / 2, -hi log2(e) / 2 (VarBW)}
+  float4 *cb;       // per source: {xs |b|^2 (Gaussian) or |b|^2, hj, w, 0}
+  double2 *rd;      // per target: {|a|^2, -hi / 2}
+  double4 *cd;      // per source: {|b|^2, w, hj, 0}
+  const int *prow;  // sorted target position -> target index of the call
+  const int *pcol;  // sorted source position -> source index of the call
 };
 
+// FP32 copies of the pivots: pivot i is target (i * m) / P, zero past k.
+__global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P * kp) return;
+  const int i = t / kp, q = t - i * kp;
+  const int r = (int)(((long)i * p.m) / P);
+  piv[t] = q < p.k ? (float)p.XA[(long)p.amap[r] * p.k + q] : 0.0f;
+}
+
+// Bucket (nearest pivot, FP32 squared distance) of every target, then every source.
+template <int KP>
+__global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p, int P,
+                                                         const float *__restrict__ piv,
+                                                         unsigned *key_a, unsigned *key_b) {
+  __shared__ float sp[kPivots * KP];
+  for (int e = threadIdx.x; e < P * KP; e += 256) sp[e] = piv[e];
+  __syncthreads();
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)p.m + p.n) return;
+  const double *x = t < p.m ? p.XA + (long)p.amap[t] * p.k : p.XB + (long)p.bmap[t - p.m] * p.k;
+  float xv[KP];
+#pragma unroll
+  for (int q = 0; q < KP; ++q) xv[q] = q < p.k ? (float)x[q] : 0.0f;
+  float best = INFINITY;
+  unsigned bi = 0;
+  for (int i = 0; i < P; ++i) {
+    const float *pv = sp + i * KP;
+    float d = 0.0f;
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+      const float e = xv[q] - pv[q];
+      d = fmaf(e, e, d);
+    }
+    if (d < best) {
+      best = d;
+      bi = (unsigned)i;
+    }
+  }
+  if (t < p.m) key_a[t] = bi;
+  else key_b[t - p.m] = bi;
+}
+
+__global__ void mix_identity_kernel(int *v, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (int)i;
+}
+
+// Writes point `pt` (coordinates x, or zeros when x is null) into the blocked
+// BF16 / FP32 layout, and with Bd non-null into the FP64 source layout too.
 template <int PREC>
-__device__ __forceinline__ void pack_point(const double *__restrict__ x, int k, int nc, uint4 *out) {
+__device__ __forceinline__ void pack_point(const double *__restrict__ x, int k, int nc, long pt,
+                                           uint4 *pack, double4 *Bd) {
+  uint4 *blk = pack + (pt >> 4) * nc * 128;
+  const int p16 = (int)(pt & 15);
   for (int c = 0; c < nc; ++c) {
     for (int g = 0; g < 4; ++g) {
+      uint4 *o0 = blk + (c * 2) * 64 + g * 16 + p16;
+      uint4 *o1 = o0 + 64;
       if constexpr (PREC == KS_PREC_MIXED_BF16X3) {
         bf8 hi, lo;
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
           int idx = c * 32 + 8 * g + t;
-          double v = idx < k ? x[idx] : 0.0;
+          double v = (x && idx < k) ? x[idx] : 0.0;
           __bf16 h = (__bf16)(float)v;
           hi[t] = h;
           lo[t] = (__bf16)(float)(v - (double)(float)h);
         }
-        uint4 *o = out + (c * 4 + g) * 2;
-        o[0] = *reinterpret_cast<uint4 *>(&hi);
-        o[1] = *reinterpret_cast<uint4 *>(&lo);
+        *o0 = *reinterpret_cast<uint4 *>(&hi);
+        *o1 = *reinterpret_cast<uint4 *>(&lo);
       } else {
         float f[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
           int idx = c * 32 + 8 * g + t;
-          f[t] = idx < k ? (float)x[idx] : 0.0f;
+          f[t] = (x && idx < k) ? (float)x[idx] : 0.0f;
         }
-        uint4 *o = out + (c * 4 + g) * 2;
-        o[0] = *reinterpret_cast<uint4 *>(&f[0]);
-        o[1] = *reinterpret_cast<uint4 *>(&f[4]);
+        *o0 = *reinterpret_cast<uint4 *>(&f[0]);
+        *o1 = *reinterpret_cast<uint4 *>(&f[4]);
       }
     }
   }
+  if (Bd) {
+    double4 *dblk = Bd + (pt >> 4) * (2 * nc) * 64;
+    for (int c = 0; c < 2 * nc; ++c)
+      for (int g = 0; g < 4; ++g) {
+        double v[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          int idx = c * 16 + 4 * g + t;
+          v[t] = (x && idx < k) ? x[idx] : 0.0;
+        }
+        dblk[c * 64 + g * 16 + p16] = make_double4(v[0], v[1], v[2], v[3]);
+      }
+  }
 }
 
+// One thread per point of the padded target range [0, mp) then source range [0, np),
+// both in sorted order.
 template <int PREC, int TYPE>
-__global__ void mixed_pack_kernel(const Problem p, int nc, float xs, MixPack pk) {
+__global__ void mixed_pack_kernel(const Problem p, int nc, float xs, long mp, long np, MixPack pk) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < p.m) {
-    int ai = p.amap[t];
-    pack_point<PREC>(p.XA + (long)ai * p.k, p.k, nc, pk.Ap + t * nc * 8);
-    double aa = p.XA2[ai];
-    if (TYPE == GAUSSIAN) pk.ra[t] = make_float2((float)(aa * (double)xs), 0.0f);
-    else pk.ra[t] = make_float2((float)aa, (float)(p.hi[ai] * -0.5 * (double)kLog2e));
-  } else if (t < (long)p.m + p.n) {
-    long j = t - p.m;
-    int bj = p.bmap[j];
-    pack_point<PREC>(p.XB + (long)bj * p.k, p.k, nc, pk.Bp + j * nc * 8);
-    double bb = p.XB2[bj];
-    float w = (float)p.w[p.wmap[j]];
-    if (TYPE == GAUSSIAN) pk.cb[j] = make_float4((float)(bb * (double)xs), 0.0f, w, 0.0f);
-    else pk.cb[j] = make_float4((float)bb, (float)p.hj[bj], w, 0.0f);
+  if (t < mp) {
+    if (t < p.m) {
+      const int ai = p.amap[pk.prow[t]];
+      pack_point<PREC>(p.XA + (long)ai * p.k, p.k, nc, t, pk.Ap, nullptr);
+      const double aa = p.XA2[ai];
+      const double hi = (TYPE == VAR_BANDWIDTH) ? p.hi[ai] : 0.0;
+      pk.ra[t] = make_float2((float)(-0.5 * aa), (float)(hi * -0.5 * (double)kLog2e));
+      pk.rd[t] = make_double2(aa, -0.5 * hi);
+    } else {  // padding target: its estimated exponent is -inf, so it is never near
+      pack_point<PREC>(nullptr, p.k, nc, t, pk.Ap, nullptr);
+      // Gaussian x2 = -2 xs acc + ...: acc = -inf gives -inf for xs < 0 (a
+      // decaying kernel); for xs > 0 the row starts at +inf instead
+      const float pad = (TYPE == GAUSSIAN && xs > 0.0f) ? INFINITY : -INFINITY;
+      pk.ra[t] = make_float2(pad, -1.0f);
+      pk.rd[t] = make_double2(0.0, 0.0);
+    }
+  } else if (t < mp + np) {
+    const long j = t - mp;
+    if (j < p.n) {
+      const int jj = pk.pcol[j];
+      const int bj = p.bmap[jj];
+      pack_point<PREC>(p.XB + (long)bj * p.k, p.k, nc, j, pk.Bp, pk.Bd);
+      const double bb = p.XB2[bj];
+      const double w = p.w[p.wmap[jj]];
+      const double hj = (TYPE == VAR_BANDWIDTH) ? p.hj[bj] : 0.0;
+      if (TYPE == GAUSSIAN) pk.cb[j] = make_float4((float)(bb * (double)xs), 0.0f, (float)w, 0.0f);
+      else pk.cb[j] = make_float4((float)bb, (float)hj, (float)w, 0.0f);
+      pk.cd[j] = make_double4(bb, w, hj, 0.0);
+    } else {  // padding source: exponent -inf, weight 0
+      pack_point<PREC>(nullptr, p.k, nc, j, pk.Bp, pk.Bd);
+      pk.cb[j] = (TYPE == GAUSSIAN) ? make_float4(-INFINITY, 0.0f, 0.0f, 0.0f)
+                                    : make_float4(INFINITY, 1.0f, 0.0f, 0.0f);
+      pk.cd[j] = make_double4(0.0, 0.0, 1.0, 0.0);
+    }
   }
 }
 
@@ -111,19 +218,19 @@ struct Frag {
   uint4 v[NC][2];  // per chunk: 32 B (bf16: hi | lo, fp32: floats 0-3 | 4-7)
 };
 
+// The fragment of one 16-point block: each load is 64 lanes x 16 B contiguous.
 template <int PREC, int NC>
-__device__ __forceinline__ void load_frag_packed(const uint4 *__restrict__ base, int g,
+__device__ __forceinline__ void load_frag_packed(const uint4 *__restrict__ blk, int lane,
                                                  Frag<PREC, NC> &f) {
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    f.v[c][0] = base[(c * 4 + g) * 2];
-    f.v[c][1] = base[(c * 4 + g) * 2 + 1];
+    f.v[c][0] = blk[(c * 2) * 64 + lane];
+    f.v[c][1] = blk[(c * 2 + 1) * 64 + lane];
   }
 }
 
 template <int PREC, int NC>
-__device__ __forceinline__ f4 cross_block(const Frag<PREC, NC> &a, const Frag<PREC, NC> &b) {
-  f4 acc = {0.f, 0.f, 0.f, 0.f};
+__device__ __forceinline__ f4 cross_block(const Frag<PREC, NC> &a, const Frag<PREC, NC> &b, f4 acc) {
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     if constexpr (PREC == KS_PREC_MIXED_BF16X3) {
@@ -144,25 +251,39 @@ __device__ __forceinline__ f4 cross_block(const Frag<PREC, NC> &a, const Frag<PR
   return acc;
 }
 
-// FP64 kernel value times weight of one near pair from its exact rank-k sum c.
-template <int TYPE>
-__device__ __forceinline__ double near_value(const Problem &p, int row, int col, double c) {
-  const int ai = p.amap[row], bj = p.bmap[col];
-  double d2 = fmax(fma(-2.0, c, p.XA2[ai]) + p.XB2[bj], 0.0);
-  double x = (TYPE == GAUSSIAN) ? d2 * p.P.scal : (d2 * (-0.5 * p.hi[ai])) * p.hj[bj];
-  return exp(x) * p.w[p.wmap[col]];
-}
-
 // Row of the 16 x 16 result block held in accumulator slot v by lane group g.
 // Measured on gfx950: v_mfma_f32_16x16x32_bf16 and v_mfma_f32_16x16x4f32 put
-// rows 4g + v of column lane%16 in lane g*16 + col (the FP64 16x16x4 shape
-// uses g + 4v instead).
+// rows 4g + v of column lane%16 in lane g*16 + col; v_mfma_f64_16x16x4f64 puts
+// A-operand row g + 4v there instead, so the FP64 A operand of lane p carries
+// target row 4 (p % 4) + p / 4, which lands FP64 results on the same rows.
 __device__ __forceinline__ int out_row(int g, int v) { return 4 * g + v; }
+__device__ __forceinline__ int f64_a_row(int p16) { return 4 * (p16 & 3) + (p16 >> 2); }
+
+// One DPP lane exchange of a double (two 32-bit moves), all lanes valid.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  int2 v = __builtin_bit_cast(int2, x);
+  v.x = __builtin_amdgcn_mov_dpp(v.x, CTRL, 0xF, 0xF, true);
+  v.y = __builtin_amdgcn_mov_dpp(v.y, CTRL, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, v);
+}
+
+// Sum over the 16 lanes of a DPP row; every lane gets the same value.
+// Butterflies: swap neighbours, swap pairs, mirror the 8-lane halves, mirror the row.
+__device__ __forceinline__ double row_sum16(double x) {
+  x += dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+  x += dpp_f64<0x141>(x);  // row_half_mirror
+  x += dpp_f64<0x140>(x);  // row_mirror
+  return x;
+}
 
 constexpr int kFlush = 8;  // far-pair FP32 partials go to FP64 every 8 source blocks
 
-template <int PREC, int TYPE, int NC, int RB>
+template <int PREC, int TYPE, int NC>
 __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, float xs) {
+  constexpr int RB = kMixRB;
+  constexpr int KC = 2 * NC;  // 16-coordinate chunks of the FP64 operand
   const Problem &p = A.p;
   const int m = p.m;
   const int lane = threadIdx.x & 63;
@@ -173,159 +294,123 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
   const int split = blockIdx.y;
   constexpr int WROWS = RB * 16;
   const int wg_rows = 4 * WROWS;
-  const int row0 = rblk * wg_rows + wave * WROWS;
+  const int row0 = rblk * wg_rows + wave * WROWS;  // sorted target position
 
-  __shared__ int s_queue[4][kQueue];
-  __shared__ double s_near[4][WROWS];
-  // the wave's target rows in FP64 for the near-pair recompute: each is reused
-  // by every near pair of its row, so only the source side is gathered later
-  // (row stride 32*NC + 2 doubles keeps the 16 lanes' 32-byte reads apart)
+  // the wave's targets in FP64 (the A operand of near blocks), row stride
+  // 32 NC + 2 doubles so the 16 rows of one FP64 fragment load fall on
+  // different banks; zero past k and for padding rows
   constexpr int KS = 32 * NC + 2;
   __shared__ double s_rows[4][WROWS * KS];
-  int *queue = s_queue[wave];
-  double *nacc = s_near[wave];
+  __shared__ double s_tot[4][WROWS];
   double *arows = s_rows[wave];
-  for (int q = lane; q < WROWS; q += 64) nacc[q] = 0.0;
   for (int e = lane; e < WROWS * 32 * NC; e += 64) {
     const int rl = e / (32 * NC), q = e - rl * (32 * NC);
     const int r = row0 + rl;
-    arows[rl * KS + q] = q < p.k ? p.XA[(long)p.amap[r < m ? r : m - 1] * p.k + q] : 0.0;
+    arows[rl * KS + q] = (q < p.k && r < m) ? p.XA[(long)p.amap[pk.prow[r]] * p.k + q] : 0.0;
   }
 
+  // BF16 / FP32 accumulators start at -|a|^2/2 (the packed row factor), so the
+  // block result is c - |a|^2/2 and the exponent is one FMA away: Gaussian
+  // x2 = -2 xs acc + xs |b|^2, variable bandwidth d^2 = -2 acc + |b|^2.
   Frag<PREC, NC> af[RB];
-  float ra_s[RB][4], ra_h[RB][4], facc[RB][4];
-  double racc[RB][4];
+  f4 cinit[RB];
+  float ra_h[RB][4], facc[RB][4];
+  double racc[RB][4], aa[RB][4], hi[RB][4];
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) {
-    int r = row0 + rb * 16 + p16;
-    load_frag_packed<PREC, NC>(pk.Ap + (long)(r < m ? r : m - 1) * NC * 8, g, af[rb]);
+    load_frag_packed<PREC, NC>(pk.Ap + (long)((row0 >> 4) + rb) * NC * 128, lane, af[rb]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      int ro = row0 + rb * 16 + out_row(g, v);
-      float2 t = pk.ra[ro < m ? ro : m - 1];
-      // a row past m can never be near: its estimated exponent is -inf
-      if (ro >= m) t.x = (TYPE == GAUSSIAN) ? -INFINITY : INFINITY;
-      ra_s[rb][v] = t.x;
-      ra_h[rb][v] = t.y;
+      const int t = row0 + rb * 16 + out_row(g, v);
+      const float2 f = pk.ra[t];
+      const double2 d = pk.rd[t];
+      cinit[rb][v] = f.x;
+      ra_h[rb][v] = f.y;
+      aa[rb][v] = d.x;
+      hi[rb][v] = d.y;
       racc[rb][v] = 0.0;
       facc[rb][v] = 0.0f;
     }
   }
+  __syncthreads();  // the staged rows are visible to every lane of the wave
+  const float m2xs = -2.0f * xs;
+  const float thr = -kFarT * kLog2e;
+  const KerParams P = p.P;
   const int j0 = split * A.cols_per_split;
   const int j1 = min(p.n, j0 + A.cols_per_split);
-  unsigned head = 0, tail = 0;  // wave-uniform ring indices
 
-  Frag<PREC, NC> bf, bn;
-  float4 cb = make_float4(0.f, 0.f, 0.f, 0.f), cbn = cb;
+  // FP64 source fragment of the block being (or about to be) recomputed exactly
+  double bd[KC][4];
+  double4 cdv = make_double4(0.0, 0.0, 0.0, 0.0);
+  int bd_block = -1;  // source block whose FP64 fragment bd holds (or has in flight)
+  auto fetch64 = [&](int jb) {
+    const double4 *src = pk.Bd + (long)(jb >> 4) * KC * 64 + lane;
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      const double4 q = src[c * 64];
+      bd[c][0] = q.x;
+      bd[c][1] = q.y;
+      bd[c][2] = q.z;
+      bd[c][3] = q.w;
+    }
+    cdv = pk.cd[jb + p16];
+    bd_block = jb;
+  };
+
+  // source blocks start on multiples of 16 and a slice ends on one or at n, so
+  // a block's columns past j1 are padding of the packed arrays
   auto fetch = [&](int jb, Frag<PREC, NC> &f, float4 &c4) {
-    int col = jb + p16;
-    bool ok = col < j1;
-    load_frag_packed<PREC, NC>(pk.Bp + (long)(ok ? col : j1 - 1) * NC * 8, g, f);
-    c4 = pk.cb[ok ? col : j1 - 1];
-    if (!ok) {  // a column past the slice: exponent -inf, weight 0
-      c4.x = (TYPE == GAUSSIAN) ? -INFINITY : INFINITY;
-      c4.z = 0.0f;
-    }
-  };
-  // Near pairs, up to 64 per call: 16 rounds of 4 pairs x 16 lanes.  In round r
-  // the 16 lanes of group s sum pair 4r + s, each over 2*NC consecutive
-  // coordinates (contiguous 16-byte loads of both rows), reduce across the 16
-  // lanes, and lane 16s + r keeps the sum; afterwards every lane holds one
-  // pair's exact rank-k sum and evaluates its kernel value in FP64.
-  auto drain = [&](unsigned count) {
-    const int grp = lane >> 4, sub = lane & 15;
-    constexpr int CPL = 2 * NC;  // coordinates per lane (k <= 32*NC)
-    double mine_c = 0.0;
-    int mine_e = -1;
-    // 4 rounds at a time: all their loads are issued before the first FMA
-#pragma unroll 1
-    for (int r0 = 0; r0 < 16; r0 += 4) {
-      if (4 * r0 >= (int)count) break;
-      double2 av[4][CPL / 2], bv[4][CPL / 2];
-      int ev[4];
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int idx = 4 * (r0 + rr) + grp;
-        const int e = idx < (int)count ? queue[(head + idx) & (kQueue - 1)] : -1;
-        ev[rr] = e;
-        const int rl = (e >= 0 ? e : 0) & 31, col = e >= 0 ? (e >> 5) : j0;
-        const double *b = p.XB + (long)p.bmap[col] * p.k + sub * CPL;
-        const double *a = arows + rl * KS + sub * CPL;  // zero-padded past k
-#pragma unroll
-        for (int t = 0; t < CPL / 2; ++t) {
-          const bool in = (e >= 0) && (sub * CPL + 2 * t < p.k);
-          av[rr][t] = *reinterpret_cast<const double2 *>(a + 2 * t);
-          if (A.vec2) {
-            bv[rr][t] = in ? reinterpret_cast<const double2 *>(b)[t] : make_double2(0.0, 0.0);
-          } else {
-            const bool in1 = in && (sub * CPL + 2 * t + 1 < p.k);
-            bv[rr][t] = make_double2(in ? b[2 * t] : 0.0, in1 ? b[2 * t + 1] : 0.0);
-          }
-        }
-      }
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        double c = 0.0;
-#pragma unroll
-        for (int t = 0; t < CPL / 2; ++t) {
-          c = fma(av[rr][t].x, bv[rr][t].x, c);
-          c = fma(av[rr][t].y, bv[rr][t].y, c);
-        }
-        c += __shfl_xor(c, 8, 16);
-        c += __shfl_xor(c, 4, 16);
-        c += __shfl_xor(c, 2, 16);
-        c += __shfl_xor(c, 1, 16);
-        if (sub == r0 + rr) {
-          mine_c = c;
-          mine_e = ev[rr];
-        }
-      }
-    }
-    if (mine_e >= 0) {
-      const int rl = mine_e & 31, col = mine_e >> 5;
-      atomicAdd(&nacc[rl], near_value<TYPE>(p, row0 + rl, col, mine_c));
-    }
-    head += count;
+    load_frag_packed<PREC, NC>(pk.Bp + (long)(jb >> 4) * NC * 128, lane, f);
+    c4 = pk.cb[jb + p16];
   };
 
-  if (j0 < j1) fetch(j0, bf, cb);
+  // One 16-source block against the wave's RB row blocks.
   int nblk = 0;
-  for (int jb = j0; jb < j1; jb += 16) {
-    const bool more = jb + 16 < j1;
-    if (more) fetch(jb + 16, bn, cbn);
+  auto block = [&](int jb, int jnext, const Frag<PREC, NC> &bf, const float4 &cb) {
+    bool used64 = false;
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
-      f4 c = cross_block<PREC, NC>(af[rb], bf);
-      unsigned long long nmask[4];
+      const f4 c = cross_block<PREC, NC>(af[rb], bf, cinit[rb]);
+      float x2[4];
+      bool nr = false;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        float x2;  // estimated exponent, log2 units (-inf for padding)
         if constexpr (TYPE == GAUSSIAN) {
-          x2 = fmaf(-2.0f * xs, c[v], ra_s[rb][v] + cb.x);
+          x2[v] = fmaf(m2xs, c[v], cb.x);
         } else {
-          float d2 = fmaxf(fmaf(-2.0f, c[v], ra_s[rb][v] + cb.x), 0.0f);
-          x2 = d2 * (ra_h[rb][v] * cb.y);
+          x2[v] = fmaxf(fmaf(-2.0f, c[v], cb.x), 0.0f) * (ra_h[rb][v] * cb.y);
         }
-        const bool nr = x2 > -kFarT * kLog2e;
-        const float K = __builtin_amdgcn_exp2f(x2);
-        facc[rb][v] = fmaf(nr ? 0.0f : K, cb.z, facc[rb][v]);
-        nmask[v] = __ballot(nr);
+        nr = nr || (x2[v] > thr);
       }
-      // queue this row block's near pairs (row-local index in the low 5 bits);
-      // the ring holds < 64 + 4*64 entries at any time
+      if (__ballot(nr) == 0ull) {
+        // far block: FP32 kernel values from the estimate
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const unsigned long long mask = nmask[v];
-        if (mask) {
-          if ((mask >> lane) & 1ull) {
-            const unsigned off = __popcll(mask & ((1ull << lane) - 1ull));
-            queue[(tail + off) & (kQueue - 1)] = ((jb + p16) << 5) | (rb * 16 + out_row(g, v));
-          }
-          tail += (unsigned)__popcll(mask);
+        for (int v = 0; v < 4; ++v) facc[rb][v] = fmaf(__builtin_amdgcn_exp2f(x2[v]), cb.z, facc[rb][v]);
+      } else {
+        // near block: exact FP64 cross terms on the matrix cores, FP64 kernel values
+        if (bd_block != jb) fetch64(jb);
+        used64 = true;
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+        const double *ar = arows + (rb * 16 + f64_a_row(p16)) * KS + 4 * g;
+#pragma unroll
+        for (int cc = 0; cc < KC; ++cc) {
+          const double2 a01 = *reinterpret_cast<const double2 *>(ar + cc * 16);
+          const double2 a23 = *reinterpret_cast<const double2 *>(ar + cc * 16 + 2);
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.x, bd[cc][0], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.y, bd[cc][1], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.x, bd[cc][2], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.y, bd[cc][3], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double K = kval_f64<TYPE>(acc[v], aa[rb][v], cdv.x, hi[rb][v], cdv.z, P);
+          racc[rb][v] = fma(K, cdv.y, racc[rb][v]);
         }
       }
-      while (tail - head >= 64) drain(64);
     }
+    // near blocks come in runs once the points are bucketed: after one, start
+    // the next block's FP64 fragment loads
+    if (used64 && jnext != jb) fetch64(jnext);
     if (++nblk == kFlush) {
       nblk = 0;
 #pragma unroll
@@ -336,47 +421,44 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
           facc[rb][v] = 0.0f;
         }
     }
-    if (more) {
-      bf = bn;
-      cb = cbn;
+  };
+
+  // two fragment buffers in turn: the next block's loads are in flight while
+  // this block computes, with no register copies between them (the last
+  // block's "next" is itself again, an in-bounds re-read)
+  Frag<PREC, NC> b0, b1;
+  float4 c0, c1;
+  const int jlast = j0 + ((j1 - j0 - 1) >> 4) * 16;
+  if (j0 < j1) {
+    fetch(j0, b0, c0);
+    for (int jb = j0;; jb += 32) {
+      const int jn1 = min(jb + 16, jlast), jn2 = min(jb + 32, jlast);
+      fetch(jn1, b1, c1);
+      block(jb, jn1, b0, c0);
+      if (jb + 16 >= j1) break;
+      fetch(jn2, b0, c0);
+      block(jb + 16, jn2, b1, c1);
+      if (jb + 32 >= j1) break;
     }
   }
-  while (tail != head) drain(tail - head < 64 ? tail - head : 64);
 
-  // ---- row totals: far partials summed over the 16 column lanes + near sums
+  // ---- row totals: partials summed over the 16 column lanes, one writer lane per row
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      double s = racc[rb][v] + (double)facc[rb][v];
-      s += __shfl_xor(s, 8, 16);
-      s += __shfl_xor(s, 4, 16);
-      s += __shfl_xor(s, 2, 16);
-      s += __shfl_xor(s, 1, 16);
-      racc[rb][v] = s;
+      const double s = row_sum16(racc[rb][v] + (double)facc[rb][v]);
+      if (p16 == 0) s_tot[wave][rb * 16 + out_row(g, v)] = s;
     }
-  const bool writer = p16 < 4;
-  double mine[RB];
-  int myrow[RB];
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb) {
-    double x = racc[rb][0];
-    x = (p16 == 1) ? racc[rb][1] : x;
-    x = (p16 == 2) ? racc[rb][2] : x;
-    x = (p16 == 3) ? racc[rb][3] : x;
-    const int rl = rb * 16 + out_row(g, p16 & 3);
-    mine[rb] = x + (writer ? nacc[rl] : 0.0);
-    myrow[rb] = row0 + rl;
-  }
+  __syncthreads();
+  const bool writer = lane < WROWS;
+  const int myrow = row0 + (writer ? lane : 0);
+  const double mine = writer ? s_tot[wave][lane] : 0.0;
   if (A.nsplit == 1) {
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-      if (writer && myrow[rb] < m) p.u[p.umap[myrow[rb]]] += mine[rb];
+    if (writer && myrow < m) p.u[p.umap[pk.prow[myrow]]] += mine;
     return;
   }
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
-    if (writer && myrow[rb] < m) A.partial[(long)split * m + myrow[rb]] = mine[rb];
+  if (writer && myrow < m) A.partial[(long)split * m + myrow] = mine;
   __threadfence();
   __syncthreads();
   __shared__ int s_last;
@@ -392,7 +474,7 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
     if (r < m) {
       double s = 0.0;
       for (int sp = 0; sp < A.nsplit; ++sp) s += A.partial[(long)sp * m + r];
-      p.u[p.umap[r]] += s;
+      p.u[p.umap[pk.prow[r]]] += s;
     }
   }
   if (threadIdx.x == 0) A.counters[rblk] = 0u;
@@ -400,14 +482,11 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
 
 // ------------------------------------------------------------------ launch
 
-constexpr int kMixRB = 2;
-constexpr int kMixRowsPerWG = 4 * kMixRB * 16;
-
 template <int PREC, int TYPE, int NC>
 static hipError_t mixed_go(const FusedArgs &A, const MixPack &pk, float xs, int row_blocks,
                            hipStream_t s) {
   dim3 grid(row_blocks, A.nsplit);
-  hipLaunchKernelGGL((mixed_kernel<PREC, TYPE, NC, kMixRB>), grid, dim3(XWG), 0, s, A, pk, xs);
+  hipLaunchKernelGGL((mixed_kernel<PREC, TYPE, NC>), grid, dim3(XWG), 0, s, A, pk, xs);
   return hipGetLastError();
 }
 
@@ -431,16 +510,26 @@ static hipError_t mixed_launch(const Problem &p, const FusedPlan &plan, double *
   A.nsplit = plan.nsplit;
   A.partial = partial;
   A.counters = counters;
-  A.vec2 = (p.k % 2 == 0) && ((uintptr_t)p.XA % 16 == 0) && ((uintptr_t)p.XB % 16 == 0);
+  A.vec2 = 0;
   const float xs = (float)(p.P.scal * (double)kLog2e);
-  long tot = (long)p.m + p.n;
-  dim3 pg((unsigned)((tot + 255) / 256));
+  const long mp = (long)plan.row_blocks * kMixRowsPerWG, np = ((long)p.n + 15) / 16 * 16;
+  dim3 pg((unsigned)((mp + np + 255) / 256));
   if (p.type == GAUSSIAN) {
-    hipLaunchKernelGGL((mixed_pack_kernel<PREC, GAUSSIAN>), pg, dim3(256), 0, s, p, nc, xs, pk);
+    hipLaunchKernelGGL((mixed_pack_kernel<PREC, GAUSSIAN>), pg, dim3(256), 0, s, p, nc, xs, mp, np,
+                       pk);
     return mixed_nc<PREC, GAUSSIAN>(A, pk, xs, nc, plan.row_blocks, s);
   }
-  hipLaunchKernelGGL((mixed_pack_kernel<PREC, VAR_BANDWIDTH>), pg, dim3(256), 0, s, p, nc, xs, pk);
+  hipLaunchKernelGGL((mixed_pack_kernel<PREC, VAR_BANDWIDTH>), pg, dim3(256), 0, s, p, nc, xs, mp,
+                     np, pk);
   return mixed_nc<PREC, VAR_BANDWIDTH>(A, pk, xs, nc, plan.row_blocks, s);
+}
+
+template <int KP>
+static void launch_bucket(const Problem &p, int P, const float *piv, unsigned *ka, unsigned *kb,
+                          hipStream_t s) {
+  const long tot = (long)p.m + p.n;
+  hipLaunchKernelGGL((mix_bucket_kernel<KP>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                     p, P, piv, ka, kb);
 }
 
 int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
@@ -455,26 +544,91 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
     return raise_error(KS_ERR_INVALID_ARG, false,
                        "dgsks_device(): mixed precision supports k <= 128\n");
   FusedPlan plan = plan_split(p.m, p.n, kMixRowsPerWG, 2048, 128);
-  // scratch: partial sums | counters | packed operands | per-row / per-column factors
-  size_t packA = (size_t)p.m * nc * 128, packB = (size_t)p.n * nc * 128;
-  size_t ra = (size_t)p.m * sizeof(float2), cbb = (size_t)p.n * sizeof(float4);
-  size_t extra = packA + packB + ra + cbb + 4 * 256;
+  // source slices start on 16-point blocks of the packed layout
+  plan.cols_per_split = (plan.cols_per_split + 15) / 16 * 16;
+  plan.nsplit = (p.n + plan.cols_per_split - 1) / plan.cols_per_split;
+  const bool bucket = p.m >= kSortMin && p.n >= kSortMin;
+  const int P = bucket ? kPivots : 0;
+  const int kp = 32 * nc;
+
+  // scratch: partial sums | packed operands and factors (rows padded to whole
+  // workgroups, sources to 16-point blocks) | orders | bucket keys | sort space
+  const size_t mp = (size_t)plan.row_blocks * kMixRowsPerWG, np = ((size_t)p.n + 15) / 16 * 16;
+  const size_t mx = (size_t)(p.m > p.n ? p.m : p.n);
+  size_t sort_bytes = 0;
+  if (bucket)
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const unsigned *)nullptr,
+                                             (unsigned *)nullptr, (const int *)nullptr,
+                                             (int *)nullptr, (int)mx, 0, 8, s);
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t off = 0;
+  const size_t o_Ap = off; off += al(mp * nc * 128);
+  const size_t o_Bp = off; off += al(np * nc * 128);
+  const size_t o_Bd = off; off += al(np * nc * 256);
+  const size_t o_ra = off; off += al(mp * sizeof(float2));
+  const size_t o_cb = off; off += al(np * sizeof(float4));
+  const size_t o_rd = off; off += al(mp * sizeof(double2));
+  const size_t o_cd = off; off += al(np * sizeof(double4));
+  const size_t o_prow = off; off += al((size_t)p.m * sizeof(int));
+  const size_t o_pcol = off; off += al((size_t)p.n * sizeof(int));
+  const size_t o_ids = off; off += al(mx * sizeof(int));
+  const size_t o_ka = off; off += bucket ? al((size_t)p.m * sizeof(unsigned)) : 0;
+  const size_t o_kb = off; off += bucket ? al((size_t)p.n * sizeof(unsigned)) : 0;
+  const size_t o_kout = off; off += bucket ? al(mx * sizeof(unsigned)) : 0;
+  const size_t o_piv = off; off += bucket ? al((size_t)P * kp * sizeof(float)) : 0;
+  const size_t o_sort = off; off += al(sort_bytes);
+  const size_t npart = (size_t)(plan.nsplit > 1 ? plan.nsplit : 0) * p.m;
   double *partial = nullptr;
   unsigned *counters = nullptr;
-  size_t pelems = (size_t)(plan.nsplit > 1 ? plan.nsplit : 0) * p.m + (extra + 7) / 8;
-  int rc = get_scratch(s, pelems, (size_t)plan.row_blocks, &partial, &counters);
+  int rc = get_scratch(s, npart + (off + 255) / 8 + 32, (size_t)plan.row_blocks, &partial, &counters);
   if (rc != KS_OK) return rc;
-  char *base = reinterpret_cast<char *>(partial + (size_t)(plan.nsplit > 1 ? plan.nsplit : 0) * p.m);
-  auto align = [](char *q) { return (char *)(((uintptr_t)q + 255) & ~(uintptr_t)255); };
+  char *base = reinterpret_cast<char *>(partial + npart);
+  base = (char *)(((uintptr_t)base + 255) & ~(uintptr_t)255);
   MixPack pk;
-  char *q = align(base);
-  pk.Ap = (uint4 *)q;
-  q = align(q + packA);
-  pk.Bp = (uint4 *)q;
-  q = align(q + packB);
-  pk.ra = (float2 *)q;
-  q = align(q + ra);
-  pk.cb = (float4 *)q;
+  pk.Ap = (uint4 *)(base + o_Ap);
+  pk.Bp = (uint4 *)(base + o_Bp);
+  pk.Bd = (double4 *)(base + o_Bd);
+  pk.ra = (float2 *)(base + o_ra);
+  pk.cb = (float4 *)(base + o_cb);
+  pk.rd = (double2 *)(base + o_rd);
+  pk.cd = (double4 *)(base + o_cd);
+  int *prow = (int *)(base + o_prow);
+  int *pcol = (int *)(base + o_pcol);
+  pk.prow = prow;
+  pk.pcol = pcol;
+
+  // ---- target and source orders
+  if (!bucket) {
+    hipLaunchKernelGGL(mix_identity_kernel, dim3((unsigned)((p.m + 255) / 256)), dim3(256), 0, s,
+                       prow, (long)p.m);
+    hipLaunchKernelGGL(mix_identity_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, s,
+                       pcol, (long)p.n);
+  } else {
+    int *ids = (int *)(base + o_ids);
+    unsigned *ka = (unsigned *)(base + o_ka), *kb = (unsigned *)(base + o_kb);
+    unsigned *kout = (unsigned *)(base + o_kout);
+    float *piv = (float *)(base + o_piv);
+    hipLaunchKernelGGL(mix_pivot_kernel, dim3((unsigned)((P * kp + 255) / 256)), dim3(256), 0, s, p,
+                       P, kp, piv);
+    switch (nc) {
+      case 1: launch_bucket<32>(p, P, piv, ka, kb, s); break;
+      case 2: launch_bucket<64>(p, P, piv, ka, kb, s); break;
+      case 3: launch_bucket<96>(p, P, piv, ka, kb, s); break;
+      default: launch_bucket<128>(p, P, piv, ka, kb, s); break;
+    }
+    hipLaunchKernelGGL(mix_identity_kernel, dim3((unsigned)((mx + 255) / 256)), dim3(256), 0, s,
+                       ids, (long)mx);
+    size_t tb = sort_bytes;
+    if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, tb, (const unsigned *)ka, kout,
+                                           (const int *)ids, prow, p.m, 0, 8, s) != hipSuccess)
+      return KS_ERR_DEVICE;
+    tb = sort_bytes;
+    if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, tb, (const unsigned *)kb, kout,
+                                           (const int *)ids, pcol, p.n, 0, 8, s) != hipSuccess)
+      return KS_ERR_DEVICE;
+  }
+  if (hipGetLastError() != hipSuccess) return KS_ERR_DEVICE;
+
   hipError_t e = (precision == KS_PREC_MIXED_BF16X3)
                      ? mixed_launch<KS_PREC_MIXED_BF16X3>(p, plan, partial, counters, pk, nc, s)
                      : mixed_launch<KS_PREC_MIXED_FP32>(p, plan, partial, counters, pk, nc, s);
