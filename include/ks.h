@@ -205,6 +205,42 @@ int dgsks_list(int ntask, ks_t *kernels, int k, double *u, long nu,
                const double *XB, const double *XB2, const double *w,
                int precision);
 
+/* Reusable batch plans.  No counterpart in the reference (its batches are
+ * re-laid out on every omp_dgsks_list call, frame/omp_dgsks_list.cpp:345-359):
+ * for callers that run the same batch structure many times, and for HIP graph
+ * capture.  ks_list_plan_create does all host work once -- tile layout with
+ * the autotuned tile shapes, device descriptors, the merge order of umap --
+ * and synchronises; it must not run while a stream of the process is being
+ * captured.  Arguments as for dgsks_list, except that umap/amap/bmap/wmap (and
+ * the hi/hj vectors of variable-bandwidth descriptors) are DEVICE arrays that
+ * must stay valid and unchanged while the plan lives; the Laplace descriptor
+ * side effect is not applied (kernels is read only).
+ * ks_list_plan_execute enqueues one run on `stream` (NULL: the HIP null
+ * stream): u[umap[i]] += task sums, with device XA/XA2/XB/XB2/w of the sizes
+ * the maps index.  It only launches kernels, so it can be captured into a HIP
+ * graph and replayed (FP64 plans; a mixed-precision plan with tasks routed to
+ * the mixed kernels needs one eager run on the same stream first, which sizes
+ * its scratch).  Runs of one plan must not overlap. */
+typedef struct ks_list_plan_s ks_list_plan;
+int ks_list_plan_create(ks_list_plan **plan, int ntask, const ks_t *kernels, int k,
+                        const long *a_off, const int *umap, const int *amap,
+                        const long *b_off, const int *bmap, const int *wmap,
+                        int precision);
+int ks_list_plan_execute(const ks_list_plan *plan, double *u, const double *XA,
+                         const double *XA2, const double *XB, const double *XB2,
+                         const double *w, void *stream);
+void ks_list_plan_destroy(ks_list_plan *plan);
+
+/* Tile shape of the batched launch for tasks of kernel `type` at dimension k
+ * and `precision`: *slice sources per 64-row tile, *tile = 1 for FP64
+ * matrix-core tiles, 0 for one row per lane.  Tuned on first use (timing every
+ * candidate on a synthetic batch of that type) and cached for the process and,
+ * when the environment variable KS_TUNE_CACHE names a file, across processes;
+ * KS_AUTOTUNE=0 returns the default shape without tuning.  No counterpart in
+ * the reference (its blocking is fixed at compile time,
+ * micro_kernel/x86_64/sandybridge/gsks_config.h:3-5). */
+int ks_list_autotune(int type, int k, int precision, int *slice, int *tile);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,13 @@ _SIGS = {
                                              ctypes.c_int, ctypes.c_int] + [_vp] * 10 + [_vp]),
     "ks_gauss_grad_device": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp,
                                             _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
+    "ks_list_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(KsT),
+                                           ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                           ctypes.c_int]),
+    "ks_list_plan_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ks_list_plan_destroy": (None, [_vp]),
+    "ks_list_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
 }
 
 

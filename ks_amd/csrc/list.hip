@@ -47,6 +47,8 @@ __device__ __forceinline__ cptr<T> to_const(const T *p) {
 struct DevTask {
   int type;
   int m, n;
+  int bc;    // source slice per tile (the tuned tile shape)
+  int tile;  // 1: FP64 matrix-core tile (9 <= k <= 64), 0: lane-per-row tile
   int pad;
   long a0, b0;
   KerParams P;
@@ -61,7 +63,7 @@ struct RowBlock {
 };
 
 struct ListArgs {
-  int ntiles, k, bc;
+  int ntiles, k;
   const int2 *tiles;  // {row block, slice}
   const RowBlock *rbs;
   const DevTask *tasks;
@@ -214,7 +216,7 @@ template <int TYPE, int K>
 __device__ double run_tile(const ListArgs &L, const DevTask &T, int row0, int c0, int c1,
                            double *red) {
   if constexpr (TYPE != LAPLACE && K == 0) {
-    if (L.k >= 9 && L.k <= 64) {
+    if (T.tile == 1 && L.k >= 9 && L.k <= 64) {
       switch ((L.k + 15) / 16) {
         case 1: return tile_mfma<TYPE, 1>(L, T, row0, c0, c1, red);
         case 2: return tile_mfma<TYPE, 2>(L, T, row0, c0, c1, red);
@@ -240,8 +242,8 @@ __global__ __launch_bounds__(256) void list_kernel(ListArgs L) {
     const int2 tl = L.tiles[t];
     const RowBlock rb = L.rbs[tl.x];
     const DevTask T = L.tasks[rb.task];
-    const int c0 = tl.y * L.bc;
-    const int c1 = min(T.n, c0 + L.bc);
+    const int c0 = tl.y * T.bc;
+    const int c1 = min(T.n, c0 + T.bc);
     double s;
     switch (T.type) {
       case GAUSSIAN: s = run_tile<GAUSSIAN, K>(L, T, rb.row0, c0, c1, red); break;
@@ -306,56 +308,58 @@ static void launch_list(const ListArgs &A, int grid, hipStream_t s) {
   hipLaunchKernelGGL(list_kernel<K>, dim3(grid), dim3(256), 0, s, A);
 }
 
-struct ListScratch {
+bool list_mfma_eligible(int type, int k) { return type != KS_LAPLACE && k >= 9 && k <= 64; }
+
+// Device state of one batch layout: everything a run needs except the point
+// data, so a run is kernel launches only.
+struct ListPlan {
+  int k = 0, precision = KS_PREC_FP64;
+  long total_a = 0, ntiles = 0;
+  int grid = 0;
   char *buf = nullptr;
-  size_t bytes = 0;
-  int dev = -1;
-  hipStream_t stream = nullptr;
+  ListArgs A{};  // layout part (tiles, row blocks, tasks, partials, counters, R)
+  const int *keys = nullptr, *pos = nullptr;  // umap sorted, with positions
+  std::vector<TaskDesc> mixed;                // tasks the mixed kernels run
+  const int *umap = nullptr, *amap = nullptr, *bmap = nullptr, *wmap = nullptr;
 };
 
-static int list_tile_cols(int k) {
-  (void)k;
-  return 1024;
-}
-
-int list_enqueue(const std::vector<TaskDesc> &tasks, int k, const ListArrays &L, int precision,
-                 hipStream_t s) {
-  if (precision != KS_PREC_FP64) {
-    // mixed-precision lists run task by task through the mixed kernels
-    for (const TaskDesc &t : tasks) {
-      if (t.m == 0 || t.n == 0) continue;
-      Problem p;
-      p.m = t.m; p.n = t.n; p.k = k; p.type = t.type; p.P = t.P;
-      p.u = L.u; p.umap = L.umap + t.a0; p.XA = L.XA; p.XA2 = L.XA2; p.amap = L.amap + t.a0;
-      p.XB = L.XB; p.XB2 = L.XB2; p.bmap = L.bmap + t.b0; p.w = L.w; p.wmap = L.wmap + t.b0;
-      p.hi = t.hi; p.hj = t.hj;
-      int rc = enqueue_mixed(p, precision, s);
-      if (rc != KS_OK) return rc;
-    }
-    return KS_OK;
-  }
-  const int bc = list_tile_cols(k);
-  // ---- tile layout, largest tiles first
+int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
+                    const int *umap, const int *amap, const int *bmap, const int *wmap,
+                    long total_a, int precision, const TileChoice *force, hipStream_t s) {
+  *out = nullptr;
+  ListPlan *pl = new ListPlan();
+  pl->k = k;
+  pl->precision = precision;
+  pl->total_a = total_a;
+  pl->umap = umap;
+  pl->amap = amap;
+  pl->bmap = bmap;
+  pl->wmap = wmap;
+  // ---- which tasks the persistent FP64 launch runs, with their tile shapes
   std::vector<DevTask> dt(tasks.size());
   std::vector<RowBlock> rbs;
-  std::vector<int2> tiles;
-  long total_a = 0;
   for (size_t t = 0; t < tasks.size(); ++t) {
     const TaskDesc &td = tasks[t];
-    dt[t] = DevTask{td.type, td.m, td.n, 0, td.a0, td.b0, td.P, td.hi, td.hj};
-    total_a = std::max(total_a, td.a0 + td.m);
+    dt[t] = DevTask{td.type, td.m, td.n, 1024, 0, 0, td.a0, td.b0, td.P, td.hi, td.hj};
     if (td.m == 0 || td.n == 0) continue;
-    const int nsplit = (td.n + bc - 1) / bc;
+    if (precision != KS_PREC_FP64 && (td.type == KS_GAUSSIAN || td.type == KS_GAUSSIAN_VAR_BANDWIDTH) &&
+        (long)td.m * td.n >= kMixedListMin) {
+      pl->mixed.push_back(td);
+      continue;
+    }
+    const TileChoice tc = force ? *force : list_tile_choice(td.type, k, KS_PREC_FP64);
+    dt[t].bc = tc.slice;
+    dt[t].tile = (tc.tile == 1 && list_mfma_eligible(td.type, k)) ? 1 : 0;
+    const int nsplit = (td.n + dt[t].bc - 1) / dt[t].bc;
     for (int r0 = 0; r0 < td.m; r0 += 64) rbs.push_back(RowBlock{(int)t, r0, nsplit, 0});
   }
-  if (rbs.empty()) return KS_OK;
   // order row blocks by work (columns per tile x rows) descending, then lay tiles out
   std::vector<int> order(rbs.size());
   std::iota(order.begin(), order.end(), 0);
   auto work = [&](const RowBlock &r) {
     const DevTask &T = dt[r.task];
     long rows = std::min(64, T.m - r.row0);
-    long cols = std::min(bc, T.n);
+    long cols = std::min(T.bc, T.n);
     return rows * cols;
   };
   std::stable_sort(order.begin(), order.end(),
@@ -363,25 +367,32 @@ int list_enqueue(const std::vector<TaskDesc> &tasks, int k, const ListArrays &L,
   std::vector<RowBlock> rbs_sorted(rbs.size());
   for (size_t i = 0; i < order.size(); ++i) rbs_sorted[i] = rbs[order[i]];
   rbs.swap(rbs_sorted);
+  std::vector<int2> tiles;
   for (size_t i = 0; i < rbs.size(); ++i) {
     rbs[i].tile_base = (int)tiles.size();
     for (int sp = 0; sp < rbs[i].nsplit; ++sp) tiles.push_back(make_int2((int)i, sp));
   }
-  const long ntiles = (long)tiles.size();
-  if (ntiles >= (1l << 31)) return KS_ERR_INVALID_ARG;
+  pl->ntiles = (long)tiles.size();
+  if (pl->ntiles >= (1l << 31)) {
+    delete pl;
+    return KS_ERR_INVALID_ARG;
+  }
+  if (pl->ntiles == 0) {
+    *out = pl;
+    return KS_OK;
+  }
 
-  // ---- scratch: descriptors, partials, counters, R, sort buffers
+  // ---- device buffer: descriptors, partials, counters, R, merge order, sort space
   size_t sort_bytes = 0;
-  const int nbits = 32;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int *)nullptr,
                                            (int *)nullptr, (const int *)nullptr, (int *)nullptr,
-                                           (int)total_a, 0, nbits, s);
+                                           (int)total_a, 0, 32, s);
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t off_tiles = 0;
-  size_t off_rbs = off_tiles + al(ntiles * sizeof(int2));
+  size_t off_rbs = off_tiles + al(pl->ntiles * sizeof(int2));
   size_t off_tasks = off_rbs + al(rbs.size() * sizeof(RowBlock));
   size_t off_part = off_tasks + al(dt.size() * sizeof(DevTask));
-  size_t off_cnt = off_part + al((size_t)ntiles * 64 * sizeof(double));
+  size_t off_cnt = off_part + al((size_t)pl->ntiles * 64 * sizeof(double));
   size_t off_head = off_cnt + al(rbs.size() * sizeof(unsigned));
   size_t off_R = off_head + 256;
   size_t off_iota = off_R + al((size_t)total_a * sizeof(double));
@@ -390,67 +401,99 @@ int list_enqueue(const std::vector<TaskDesc> &tasks, int k, const ListArrays &L,
   size_t off_sort = off_vout + al((size_t)total_a * sizeof(int));
   size_t bytes = off_sort + al(sort_bytes);
   char *buf = nullptr;
-  if (hipMallocAsync((void **)&buf, bytes, s) != hipSuccess) {
+  if (hipMalloc((void **)&buf, bytes) != hipSuccess) {
     (void)hipGetLastError();
+    delete pl;
     return KS_ERR_ALLOC;
   }
+  pl->buf = buf;
   int rc = KS_OK;
-  if (hipMemcpyAsync(buf + off_tiles, tiles.data(), ntiles * sizeof(int2), hipMemcpyHostToDevice, s) != hipSuccess ||
+  // counters and R start at zero (the kernel leaves its counters at zero after
+  // every run, and R rows of tasks without sources are never written)
+  if (hipMemcpyAsync(buf + off_tiles, tiles.data(), pl->ntiles * sizeof(int2), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(buf + off_rbs, rbs.data(), rbs.size() * sizeof(RowBlock), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(buf + off_tasks, dt.data(), dt.size() * sizeof(DevTask), hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemsetAsync(buf + off_cnt, 0, off_R - off_cnt, s) != hipSuccess ||
-      hipMemsetAsync(buf + off_R, 0, (size_t)total_a * sizeof(double), s) != hipSuccess)
+      hipMemsetAsync(buf + off_cnt, 0, off_iota - off_cnt, s) != hipSuccess)
     rc = KS_ERR_DEVICE;
-  ListArgs A;
-  A.ntiles = (int)ntiles;
+  ListArgs &A = pl->A;
+  A.ntiles = (int)pl->ntiles;
   A.k = k;
-  A.bc = bc;
   A.tiles = (const int2 *)(buf + off_tiles);
   A.rbs = (const RowBlock *)(buf + off_rbs);
   A.tasks = (const DevTask *)(buf + off_tasks);
-  A.amap = L.amap;
-  A.bmap = L.bmap;
-  A.wmap = L.wmap;
-  A.XA = L.XA;
-  A.XA2 = L.XA2;
-  A.XB = L.XB;
-  A.XB2 = L.XB2;
-  A.w = L.w;
   A.partial = (double *)(buf + off_part);
   A.rbcount = (unsigned *)(buf + off_cnt);
   A.head = (unsigned *)(buf + off_head);
   A.done = A.head + 1;
   A.R = (double *)(buf + off_R);
+  // merge order: target ids stably sorted with their positions
+  int *iota = (int *)(buf + off_iota);
+  int *kout = (int *)(buf + off_kout);
+  int *vout = (int *)(buf + off_vout);
   if (rc == KS_OK) {
-    // persistent grid: enough 256-thread workgroups to hold ~4 waves per SIMD
-    int dev = 0, ncu = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    long grid = std::min<long>((long)ncu * 4, (ntiles + 3) / 4);
-    if (grid < 1) grid = 1;
-    switch (k) {
-      case 3: launch_list<3>(A, (int)grid, s); break;
-      case 8: launch_list<8>(A, (int)grid, s); break;
-      default: launch_list<0>(A, (int)grid, s); break;
-    }
-    if (hipGetLastError() != hipSuccess) rc = KS_ERR_DEVICE;
-  }
-  if (rc == KS_OK) {
-    int *iota = (int *)(buf + off_iota);
-    int *kout = (int *)(buf + off_kout);
-    int *vout = (int *)(buf + off_vout);
-    long nb = (total_a + 255) / 256;
+    const long nb = (total_a + 255) / 256;
     hipLaunchKernelGGL(iota_kernel, dim3((unsigned)nb), dim3(256), 0, s, iota, total_a);
     size_t tb = sort_bytes;
-    if (hipcub::DeviceRadixSort::SortPairs(buf + off_sort, tb, L.umap, kout, (const int *)iota,
-                                           vout, (int)total_a, 0, nbits, s) != hipSuccess)
+    if (hipcub::DeviceRadixSort::SortPairs(buf + off_sort, tb, umap, kout, (const int *)iota, vout,
+                                           (int)total_a, 0, 32, s) != hipSuccess)
       rc = KS_ERR_DEVICE;
-    hipLaunchKernelGGL(merge_kernel, dim3((unsigned)nb), dim3(256), 0, s, (const int *)kout,
-                       (const int *)vout, (const double *)A.R, total_a, L.u);
-    if (hipGetLastError() != hipSuccess) rc = KS_ERR_DEVICE;
   }
-  (void)hipFreeAsync(buf, s);
-  return rc;
+  if (rc == KS_OK && hipStreamSynchronize(s) != hipSuccess) rc = KS_ERR_DEVICE;
+  pl->keys = kout;
+  pl->pos = vout;
+  // persistent grid: enough 256-thread workgroups to hold ~4 waves per SIMD
+  int dev = 0, ncu = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  long grid = std::min<long>((long)ncu * 4, (pl->ntiles + 3) / 4);
+  pl->grid = (int)std::max<long>(grid, 1);
+  if (rc != KS_OK) {
+    list_plan_free(pl);
+    return rc;
+  }
+  *out = pl;
+  return KS_OK;
 }
+
+int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
+  if (pl.ntiles > 0) {
+    ListArgs A = pl.A;
+    A.amap = pl.amap;
+    A.bmap = pl.bmap;
+    A.wmap = pl.wmap;
+    A.XA = L.XA;
+    A.XA2 = L.XA2;
+    A.XB = L.XB;
+    A.XB2 = L.XB2;
+    A.w = L.w;
+    switch (pl.k) {
+      case 3: launch_list<3>(A, pl.grid, s); break;
+      case 8: launch_list<8>(A, pl.grid, s); break;
+      default: launch_list<0>(A, pl.grid, s); break;
+    }
+    const long nb = (pl.total_a + 255) / 256;
+    hipLaunchKernelGGL(merge_kernel, dim3((unsigned)nb), dim3(256), 0, s, pl.keys, pl.pos,
+                       (const double *)A.R, pl.total_a, L.u);
+    if (hipGetLastError() != hipSuccess) return KS_ERR_DEVICE;
+  }
+  for (const TaskDesc &t : pl.mixed) {
+    Problem p;
+    p.m = t.m; p.n = t.n; p.k = pl.k; p.type = t.type; p.P = t.P;
+    p.u = L.u; p.umap = pl.umap + t.a0; p.XA = L.XA; p.XA2 = L.XA2; p.amap = pl.amap + t.a0;
+    p.XB = L.XB; p.XB2 = L.XB2; p.bmap = pl.bmap + t.b0; p.w = L.w; p.wmap = pl.wmap + t.b0;
+    p.hi = t.hi; p.hj = t.hj;
+    int rc = enqueue_mixed(p, pl.precision, s);
+    if (rc != KS_OK) return rc;
+  }
+  return KS_OK;
+}
+
+void list_plan_free(ListPlan *pl) {
+  if (!pl) return;
+  if (pl->buf) (void)hipFree(pl->buf);
+  delete pl;
+}
+
+long list_plan_tiles(const ListPlan *pl) { return pl ? pl->ntiles : 0; }
 
 }  // namespace gsks

@@ -1,8 +1,9 @@
 // dgsks_list(): many independent summations over shared point pools in one call
 // (the batched path of frame/omp_dgsks_list.cpp:297-420, with one kernel
 // descriptor per task).  Host arrays are staged through HBM once for the whole
-// batch; device arrays are used in place.  The tasks run on the GPU through
-// list_enqueue (list.hip).
+// batch; device arrays are used in place.  The tasks run on the GPU through a
+// batch plan (list.hip), built and freed within the call; ks_list_plan_* keeps
+// a plan across runs for callers that replay the same batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -178,7 +179,16 @@ int list_run(int ntask, const ks_t *kernels, int k, double *u, long nu, const lo
       L.XB2 = d;
     }
   }
-  if (rc == KS_OK) rc = list_enqueue(tasks, k, L, precision, s);
+  if (rc == KS_OK) {
+    ListPlan *pl = nullptr;
+    rc = list_plan_build(&pl, tasks, k, L.umap, L.amap, L.bmap, L.wmap, total_a, precision,
+                         nullptr, s);
+    if (rc == KS_OK) rc = list_plan_exec(*pl, L, s);
+    if (pl) {
+      (void)hipStreamSynchronize(s);
+      list_plan_free(pl);
+    }
+  }
   if (rc == KS_OK && !on_device &&
       hipMemcpyAsync(u, L.u, (size_t)nu * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
     rc = KS_ERR_DEVICE;
@@ -204,4 +214,99 @@ extern "C" int dgsks_list(int ntask, ks_t *kernels, int k, double *u, long nu,
     for (int t = 0; t < ntask; ++t)  // the Laplace descriptor side effect, per task
       if (kernels[t].type == KS_LAPLACE) laplace_params(k, &kernels[t].powe, &kernels[t].scal);
   return rc;
+}
+
+// ---- reusable batch plans (device-resident batches, replayable runs)
+
+struct ks_list_plan_s {
+  gsks::ListPlan *pl = nullptr;
+  int k = 0;
+};
+
+extern "C" int ks_list_plan_create(ks_list_plan **plan, int ntask, const ks_t *kernels, int k,
+                                   const long *a_off, const int *umap, const int *amap,
+                                   const long *b_off, const int *bmap, const int *wmap,
+                                   int precision) {
+  using namespace gsks;
+  const char *name = "ks_list_plan_create";
+  if (!plan) return raise_error(KS_ERR_INVALID_ARG, false, "%s(): NULL plan\n", name);
+  *plan = nullptr;
+  if (ntask < 0 || k <= 0 || (ntask > 0 && (!kernels || !a_off || !b_off)))
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): invalid argument\n", name);
+  if (precision != KS_PREC_FP64 && precision != KS_PREC_MIXED_FP32 &&
+      precision != KS_PREC_MIXED_BF16X3)
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): unknown precision\n", name);
+  for (int t = 0; t < ntask; ++t)
+    if (a_off[t + 1] < a_off[t] || b_off[t + 1] < b_off[t] || a_off[t] < 0 || b_off[t] < 0)
+      return raise_error(KS_ERR_LIST, false, "%s(): offsets must be non-decreasing\n", name);
+  if (ntask > 0 && a_off[0] != 0)
+    return raise_error(KS_ERR_LIST, false, "%s(): a_off[0] must be 0\n", name);
+  if (ensure_device() != KS_OK)
+    return raise_error(KS_ERR_DEVICE, true, "%s(): no GPU available\n", name);
+  const long total_a = ntask > 0 ? a_off[ntask] : 0;
+  std::vector<TaskDesc> tasks(ntask);
+  for (int t = 0; t < ntask; ++t) {
+    KerParams P;
+    int rc = make_params(&kernels[t], k, P, true);
+    if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return rc;
+    tasks[t].type = (int)kernels[t].type;
+    tasks[t].P = P;
+    tasks[t].a0 = a_off[t];
+    tasks[t].m = (int)(a_off[t + 1] - a_off[t]);
+    tasks[t].b0 = b_off[t];
+    tasks[t].n = (int)(b_off[t + 1] - b_off[t]);
+    tasks[t].hi = kernels[t].hi;
+    tasks[t].hj = kernels[t].hj;
+    if (kernels[t].type == KS_GAUSSIAN_VAR_BANDWIDTH &&
+        !(is_device_ptr(kernels[t].hi) && is_device_ptr(kernels[t].hj)))
+      return raise_error(KS_ERR_INVALID_ARG, false,
+                         "%s(): the bandwidth vectors of a plan must be device arrays\n", name);
+  }
+  if (total_a > 0 && !(is_device_ptr(umap) && is_device_ptr(amap) && is_device_ptr(bmap) &&
+                       is_device_ptr(wmap)))
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): the maps of a plan must be device arrays\n",
+                       name);
+  hipStream_t s = thread_stream();
+  if (!s) return raise_error(KS_ERR_DEVICE, true, "%s(): stream creation failed\n", name);
+  ks_list_plan *h = new ks_list_plan();
+  h->k = k;
+  int rc = total_a > 0 ? list_plan_build(&h->pl, tasks, k, umap, amap, bmap, wmap, total_a,
+                                         precision, nullptr, s)
+                       : KS_OK;
+  if (rc != KS_OK) {
+    delete h;
+    return raise_error(rc, rc == KS_ERR_DEVICE, "%s(): %s\n", name, err_string(rc));
+  }
+  *plan = h;
+  return KS_OK;
+}
+
+extern "C" int ks_list_plan_execute(const ks_list_plan *plan, double *u, const double *XA,
+                                    const double *XA2, const double *XB, const double *XB2,
+                                    const double *w, void *stream) {
+  using namespace gsks;
+  const char *name = "ks_list_plan_execute";
+  if (!plan) return raise_error(KS_ERR_INVALID_ARG, false, "%s(): NULL plan\n", name);
+  if (!plan->pl) return KS_OK;
+  if (!u || !XA || !XA2 || !XB || !XB2 || !w)
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): NULL array\n", name);
+  if (!(device_ptr_cached(u) && device_ptr_cached(XA) && device_ptr_cached(XA2) &&
+        device_ptr_cached(XB) && device_ptr_cached(XB2) && device_ptr_cached(w)))
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): arrays must be device arrays\n", name);
+  ListArrays L;
+  L.u = u;
+  L.XA = XA;
+  L.XA2 = XA2;
+  L.XB = XB;
+  L.XB2 = XB2;
+  L.w = w;
+  int rc = list_plan_exec(*plan->pl, L, (hipStream_t)stream);
+  if (rc != KS_OK) return raise_error(rc, rc == KS_ERR_DEVICE, "%s(): %s\n", name, err_string(rc));
+  return KS_OK;
+}
+
+extern "C" void ks_list_plan_destroy(ks_list_plan *plan) {
+  if (!plan) return;
+  gsks::list_plan_free(plan->pl);
+  delete plan;
 }

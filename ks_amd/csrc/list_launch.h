@@ -23,9 +23,34 @@ struct ListArrays {
   const double *XA = nullptr, *XA2 = nullptr, *XB = nullptr, *XB2 = nullptr, *w = nullptr;
 };
 
-// Enqueue every task on stream s; all ListArrays pointers are device pointers.
-int list_enqueue(const std::vector<TaskDesc> &tasks, int k, const ListArrays &L, int precision,
-                 hipStream_t s);
+// Tile shape of a task: sources per tile, and the tile kind (1: 64 rows as four
+// 16 x 16 blocks on the FP64 matrix cores, for 9 <= k <= 64 except Laplace;
+// 0: one row per lane on the vector units).
+struct TileChoice {
+  int slice;
+  int tile;
+};
+bool list_mfma_eligible(int type, int k);
+// The shape for tasks of one kernel type at one k and precision: tuned once per
+// process (list_tune.cpp) and cached; KS_AUTOTUNE=0 keeps the default shape.
+TileChoice list_tile_choice(int type, int k, int precision);
+TileChoice list_default_tile(int type, int k);
+
+// In a mixed-precision batch the Gaussian tasks with at least this many pairs
+// run through the mixed kernels; every other task runs in the persistent FP64 launch.
+constexpr long kMixedListMin = 1l << 22;
+
+// A batch layout on the device (list.hip).  Building it does the host work
+// (tile layout, descriptors, the merge order of umap) and synchronises; running
+// it only enqueues kernels, so a run can be captured into a HIP graph.  The
+// maps are device arrays that must outlive the plan.
+struct ListPlan;
+int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k, const int *umap,
+                    const int *amap, const int *bmap, const int *wmap, long total_a,
+                    int precision, const TileChoice *force, hipStream_t s);
+int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s);
+void list_plan_free(ListPlan *pl);
+long list_plan_tiles(const ListPlan *pl);
 
 int list_run(int ntask, const ks_t *kernels, int k, double *u, long nu, const long *a_off,
              const int *umap, const int *amap, const long *b_off, const int *bmap,

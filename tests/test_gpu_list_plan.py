@@ -1,0 +1,244 @@
+"""GPU tests of the reusable batch plans (ks_list_plan_*), HIP graph capture of
+the stream-ordered entry points, the tile-shape autotuner and the routing of
+mixed-precision batches.  FP64 results: relative L2 <= 1e-12 against the
+oracle (the reference's error metric, test/test_dgsks.c:43-79); graph replays
+must equal the eager run bit for bit."""
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+@pytest.fixture(scope="module")
+def L():
+    from ks_amd import _lib
+    return _lib.lib()
+
+
+def _batch(k, seed, pool=5000, ntask=36, big_gauss=0):
+    """Tasks over one pool: Gaussian / Laplace / polynomial kernels, shared
+    targets, an empty task, sizes across the tile slices; `big_gauss` extra
+    Gaussian tasks of 2100 x 2100 (past the mixed-routing threshold)."""
+    from ks_amd import _lib
+    rng = np.random.default_rng(seed)
+    X = rng.random((pool, k)) * (1.5 / np.sqrt(k))
+    X2 = np.zeros(pool)
+    for p in range(k):
+        X2 += X[:, p] * X[:, p]
+    w = rng.standard_normal(pool)
+    ms = list(rng.integers(1, 300, ntask)) + [2100] * big_gauss
+    ns = list(rng.integers(1, 2600, ntask)) + [2100] * big_gauss
+    ms[2] = 0
+    kinds = [dict(type=0, scal=-1.5), dict(type=2), dict(type=1, scal=1.0 / k, cons=1.0, powe=2.0)]
+    kernels_py = [kinds[t % 3] for t in range(ntask)] + [dict(type=0, scal=-40.0)] * big_gauss
+    amap = np.concatenate([rng.choice(pool, int(s), replace=False).astype(np.int32) for s in ms])
+    bmap = np.concatenate([rng.choice(pool, int(s), replace=False).astype(np.int32) for s in ns])
+    a_off = np.concatenate([[0], np.cumsum(ms)]).astype(np.int64)
+    b_off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    nt = len(kernels_py)
+    kernels = (_lib.KsT * nt)()
+    for t, kd in enumerate(kernels_py):
+        kernels[t].type = kd["type"]
+        kernels[t].scal = kd.get("scal", 0.0)
+        kernels[t].cons = kd.get("cons", 0.0)
+        kernels[t].powe = kd.get("powe", 0.0)
+    return dict(k=k, X=X, X2=X2, w=w, amap=amap, bmap=bmap, a_off=a_off, b_off=b_off,
+                kernels=kernels, kernels_py=kernels_py, ntask=nt, pool=pool)
+
+
+class _Plan:
+    """A plan over device copies of the batch's maps."""
+
+    def __init__(self, L, torch, B, precision=0):
+        self.L, self.torch = L, torch
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+        self.amap, self.bmap = dev(B["amap"]), dev(B["bmap"])
+        self.X, self.X2, self.w = dev(B["X"]), dev(B["X2"]), dev(B["w"])
+        self.h = ctypes.c_void_p()
+        P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        hp = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+        rc = L.ks_list_plan_create(ctypes.byref(self.h), B["ntask"], B["kernels"], B["k"],
+                                   hp(B["a_off"]), P(self.amap), P(self.amap), hp(B["b_off"]),
+                                   P(self.bmap), P(self.bmap), precision)
+        assert rc == 0, rc
+
+    def run(self, u, stream):
+        P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        return self.L.ks_list_plan_execute(self.h, P(u), P(self.X), P(self.X2), P(self.X),
+                                           P(self.X2), P(self.w), ctypes.c_void_p(stream.cuda_stream))
+
+    def close(self):
+        self.L.ks_list_plan_destroy(self.h)
+
+
+def _oracle_list(oracle, B, u0):
+    ref = u0.copy()
+    oracle.dgsks_list(B["kernels_py"], B["k"], ref, B["a_off"], B["amap"], B["amap"], B["b_off"],
+                      B["bmap"], B["bmap"], B["X"], B["X2"], B["X"], B["X2"], B["w"])
+    return ref
+
+
+@pytest.mark.parametrize("k", [3, 20])
+def test_plan_matches_oracle_and_dgsks_list(L, torch_cuda, oracle, k):
+    torch = torch_cuda
+    B = _batch(k, 400 + k)
+    rng = np.random.default_rng(1)
+    u0 = rng.standard_normal(B["pool"])
+    ref = _oracle_list(oracle, B, u0)
+    plan = _Plan(L, torch, B)
+    try:
+        s = torch.cuda.current_stream()
+        u = torch.from_numpy(u0.copy()).cuda()
+        assert plan.run(u, s) == 0
+        torch.cuda.synchronize()
+        got = u.cpu().numpy()
+        err = oracle.rel_l2(got, ref)
+        print(f"plan k={k}: rel L2 = {err:.3e}")
+        assert err <= TOL
+        # the one-shot API lays out the same tiles, so it agrees bit for bit
+        u1 = u0.copy()
+        p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+        kern = B["kernels"]
+        assert L.dgsks_list(B["ntask"], kern, k, p(u1), u1.size, p(B["a_off"]), p(B["amap"]),
+                            p(B["amap"]), p(B["b_off"]), p(B["bmap"]), p(B["bmap"]), p(B["X"]),
+                            p(B["X2"]), p(B["X"]), p(B["X2"]), p(B["w"]), 0) == 0
+        assert np.array_equal(u1, got)
+    finally:
+        plan.close()
+
+
+def _replay_check(torch, run_on, u, reps=4):
+    """Eager run, then a captured run replayed `reps` times with eager work and
+    synchronisations between replays; every replay must equal the eager result."""
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        u.zero_()
+        assert run_on(s) == 0  # eager (also sizes any scratch on this stream)
+    torch.cuda.synchronize()
+    eager = u.clone()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        assert run_on(s) == 0
+    torch.cuda.synchronize()
+    junk = torch.ones(1 << 20, dtype=torch.float64, device="cuda")
+    for r in range(reps):
+        u.zero_()
+        junk.mul_(1.0001).add_(0.5)  # unrelated eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(u, eager), f"replay {r} differs from the eager run"
+    return eager
+
+
+@pytest.mark.parametrize("k", [8, 32])
+def test_plan_graph_capture_replays(L, torch_cuda, oracle, k):
+    torch = torch_cuda
+    B = _batch(k, 500 + k, ntask=24)
+    plan = _Plan(L, torch, B)
+    try:
+        u = torch.zeros(B["pool"], dtype=torch.float64, device="cuda")
+        eager = _replay_check(torch, lambda s: plan.run(u, s), u)
+        ref = _oracle_list(oracle, B, np.zeros(B["pool"]))
+        assert oracle.rel_l2(eager.cpu().numpy(), ref) <= TOL
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_dgsks_device_graph_capture_replays(L, torch_cuda, oracle, precision):
+    """The stream-ordered single summation, FP64 and mixed (its bucketing sort
+    and packing included), captured after one eager run on the same stream."""
+    from ks_amd import _lib, workloads
+    torch = torch_cuda
+    d = workloads.mixture_k64(N=8192, k=64)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    X, X2, w, ids = dev(d["X"]), dev(d["X2"]), dev(d["w"]), dev(d["amap"])
+    kt = _lib.KsT()
+    kt.type = 0
+    kt.scal = d["kernel"]["scal"]
+    u = torch.zeros(8192, dtype=torch.float64, device="cuda")
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+
+    def run_on(s):
+        return L.dgsks_device(ctypes.byref(kt), 8192, 8192, 64, P(u), P(ids), P(X), P(X2), P(ids),
+                              P(X), P(X2), P(ids), P(w), P(ids), precision,
+                              ctypes.c_void_p(s.cuda_stream))
+
+    eager = _replay_check(torch, run_on, u)
+    rows = np.arange(0, 8192, 37).astype(np.int32)
+    ref = oracle.dgsks_rows(0, d["X"], d["X2"], d["amap"], d["X"], d["X2"], d["amap"], d["w"],
+                            d["amap"], np.zeros(8192), d["amap"], rows, scal=d["kernel"]["scal"])
+    err = oracle.rel_l2(eager.cpu().numpy()[rows], ref)
+    assert err <= (TOL if precision == 0 else 1e-6)
+
+
+def test_autotune_choices(L, torch_cuda):
+    out = []
+    for t, k in [(0, 3), (2, 3), (0, 32), (2, 32), (1, 20)]:
+        sl, tl = ctypes.c_int(), ctypes.c_int()
+        assert L.ks_list_autotune(t, k, 0, ctypes.byref(sl), ctypes.byref(tl)) == 0
+        assert sl.value in (256, 512, 1024, 2048, 4096)
+        assert tl.value in (0, 1)
+        if t == 2 or k < 9:
+            assert tl.value == 0  # Laplace and k <= 8 have no matrix-core tile
+        sl2, tl2 = ctypes.c_int(), ctypes.c_int()
+        assert L.ks_list_autotune(t, k, 0, ctypes.byref(sl2), ctypes.byref(tl2)) == 0
+        assert (sl2.value, tl2.value) == (sl.value, tl.value)  # cached
+        out.append((t, k, sl.value, tl.value))
+    print("tuned shapes:", out)
+
+
+def test_autotune_cache_file(tmp_path, torch_cuda):
+    """KS_TUNE_CACHE: a fresh process tunes and appends its choice; a second
+    process reads it back without tuning (KS_AUTOTUNE=0 would otherwise give the
+    default shape)."""
+    cache = tmp_path / "tiles.txt"
+    code = ("import ctypes,sys; sys.path.insert(0, %r); from ks_amd import _lib; L=_lib.lib(); "
+            "a,b=ctypes.c_int(),ctypes.c_int(); "
+            "assert L.ks_list_autotune(4, 12, 0, ctypes.byref(a), ctypes.byref(b))==0; "
+            "print(a.value, b.value)") % ROOT
+    env = dict(os.environ, KS_TUNE_CACHE=str(cache))
+    r1 = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                        timeout=300)
+    assert r1.returncode == 0, r1.stderr
+    lines = cache.read_text().split("\n")
+    assert lines[0].split()[:3] == ["4", "12", "0"]
+    env2 = dict(env, KS_AUTOTUNE="0")
+    r2 = subprocess.run([sys.executable, "-c", code], env=env2, capture_output=True, text=True,
+                        timeout=300)
+    assert r2.returncode == 0, r2.stderr
+    assert r2.stdout.split() == r1.stdout.split()
+
+
+def test_mixed_batch_routing(L, torch_cuda, oracle):
+    """A mixed-precision batch: Laplace and polynomial tasks and small Gaussian
+    tasks run in the persistent FP64 launch, the large Gaussian tasks in the
+    mixed kernels; the whole batch stays within 1e-6 of the oracle."""
+    B = _batch(16, 600, ntask=30, big_gauss=2)
+    u = np.zeros(B["pool"])
+    ref = _oracle_list(oracle, B, u)
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    rc = L.dgsks_list(B["ntask"], B["kernels"], 16, p(u), u.size, p(B["a_off"]), p(B["amap"]),
+                      p(B["amap"]), p(B["b_off"]), p(B["bmap"]), p(B["bmap"]), p(B["X"]),
+                      p(B["X2"]), p(B["X"]), p(B["X2"]), p(B["w"]), 2)
+    assert rc == 0
+    err = oracle.rel_l2(u, ref)
+    print(f"mixed batch: rel L2 = {err:.3e}")
+    assert err <= 1e-6
