@@ -1,15 +1,19 @@
 // Batched summation (dgsks_list): thousands of independent summations of
 // uneven m, n over shared point pools in ONE persistent launch.
 //
-// Work unit = tile: 64 target rows of one task x one slice of at most `bc`
-// of that task's sources.  The host lays the tiles out once per call, largest
+// Work unit = tile: 64 target rows of one task x one slice of that task's
+// sources.  The host lays the tiles out once per batch (a plan), largest
 // first; every wave of the persistent grid pulls the next tile index from a
 // device-scope counter (atomicAdd) and runs it, so uneven tasks balance
-// across all 1024 SIMDs without host involvement.  Per tile:
-//   * k <= 8, Laplace at any k, k > 64: lane = target row, rows in VGPRs,
+// across all 1024 SIMDs without host involvement.  The slice length and the
+// tile kind are the autotuned tile shape of the task's (kernel type, k).  Per tile:
+//   * k = 3 or 8 (and k outside 9..64): lane = target row, the row in VGPRs,
 //     sources as wave-uniform scalar loads (as fused_f64_kernel);
-//   * 9 <= k <= 64 (other types): 4 blocks of 16 rows on the FP64 matrix
-//     cores (as fused_f64_mfma_kernel).
+//   * 9 <= k <= 64, row tile (always for Laplace, whose rank-k sum must keep
+//     the reference's order): lane = target row, the row in VGPRs padded to
+//     16 / 32 / 64 coordinates, sources staged through LDS;
+//   * 9 <= k <= 64, matrix-core tile (other types): 4 blocks of 16 rows on the
+//     FP64 matrix cores (as fused_f64_mfma_kernel).
 // Each tile writes 64 row partials; the last tile of a (task, row block) to
 // finish (device-scope counter) adds its partials in slice order into
 // R[a_off[t] + i].  The last wave to leave resets the tile counter, so the
@@ -63,7 +67,7 @@ struct RowBlock {
 };
 
 struct ListArgs {
-  int ntiles, k;
+  int tile0, ntiles, k;  // this launch's tiles: [tile0, tile0 + ntiles)
   const int2 *tiles;  // {row block, slice}
   const RowBlock *rbs;
   const DevTask *tasks;
@@ -137,74 +141,186 @@ __device__ double tile_valu(const ListArgs &L, const DevTask &T, int pos, int c0
   return acc;
 }
 
-// 64 rows as 4 blocks of 16 on the FP64 matrix cores; returns the sum of the
-// lane's row (lane = row) through the wave's LDS slot `red` (64 doubles).
+// Sources staged per wave in LDS: kStageDoubles doubles (a chunk of
+// kStageDoubles / K source rows of K padded coordinates), then 3 doubles per
+// source of the chunk (|b|^2, w, hj).
+constexpr int kStageDoubles = 1024;
+constexpr int kStageSide = 3 * 64;
+
+// lane = row, k <= K (K in {16, 32, 64}, coordinates past k are zero): the
+// row stays in VGPRs; the sources go through the wave's LDS slot in chunks of
+// kStageDoubles / K rows -- loaded with coalesced vector loads (consecutive
+// lanes read consecutive doubles of a row) and read back as wave-wide
+// broadcasts -- so every source row is fetched once per tile.  Zero padding
+// keeps the Laplace rank-k sum exact: the extra terms add +0.
+template <int TYPE, int K>
+__device__ double tile_valu_lds(const ListArgs &L, const DevTask &T, int pos, int c0, int c1,
+                                double *stage) {
+  constexpr int CH = kStageDoubles / K;
+  const int k = L.k;
+  const int lane = threadIdx.x & 63;
+  const int ai = L.amap[T.a0 + (pos < T.m ? pos : T.m - 1)];
+  const double *arow = L.XA + (long)ai * k;
+  double a[K];
+#pragma unroll
+  for (int p = 0; p < K; ++p) a[p] = p < k ? arow[p] : 0.0;
+  const double aa = L.XA2[ai];
+  const double hi = (TYPE == VAR_BANDWIDTH) ? -0.5 * T.hi[ai] : 0.0;
+  const KerParams P = T.P;
+  double *side = stage + kStageDoubles;  // [CH][3]: |b|^2, w, hj
+  double acc = 0.0;
+  for (int j0 = c0; j0 < c1; j0 += CH) {
+    const int nch = min(CH, c1 - j0);
+    double v[kStageDoubles / 64];
+#pragma unroll
+    for (int i = 0; i < kStageDoubles / 64; ++i) {
+      const int e = lane + 64 * i, r = e / K, p = e % K;
+      const int rr = r < nch ? r : nch - 1;
+      const double x = L.XB[(long)L.bmap[T.b0 + j0 + rr] * k + (p < k ? p : k - 1)];
+      v[i] = (r < nch && p < k) ? x : 0.0;
+    }
+    // lane r < CH: the per-source factors of source j0 + r
+    double sb = 0.0, sw = 0.0, sh = 0.0;
+    if (lane < nch) {
+      const int bj = L.bmap[T.b0 + j0 + lane];
+      sb = L.XB2[bj];
+      sw = L.w[L.wmap[T.b0 + j0 + lane]];
+      sh = (TYPE == VAR_BANDWIDTH) ? T.hj[bj] : 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();  // the previous chunk's broadcasts are done
+#pragma unroll
+    for (int i = 0; i < kStageDoubles / 64; ++i) stage[lane + 64 * i] = v[i];
+    if (lane < CH) {
+      side[3 * lane] = sb;
+      side[3 * lane + 1] = sw;
+      side[3 * lane + 2] = sh;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 1
+    for (int r = 0; r < nch; ++r) {
+      const double bb = side[3 * r], wj = side[3 * r + 1], hj = side[3 * r + 2];
+      const double *b = stage + r * K;
+      double c;
+      if constexpr (TYPE == LAPLACE) {
+#pragma clang fp contract(off)
+        c = a[0] * b[0];
+#pragma unroll
+        for (int p = 1; p < K; ++p) c = c + a[p] * b[p];
+      } else {
+        c = a[0] * b[0];
+#pragma unroll
+        for (int p = 1; p < K; ++p) c = fma(a[p], b[p], c);
+      }
+      acc = fma(kval_f64<TYPE>(c, aa, bb, hi, hj, P), wj, acc);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  return acc;
+}
+
+// 64 rows as 4 blocks of 16 on the FP64 matrix cores, in two passes of 2 row
+// blocks (the A fragments, norms and accumulators of 64 rows at once would not
+// fit in VGPRs beside the source fragments; the second pass re-reads the
+// slice's sources from L2); returns the sum of the lane's row (lane = row)
+// through the wave's LDS slot `red` (64 doubles).
 template <int TYPE, int KC>
 __device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c0, int c1,
                             double *red) {
+  constexpr int RBP = 2;
   const int k = L.k;
+  const bool exact = (k == 16 * KC);  // whole chunks: two 16-byte loads per chunk
   const int lane = threadIdx.x & 63, p16 = lane & 15, g = lane >> 4;
-  double af[4][KC][4], aa[4][4], hi[4][4], racc[4][4];
+  // coordinates c*16 + 4g .. +3 of point row `base`
+  auto frag = [&](const double *X, long base, double (&f)[KC][4]) {
 #pragma unroll
-  for (int rb = 0; rb < 4; ++rb) {
-    int r = row0 + rb * 16 + p16;
-    int ai = L.amap[T.a0 + (r < T.m ? r : T.m - 1)];
+    for (int c = 0; c < KC; ++c) {
+      if (exact) {
+        const double2 *q = reinterpret_cast<const double2 *>(X + base + c * 16 + 4 * g);
+        const double2 x0 = q[0], x1 = q[1];
+        f[c][0] = x0.x;
+        f[c][1] = x0.y;
+        f[c][2] = x1.x;
+        f[c][3] = x1.y;
+      } else {
 #pragma unroll
-    for (int c = 0; c < KC; ++c)
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        int idx = c * 16 + 4 * g + t;
-        af[rb][c][t] = idx < k ? L.XA[(long)ai * k + idx] : 0.0;
+        for (int t = 0; t < 4; ++t) {
+          int idx = c * 16 + 4 * g + t;
+          f[c][t] = idx < k ? X[base + idx] : 0.0;
+        }
       }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      int ro = row0 + rb * 16 + g + 4 * v;
-      int ao = L.amap[T.a0 + (ro < T.m ? ro : T.m - 1)];
-      aa[rb][v] = L.XA2[ao];
-      hi[rb][v] = (TYPE == VAR_BANDWIDTH) ? -0.5 * T.hi[ao] : 0.0;
-      racc[rb][v] = 0.0;
     }
-  }
+  };
   const KerParams P = T.P;
-  for (int jb = c0; jb < c1; jb += 16) {
-    const int col = jb + p16;
-    const bool ok = col < c1;
-    const int bj = L.bmap[T.b0 + (ok ? col : c1 - 1)];
-    double bf[KC][4];
+  const int jlast = c0 + ((c1 - c0 - 1) >> 4) * 16;
+#pragma unroll 1
+  for (int pass = 0; pass < 4 / RBP; ++pass) {
+    const int rbase = row0 + pass * RBP * 16;
+    double af[RBP][KC][4], aa[RBP][4], hi[RBP][4], racc[RBP][4];
 #pragma unroll
-    for (int c = 0; c < KC; ++c)
+    for (int rb = 0; rb < RBP; ++rb) {
+      int r = rbase + rb * 16 + p16;
+      int ai = L.amap[T.a0 + (r < T.m ? r : T.m - 1)];
+      frag(L.XA, (long)ai * k, af[rb]);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        int idx = c * 16 + 4 * g + t;
-        bf[c][t] = idx < k ? L.XB[(long)bj * k + idx] : 0.0;
+      for (int v = 0; v < 4; ++v) {
+        int ro = rbase + rb * 16 + g + 4 * v;
+        int ao = L.amap[T.a0 + (ro < T.m ? ro : T.m - 1)];
+        aa[rb][v] = L.XA2[ao];
+        hi[rb][v] = (TYPE == VAR_BANDWIDTH) ? -0.5 * T.hi[ao] : 0.0;
+        racc[rb][v] = 0.0;
       }
-    const double bb = L.XB2[bj];
-    const double wj = ok ? L.w[L.wmap[T.b0 + col]] : 0.0;
-    const double hj = (TYPE == VAR_BANDWIDTH) ? T.hj[bj] : 0.0;
-#pragma unroll
-    for (int rb = 0; rb < 4; ++rb) {
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int c = 0; c < KC; ++c)
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[rb][c][t], bf[c][t], acc, 0, 0, 0);
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        racc[rb][v] = fma(kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P), wj, racc[rb][v]);
     }
+    // two source-fragment buffers in turn: the next block's loads are in flight
+    // while this block runs (the last block's "next" is an in-bounds re-read)
+    double bf0[KC][4], bf1[KC][4];
+    double bb0, wj0, hj0, bb1, wj1, hj1;
+    auto fetch = [&](int jb, double (&f)[KC][4], double &b2, double &ww, double &hh) {
+      const int col = jb + p16;
+      const bool ok = col < c1;
+      const int bj = L.bmap[T.b0 + (ok ? col : c1 - 1)];
+      frag(L.XB, (long)bj * k, f);
+      b2 = L.XB2[bj];
+      ww = ok ? L.w[L.wmap[T.b0 + col]] : 0.0;
+      hh = (TYPE == VAR_BANDWIDTH) ? T.hj[bj] : 0.0;
+    };
+    auto block = [&](const double (&f)[KC][4], double bb, double wj, double hj) {
+#pragma unroll
+      for (int rb = 0; rb < RBP; ++rb) {
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[rb][c][t], f[c][t], acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          racc[rb][v] =
+              fma(kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P), wj, racc[rb][v]);
+      }
+    };
+    fetch(c0, bf0, bb0, wj0, hj0);
+    for (int jb = c0;; jb += 32) {
+      fetch(min(jb + 16, jlast), bf1, bb1, wj1, hj1);
+      block(bf0, bb0, wj0, hj0);
+      if (jb + 16 >= c1) break;
+      fetch(min(jb + 32, jlast), bf0, bb0, wj0, hj0);
+      block(bf1, bb1, wj1, hj1);
+      if (jb + 32 >= c1) break;
+    }
+#pragma unroll
+    for (int rb = 0; rb < RBP; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double s = racc[rb][v];
+        s += __shfl_xor(s, 8, 16);
+        s += __shfl_xor(s, 4, 16);
+        s += __shfl_xor(s, 2, 16);
+        s += __shfl_xor(s, 1, 16);
+        if (p16 == v) red[(pass * RBP + rb) * 16 + g + 4 * v] = s;
+      }
   }
-#pragma unroll
-  for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double s = racc[rb][v];
-      s += __shfl_xor(s, 8, 16);
-      s += __shfl_xor(s, 4, 16);
-      s += __shfl_xor(s, 2, 16);
-      s += __shfl_xor(s, 1, 16);
-      if (p16 == v) red[rb * 16 + g + 4 * v] = s;
-    }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   double mine = red[lane];
@@ -212,33 +328,47 @@ __device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c
   return mine;
 }
 
-template <int TYPE, int K>
+// MF = 1: matrix-core tiles (padded K = 16 -> one 16-coordinate chunk, 32 ->
+// two, 64 -> three or four; never Laplace); MF = 0: row tiles.
+template <int TYPE, int K, int MF>
 __device__ double run_tile(const ListArgs &L, const DevTask &T, int row0, int c0, int c1,
-                           double *red) {
-  if constexpr (TYPE != LAPLACE && K == 0) {
-    if (T.tile == 1 && L.k >= 9 && L.k <= 64) {
-      switch ((L.k + 15) / 16) {
-        case 1: return tile_mfma<TYPE, 1>(L, T, row0, c0, c1, red);
-        case 2: return tile_mfma<TYPE, 2>(L, T, row0, c0, c1, red);
-        case 3: return tile_mfma<TYPE, 3>(L, T, row0, c0, c1, red);
-        default: return tile_mfma<TYPE, 4>(L, T, row0, c0, c1, red);
-      }
+                           double *red, double *stage) {
+  if constexpr (MF == 1) {
+    if constexpr (TYPE == LAPLACE) {
+      return 0.0;  // not reached: the plan never puts a Laplace task on the matrix cores
+    } else if constexpr (K == 16) {
+      return tile_mfma<TYPE, 1>(L, T, row0, c0, c1, red);
+    } else if constexpr (K == 32) {
+      return tile_mfma<TYPE, 2>(L, T, row0, c0, c1, red);
+    } else {
+      if (L.k <= 48) return tile_mfma<TYPE, 3>(L, T, row0, c0, c1, red);
+      return tile_mfma<TYPE, 4>(L, T, row0, c0, c1, red);
     }
+  } else if constexpr (K >= 16) {
+    return tile_valu_lds<TYPE, K>(L, T, row0 + (threadIdx.x & 63), c0, c1, stage);
+  } else {
+    return tile_valu<TYPE, K>(L, T, row0 + (threadIdx.x & 63), c0, c1);
   }
-  return tile_valu<TYPE, K>(L, T, row0 + (threadIdx.x & 63), c0, c1);
 }
 
-template <int K>
+// One persistent launch over one tile kind (MF): the row tiles and the
+// matrix-core tiles of a batch run as two launches, so that each kernel's
+// register allocation is that of its own tile.
+template <int K, int MF>
 __global__ __launch_bounds__(256) void list_kernel(ListArgs L) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   __shared__ double s_red[4][64];
   double *red = s_red[wave];
+  constexpr int NST = (K >= 16 && MF == 0) ? kStageDoubles + kStageSide : 1;
+  __shared__ double s_stage[4][NST];
+  double *stage = s_stage[wave];
   while (true) {
     unsigned t = 0;
     if (lane == 0) t = atomicAdd(L.head, 1u);
     t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
     if (t >= (unsigned)L.ntiles) break;
+    t += (unsigned)L.tile0;
     const int2 tl = L.tiles[t];
     const RowBlock rb = L.rbs[tl.x];
     const DevTask T = L.tasks[rb.task];
@@ -246,14 +376,14 @@ __global__ __launch_bounds__(256) void list_kernel(ListArgs L) {
     const int c1 = min(T.n, c0 + T.bc);
     double s;
     switch (T.type) {
-      case GAUSSIAN: s = run_tile<GAUSSIAN, K>(L, T, rb.row0, c0, c1, red); break;
-      case POLYNOMIAL: s = run_tile<POLYNOMIAL, K>(L, T, rb.row0, c0, c1, red); break;
-      case LAPLACE: s = run_tile<LAPLACE, K>(L, T, rb.row0, c0, c1, red); break;
-      case VAR_BANDWIDTH: s = run_tile<VAR_BANDWIDTH, K>(L, T, rb.row0, c0, c1, red); break;
-      case TANH: s = run_tile<TANH, K>(L, T, rb.row0, c0, c1, red); break;
-      case QUARTIC: s = run_tile<QUARTIC, K>(L, T, rb.row0, c0, c1, red); break;
-      case MULTIQUADRATIC: s = run_tile<MULTIQUADRATIC, K>(L, T, rb.row0, c0, c1, red); break;
-      default: s = run_tile<EPANECHNIKOV, K>(L, T, rb.row0, c0, c1, red); break;
+      case GAUSSIAN: s = run_tile<GAUSSIAN, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+      case POLYNOMIAL: s = run_tile<POLYNOMIAL, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+      case LAPLACE: s = run_tile<LAPLACE, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+      case VAR_BANDWIDTH: s = run_tile<VAR_BANDWIDTH, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+      case TANH: s = run_tile<TANH, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+      case QUARTIC: s = run_tile<QUARTIC, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+      case MULTIQUADRATIC: s = run_tile<MULTIQUADRATIC, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+      default: s = run_tile<EPANECHNIKOV, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
     }
     const int pos = rb.row0 + lane;
     const bool valid = pos < T.m;
@@ -303,9 +433,9 @@ __global__ void merge_kernel(const int *keys, const int *pos, const double *R, l
 
 // ------------------------------------------------------------------ host side
 
-template <int K>
+template <int K, int MF>
 static void launch_list(const ListArgs &A, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(list_kernel<K>, dim3(grid), dim3(256), 0, s, A);
+  hipLaunchKernelGGL((list_kernel<K, MF>), dim3(grid), dim3(256), 0, s, A);
 }
 
 bool list_mfma_eligible(int type, int k) { return type != KS_LAPLACE && k >= 9 && k <= 64; }
@@ -315,9 +445,11 @@ bool list_mfma_eligible(int type, int k) { return type != KS_LAPLACE && k >= 9 &
 struct ListPlan {
   int k = 0, precision = KS_PREC_FP64;
   long total_a = 0, ntiles = 0;
-  int grid = 0;
   char *buf = nullptr;
-  ListArgs A{};  // layout part (tiles, row blocks, tasks, partials, counters, R)
+  // layout part (tiles, row blocks, tasks, partials, counters, R) of the row-tile
+  // launch [0] and the matrix-core launch [1]
+  ListArgs A[2]{};
+  int grid[2] = {0, 0};
   const int *keys = nullptr, *pos = nullptr;  // umap sorted, with positions
   std::vector<TaskDesc> mixed;                // tasks the mixed kernels run
   const int *umap = nullptr, *amap = nullptr, *bmap = nullptr, *wmap = nullptr;
@@ -343,7 +475,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
     dt[t] = DevTask{td.type, td.m, td.n, 1024, 0, 0, td.a0, td.b0, td.P, td.hi, td.hj};
     if (td.m == 0 || td.n == 0) continue;
     if (precision != KS_PREC_FP64 && (td.type == KS_GAUSSIAN || td.type == KS_GAUSSIAN_VAR_BANDWIDTH) &&
-        (long)td.m * td.n >= kMixedListMin) {
+        k >= kMixedListMinK && (long)td.m * td.n >= kMixedListMin) {
       pl->mixed.push_back(td);
       continue;
     }
@@ -353,7 +485,8 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
     const int nsplit = (td.n + dt[t].bc - 1) / dt[t].bc;
     for (int r0 = 0; r0 < td.m; r0 += 64) rbs.push_back(RowBlock{(int)t, r0, nsplit, 0});
   }
-  // order row blocks by work (columns per tile x rows) descending, then lay tiles out
+  // row blocks of row tiles first, then those of matrix-core tiles; within each
+  // kind by work (columns per tile x rows) descending; then lay tiles out
   std::vector<int> order(rbs.size());
   std::iota(order.begin(), order.end(), 0);
   auto work = [&](const RowBlock &r) {
@@ -362,15 +495,20 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
     long cols = std::min(T.bc, T.n);
     return rows * cols;
   };
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int x, int y) { return work(rbs[x]) > work(rbs[y]); });
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    const int kx = dt[rbs[x].task].tile, ky = dt[rbs[y].task].tile;
+    if (kx != ky) return kx < ky;
+    return work(rbs[x]) > work(rbs[y]);
+  });
   std::vector<RowBlock> rbs_sorted(rbs.size());
   for (size_t i = 0; i < order.size(); ++i) rbs_sorted[i] = rbs[order[i]];
   rbs.swap(rbs_sorted);
   std::vector<int2> tiles;
+  long nkind[2] = {0, 0};
   for (size_t i = 0; i < rbs.size(); ++i) {
     rbs[i].tile_base = (int)tiles.size();
     for (int sp = 0; sp < rbs[i].nsplit; ++sp) tiles.push_back(make_int2((int)i, sp));
+    nkind[dt[rbs[i].task].tile] += rbs[i].nsplit;
   }
   pl->ntiles = (long)tiles.size();
   if (pl->ntiles >= (1l << 31)) {
@@ -394,7 +532,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   size_t off_part = off_tasks + al(dt.size() * sizeof(DevTask));
   size_t off_cnt = off_part + al((size_t)pl->ntiles * 64 * sizeof(double));
   size_t off_head = off_cnt + al(rbs.size() * sizeof(unsigned));
-  size_t off_R = off_head + 256;
+  size_t off_R = off_head + 256;  // two launches' {head, done} counters
   size_t off_iota = off_R + al((size_t)total_a * sizeof(double));
   size_t off_kout = off_iota + al((size_t)total_a * sizeof(int));
   size_t off_vout = off_kout + al((size_t)total_a * sizeof(int));
@@ -415,17 +553,20 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
       hipMemcpyAsync(buf + off_tasks, dt.data(), dt.size() * sizeof(DevTask), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemsetAsync(buf + off_cnt, 0, off_iota - off_cnt, s) != hipSuccess)
     rc = KS_ERR_DEVICE;
-  ListArgs &A = pl->A;
-  A.ntiles = (int)pl->ntiles;
-  A.k = k;
-  A.tiles = (const int2 *)(buf + off_tiles);
-  A.rbs = (const RowBlock *)(buf + off_rbs);
-  A.tasks = (const DevTask *)(buf + off_tasks);
-  A.partial = (double *)(buf + off_part);
-  A.rbcount = (unsigned *)(buf + off_cnt);
-  A.head = (unsigned *)(buf + off_head);
-  A.done = A.head + 1;
-  A.R = (double *)(buf + off_R);
+  for (int kind = 0; kind < 2; ++kind) {
+    ListArgs &A = pl->A[kind];
+    A.tile0 = kind == 0 ? 0 : (int)nkind[0];
+    A.ntiles = (int)nkind[kind];
+    A.k = k;
+    A.tiles = (const int2 *)(buf + off_tiles);
+    A.rbs = (const RowBlock *)(buf + off_rbs);
+    A.tasks = (const DevTask *)(buf + off_tasks);
+    A.partial = (double *)(buf + off_part);
+    A.rbcount = (unsigned *)(buf + off_cnt);
+    A.head = (unsigned *)(buf + off_head) + 2 * kind;
+    A.done = A.head + 1;
+    A.R = (double *)(buf + off_R);
+  }
   // merge order: target ids stably sorted with their positions
   int *iota = (int *)(buf + off_iota);
   int *kout = (int *)(buf + off_kout);
@@ -445,8 +586,10 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   int dev = 0, ncu = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  long grid = std::min<long>((long)ncu * 4, (pl->ntiles + 3) / 4);
-  pl->grid = (int)std::max<long>(grid, 1);
+  for (int kind = 0; kind < 2; ++kind) {
+    long grid = std::min<long>((long)ncu * 4, (nkind[kind] + 3) / 4);
+    pl->grid[kind] = (int)std::max<long>(grid, 1);
+  }
   if (rc != KS_OK) {
     list_plan_free(pl);
     return rc;
@@ -457,23 +600,36 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
 
 int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
   if (pl.ntiles > 0) {
-    ListArgs A = pl.A;
-    A.amap = pl.amap;
-    A.bmap = pl.bmap;
-    A.wmap = pl.wmap;
-    A.XA = L.XA;
-    A.XA2 = L.XA2;
-    A.XB = L.XB;
-    A.XB2 = L.XB2;
-    A.w = L.w;
-    switch (pl.k) {
-      case 3: launch_list<3>(A, pl.grid, s); break;
-      case 8: launch_list<8>(A, pl.grid, s); break;
-      default: launch_list<0>(A, pl.grid, s); break;
+    for (int kind = 0; kind < 2; ++kind) {
+      ListArgs A = pl.A[kind];
+      if (A.ntiles == 0) continue;
+      A.amap = pl.amap;
+      A.bmap = pl.bmap;
+      A.wmap = pl.wmap;
+      A.XA = L.XA;
+      A.XA2 = L.XA2;
+      A.XB = L.XB;
+      A.XB2 = L.XB2;
+      A.w = L.w;
+      // k = 3 and 8: rows and sources in registers / scalar loads; 9 <= k <= 64:
+      // padded to 16, 32 or 64 (row tiles stage the sources in LDS); other k:
+      // generic row tiles.  Matrix-core tiles only exist for 9 <= k <= 64.
+      const int kk = pl.k, g = pl.grid[kind];
+      if (kind == 1) {
+        if (kk <= 16) launch_list<16, 1>(A, g, s);
+        else if (kk <= 32) launch_list<32, 1>(A, g, s);
+        else launch_list<64, 1>(A, g, s);
+      } else if (kk == 3) launch_list<3, 0>(A, g, s);
+      else if (kk == 8) launch_list<8, 0>(A, g, s);
+      else if (kk >= 9 && kk <= 16) launch_list<16, 0>(A, g, s);
+      else if (kk >= 17 && kk <= 32) launch_list<32, 0>(A, g, s);
+      else if (kk >= 33 && kk <= 64) launch_list<64, 0>(A, g, s);
+      else launch_list<0, 0>(A, g, s);
     }
+    const double *R = pl.A[0].R;
     const long nb = (pl.total_a + 255) / 256;
-    hipLaunchKernelGGL(merge_kernel, dim3((unsigned)nb), dim3(256), 0, s, pl.keys, pl.pos,
-                       (const double *)A.R, pl.total_a, L.u);
+    hipLaunchKernelGGL(merge_kernel, dim3((unsigned)nb), dim3(256), 0, s, pl.keys, pl.pos, R,
+                       pl.total_a, L.u);
     if (hipGetLastError() != hipSuccess) return KS_ERR_DEVICE;
   }
   for (const TaskDesc &t : pl.mixed) {

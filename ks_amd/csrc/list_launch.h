@@ -36,9 +36,12 @@ bool list_mfma_eligible(int type, int k);
 TileChoice list_tile_choice(int type, int k, int precision);
 TileChoice list_default_tile(int type, int k);
 
-// In a mixed-precision batch the Gaussian tasks with at least this many pairs
-// run through the mixed kernels; every other task runs in the persistent FP64 launch.
-constexpr long kMixedListMin = 1l << 22;
+// In a mixed-precision batch the Gaussian tasks with k >= kMixedListMinK and
+// at least kMixedListMin pairs run through the mixed kernels (one pre-pass and
+// launch sequence per task, paid back only on large tasks where the BF16 cross
+// term matters); every other task runs in the persistent FP64 launch.
+constexpr long kMixedListMin = 1l << 24;
+constexpr int kMixedListMinK = 16;
 
 // A batch layout on the device (list.hip).  Building it does the host work
 // (tile layout, descriptors, the merge order of umap) and synchronises; running

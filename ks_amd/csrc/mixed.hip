@@ -54,7 +54,8 @@ constexpr float kFarT = 16.0f;          // far: estimated x < -16 (natural-log u
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr int XWG = 256;                // 4 waves
 constexpr int kPivots = 128;            // bucketing pivots (at most)
-constexpr int kSortMin = 2048;          // bucket only when m and n are both at least this
+constexpr int kSortMin = 16384;         // bucket only when m and n are both at least this
+                                        // (its sorts cost ~15 launches per call)
 constexpr int kMixRB = 2;               // 16-row blocks per wave
 constexpr int kMixRowsPerWG = 4 * kMixRB * 16;
 
@@ -117,6 +118,13 @@ __global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p, int P,
 __global__ void mix_identity_kernel(int *v, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] = (int)i;
+}
+
+// Identity target and source orders in one launch.
+__global__ void mix_identity2_kernel(int *a, long m, int *b, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) a[i] = (int)i;
+  if (i < n) b[i] = (int)i;
 }
 
 // Writes point `pt` (coordinates x, or zeros when x is null) into the blocked
@@ -599,10 +607,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
 
   // ---- target and source orders
   if (!bucket) {
-    hipLaunchKernelGGL(mix_identity_kernel, dim3((unsigned)((p.m + 255) / 256)), dim3(256), 0, s,
-                       prow, (long)p.m);
-    hipLaunchKernelGGL(mix_identity_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, s,
-                       pcol, (long)p.n);
+    hipLaunchKernelGGL(mix_identity2_kernel, dim3((unsigned)((mx + 255) / 256)), dim3(256), 0, s,
+                       prow, (long)p.m, pcol, (long)p.n);
   } else {
     int *ids = (int *)(base + o_ids);
     unsigned *ka = (unsigned *)(base + o_ka), *kb = (unsigned *)(base + o_kb);

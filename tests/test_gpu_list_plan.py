@@ -31,10 +31,10 @@ def L():
     return _lib.lib()
 
 
-def _batch(k, seed, pool=5000, ntask=36, big_gauss=0):
+def _batch(k, seed, pool=9000, ntask=36, big_gauss=0):
     """Tasks over one pool: Gaussian / Laplace / polynomial kernels, shared
     targets, an empty task, sizes across the tile slices; `big_gauss` extra
-    Gaussian tasks of 2100 x 2100 (past the mixed-routing threshold)."""
+    Gaussian tasks of 4200 x 4200 (past the mixed-routing threshold)."""
     from ks_amd import _lib
     rng = np.random.default_rng(seed)
     X = rng.random((pool, k)) * (1.5 / np.sqrt(k))
@@ -42,8 +42,8 @@ def _batch(k, seed, pool=5000, ntask=36, big_gauss=0):
     for p in range(k):
         X2 += X[:, p] * X[:, p]
     w = rng.standard_normal(pool)
-    ms = list(rng.integers(1, 300, ntask)) + [2100] * big_gauss
-    ns = list(rng.integers(1, 2600, ntask)) + [2100] * big_gauss
+    ms = list(rng.integers(1, 300, ntask)) + [4200] * big_gauss
+    ns = list(rng.integers(1, 2600, ntask)) + [4200] * big_gauss
     ms[2] = 0
     kinds = [dict(type=0, scal=-1.5), dict(type=2), dict(type=1, scal=1.0 / k, cons=1.0, powe=2.0)]
     kernels_py = [kinds[t % 3] for t in range(ntask)] + [dict(type=0, scal=-40.0)] * big_gauss
