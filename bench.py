@@ -37,8 +37,8 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix peak
 # scaled by what the counter reads for a 1 GiB device copy (FETCH_SIZE x 2,
 # WRITE_SIZE x 1), keyed by (workload, precision); the summaries and commands
 # are in profiles/mixture_k64_fp64_and_mixed.md.
-TRAFFIC = {("mixture_k64", 0): 8.867e9,   # fused_f64_mfma_kernel
-           ("mixture_k64", 2): 3.745e9}   # mixed_kernel + mixed_pack_kernel
+TRAFFIC = {("mixture_k64", 0): 9.673e9,    # fused_f64_mfma_kernel
+           ("mixture_k64", 2): 2.326e10}   # mixed_kernel + pack + bucketing pre-pass
 # flops per kernel evaluation in the reference's model, test/test_dgsks.c:254-283
 FLOP_C = {0: 37, 1: 6, 2: 60, 3: 35, 4: 89, 5: 8, 6: 6, 7: 7}
 TYPE_NAMES = ["gaussian", "polynomial", "laplace", "var_bandwidth", "tanh", "quartic",
@@ -335,7 +335,9 @@ def main():
         return {"bound": "mfma", "pipe": "bf16 dense matrix" if precision == 2 else "fp32 matrix",
                 "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": TRAFFIC.get((args.workload, precision)), "flops_per_eval": per,
-                "kernel_ms": kernel_ms, "kernel": "gsks::mixed_kernel (+ mixed_pack_kernel)"}
+                "kernel_ms": kernel_ms,
+                "kernel": "gsks::mixed_kernel (+ pre-pass: mix_bucket_kernel, radix sorts, "
+                          "mixed_pack_kernel)"}
 
     mixed_ok = kd["type"] in (0, 3)
     want = args.precision

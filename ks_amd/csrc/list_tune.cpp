@@ -6,8 +6,9 @@
 // the kernel type (the cost of one evaluation) and on k (the cost of the cross
 // term and of the loads), so the shape is chosen per (kernel type, k,
 // precision) by timing every candidate on a synthetic batch of that type --
-// 32 tasks with m, n log-uniform in [128, 8192] over a pool of 16384 points
-// uniform in [0,1)^k -- and keeping the fastest.  Results are cached for the
+// 256 tasks with m, n log-uniform in [64, 8192] over a pool of 262144 points
+// uniform in [0,1)^k, i.e. enough tiles to fill the GPU several times and a pool
+// larger than L2 -- and keeping the fastest.  Results are cached for the
 // process and, when KS_TUNE_CACHE names a file, in that file (one line
 // "type k precision slice tile" per key), which later processes read first.
 // KS_AUTOTUNE=0 switches tuning off (the default shape is then used).
@@ -104,7 +105,7 @@ bool upload(T **dst, const std::vector<T> &src) {
 }
 
 int make_synthetic(int type, int k, Synthetic &S) {
-  const int N = 16384, ntask = 32;
+  const int N = 1 << 18, ntask = 256;
   Lcg r{0x6b73616d64ull + (unsigned long long)(type * 131 + k)};
   std::vector<double> X((size_t)N * k), X2(N), w(N), h(N);
   for (auto &x : X) x = r.next();
@@ -126,8 +127,8 @@ int make_synthetic(int type, int k, Synthetic &S) {
   int rc = make_params(&kd, k, P, true);
   if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return rc;
   for (int t = 0; t < ntask; ++t) {
-    const int m = (int)std::floor(std::exp(std::log(128.0) + r.next() * std::log(64.0)));
-    const int n = (int)std::floor(std::exp(std::log(128.0) + r.next() * std::log(64.0)));
+    const int m = (int)std::floor(std::exp(std::log(64.0) + r.next() * std::log(128.0)));
+    const int n = (int)std::floor(std::exp(std::log(64.0) + r.next() * std::log(128.0)));
     TaskDesc td;
     td.type = type;
     td.P = P;
