@@ -218,8 +218,25 @@ int make_params(const ks_t *ker, int k, KerParams &P, bool warn) {
 
 constexpr int kTargetWGs = 2048;  // ~8 workgroups (32 waves) per CU on 256 CUs
 constexpr int kMinCols = 128;     // keep each workgroup's source slice >= 128 points
+// symmetric sums keep one column partial per (row block, source): at most 4 GiB
+constexpr size_t kSymMaxColPartials = (size_t)1 << 29;
 
 int enqueue_fused(const Problem &p, hipStream_t s) {
+  if (sym_eligible(p)) {
+    // sources are the targets: each pair of different row blocks evaluated once
+    const SymPlan pl = plan_sym(p.m, kTargetWGs);
+    const size_t ncol = (size_t)pl.row_blocks * p.m;
+    if (ncol <= kSymMaxColPartials) {
+      double *partial = nullptr;
+      unsigned *counters = nullptr;
+      int rc = get_scratch(s, (size_t)pl.nsplit * p.m + ncol, (size_t)pl.row_blocks, &partial,
+                           &counters);
+      if (rc != KS_OK) return rc;
+      hipError_t e = launch_fused_f64_sym(p, pl, partial, counters,
+                                          partial + (size_t)pl.nsplit * p.m, s);
+      return e == hipSuccess ? KS_OK : hip_failure(e, "fused_f64_sym launch");
+    }
+  }
   const bool mfma = mfma_f64_eligible(p.type, p.k);
   FusedPlan plan = mfma ? plan_split(p.m, p.n, mfma_rows_per_wg(), kTargetWGs, kMinCols)
                         : plan_fused(p.m, p.n, p.k, kTargetWGs, kMinCols);

@@ -24,38 +24,7 @@
 
 namespace gsks {
 
-// Pointer into the constant address space: loads through it are known not to be
-// clobbered by the kernel, so wave-uniform loads become scalar (SMEM) loads.
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T *;
-
-template <typename T>
-__device__ __forceinline__ cptr<T> to_const(const T *p) {
-  return (cptr<T>)(p);
-}
-
 constexpr int WG = 256;
-
-// Rank-k sum taken exactly as the reference's loops do (a rounded product added
-// to a rounded running sum, p = 0..k-1, no FMA).  Used where a threshold on d^2
-// makes the result discontinuous (Laplace: d^2 < 1e-15 drops the pair,
-// frame/dgsks_ref.c:343-345): with the same sum and the reference's assembly
-// order, d^2 is bit-identical to the reference's, so the same pairs are dropped.
-template <int K>
-__device__ __forceinline__ double dot_exact(const double *a, cptr<double> b) {
-#pragma clang fp contract(off)
-  double c = a[0] * b[0];
-#pragma unroll
-  for (int p = 1; p < K; ++p) c = c + a[p] * b[p];
-  return c;
-}
-
-__device__ __forceinline__ double dot_exact_rt(const double *a, cptr<double> b, int k) {
-#pragma clang fp contract(off)
-  double c = a[0] * b[0];
-  for (int p = 1; p < k; ++p) c = c + a[p] * b[p];
-  return c;
-}
 
 template <int TYPE, int K, int R>
 __global__ __launch_bounds__(WG) void fused_f64_kernel(FusedArgs A) {

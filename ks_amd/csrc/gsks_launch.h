@@ -34,6 +34,20 @@ FusedPlan plan_fused(int m, int n, int k, int target_wgs, int min_cols);
 hipError_t launch_fused_f64(const Problem &p, const FusedPlan &plan, double *partial,
                             unsigned *counters, hipStream_t s);
 
+// Symmetric sums (fused_f64_sym.hip): sources == targets, each pair of
+// different row blocks evaluated once for both of its terms.
+struct SymPlan {
+  int rows_per_wg;
+  int row_blocks;
+  int cols_per_split;  // a multiple of rows_per_wg
+  int nsplit;
+};
+bool sym_eligible(const Problem &p);
+SymPlan plan_sym(int m, int target_wgs);
+// partial: [nsplit][m]; counters: [row_blocks] (zero); colp: [row_blocks][m]
+hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *partial,
+                                unsigned *counters, double *colp, hipStream_t s);
+
 // FP64 cross term on the matrix cores (fused_f64_mfma.hip)
 bool mfma_f64_eligible(int type, int k);
 int mfma_rows_per_wg();

@@ -40,14 +40,6 @@ namespace gsks {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T *;
-
-template <typename T>
-__device__ __forceinline__ cptr<T> to_const(const T *p) {
-  return (cptr<T>)(p);
-}
-
 struct DevTask {
   int type;
   int m, n;

@@ -27,14 +27,6 @@
 
 namespace gsks {
 
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T *;
-
-template <typename T>
-__device__ __forceinline__ cptr<T> to_const(const T *p) {
-  return (cptr<T>)(p);
-}
-
 struct MrhsArgs {
   Problem p;  // p.u / p.w are the [npoints][nrhs] matrices U / W
   int nrhs;

@@ -183,7 +183,9 @@ void run_staged(const ks_t *kernel, int m, int n, int k, double *u, const int *u
   p.w = (const double *)up(w, nw * 8);
   p.amap = (const int *)up(amap, (size_t)m * 4);
   p.umap = (const int *)up(ident.data(), (size_t)m * 4);
-  p.bmap = (const int *)up(bmap, (size_t)n * 4);
+  // one copy of a map that serves as both amap and bmap keeps the sum
+  // recognisably symmetric on the device
+  p.bmap = (bmap == amap && n == m) ? p.amap : (const int *)up(bmap, (size_t)n * 4);
   p.wmap = (const int *)up(wmap, (size_t)n * 4);
   double *du = (double *)up(upack.data(), (size_t)m * 8);
   p.u = du;
