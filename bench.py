@@ -222,6 +222,42 @@ def bench_list(args, d, L):
     print(json.dumps(out), flush=True)
 
 
+def bench_single(L, kt, d, k, amap, bmap, X, w, stream, steps, u64):
+    """The single-precision front end (sgsks_device) on float copies of the same
+    inputs, norms left to the library, in the precision sgsks() would pick:
+    per-call time (HIP events, widening and scatter included) and the relative
+    difference from the FP64 result on the double inputs."""
+    import torch
+    from ks_amd import _lib
+    Xf, wf = X.float(), w.float()
+    uf = torch.zeros(X.shape[0], dtype=torch.float32, device="cuda")
+    m, n = amap.numel(), bmap.numel()
+    prec = 2 if (kt.type in (0, 3) and 16 <= k <= 128 and min(m, n) >= 16384) else 0
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    argv = (ctypes.byref(kt), m, n, k, P(uf), P(amap), P(Xf), None, P(amap), P(Xf), None,
+            P(bmap), P(wf), P(bmap), prec, ctypes.c_void_p(stream.cuda_stream))
+
+    def step():
+        rc = L.sgsks_device(*argv)
+        if rc != 0:
+            raise RuntimeError(f"sgsks_device failed: {_lib.error_string(rc)}")
+
+    step()
+    torch.cuda.synchronize()
+    rel = float(torch.linalg.norm(uf.double() - u64) / torch.linalg.norm(u64))
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(max(3, steps // 2))]
+    for a, b in ev:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    return {"entry": "sgsks_device", "precision": {0: "fp64", 2: "mixed_bf16x3"}[prec],
+            "ms_per_call": ms, "evals_per_s": float(m) * float(n) / (ms / 1e3),
+            "rel_diff_vs_fp64_on_double_inputs": rel}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,7 +286,8 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the fused summation is a single-GPU kernel "
-              "here (multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "here (it would shard by target rows and source slices; not run)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -393,6 +430,8 @@ def main():
         "kernel_evals_per_s": evals / (kernel_ms / 1e3),
     }
     out.update(extra)
+    out["single"] = bench_single(L, kt, d, k, amap, bmap, X, w, stream, args.steps,
+                                 one_result(0))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(d, budget_s=args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -21,7 +21,7 @@
  *   - KS_MULTIQUADRATIC is d^2 + cons (no square root);
  *   - m, n or k == 0 returns early.
  * Pointers may be host memory (the library stages them through HBM) or
- * device memory (used in place, no copies); see INTEGRATION.md.
+ * device memory (used in place, no copies).
  *
  * Everything after the "extensions" marker has no counterpart in the reference.
  */
@@ -127,7 +127,9 @@ void dgsks_ref_wrapper(int m, int n, int k, double *u, int *umap, double *XA, do
 /* Error reporting.  The reference prints and calls exit(1) on an illegal kernel
  * type (frame/dgsks.c:465-467); that stays the default (KS_ERRMODE_EXIT).  With
  * KS_ERRMODE_RETURN the call prints nothing, leaves u untouched and records the
- * code, which ks_last_error() returns (per thread) and then clears. */
+ * code, which ks_last_error() returns (per thread) and then clears.  The void
+ * entry points (dgsks, dgsks_ref, sgsks and their wrappers) clear it when they
+ * start, so after one of them it describes that call. */
 enum {
   KS_OK = 0,
   KS_ERR_INVALID_TYPE = 1,   /* kernel->type outside ks_type */
@@ -166,6 +168,33 @@ int dgsks_device(const ks_t *kernel, int m, int n, int k,
                  const double *XA, const double *XA2, const int *amap,
                  const double *XB, const double *XB2, const int *bmap,
                  const double *w, const int *wmap, int precision, void *stream);
+
+/* Single precision.  The reference declares a single-precision micro-kernel
+ * for every kernel type and implements none of them
+ * (micro_kernel/x86_64/haswell/gaussian_int_d8x6.c:38-54: "not yet
+ * implemented"); these take dgsks()'s arguments (frame/dgsks.c:373-390) as
+ * float arrays.  The rows the maps select are widened to FP64 (exactly), the
+ * summation runs in `precision` (mixed precision: Gaussian kernels, k <= 128),
+ * and u[umap[i]] receives float(u[umap[i]] + row sum i), one rounding.  XA2 /
+ * XB2 may be NULL: the squared norms are then computed in FP64 from the float
+ * coordinates.  hi / hj (variable bandwidth) stay double.  sgsks_device takes
+ * device pointers and is enqueued on `stream` (NULL = the HIP null stream); a
+ * umap entry may not repeat there.  sgsks takes all-host or all-device
+ * pointers, synchronises, and picks KS_PREC_MIXED_BF16X3 for the Gaussian
+ * kernels at 16 <= k <= 128 with m, n >= 16384 (where clustered points gain
+ * from it) and KS_PREC_FP64 otherwise; a repeated umap entry keeps the value
+ * of its last row, as in dgsks. */
+int sgsks_device(const ks_t *kernel, int m, int n, int k,
+                 float *u, const int *umap,
+                 const float *XA, const float *XA2, const int *amap,
+                 const float *XB, const float *XB2, const int *bmap,
+                 const float *w, const int *wmap, int precision, void *stream);
+void sgsks(ks_t *kernel, int m, int n, int k, float *u, int *umap, float *XA, float *XA2,
+           int *amap, float *XB, float *XB2, int *bmap, float *w, int *wmap);
+/* sgsks with the descriptor built from its fields (as dgsks_wrapper). */
+void sgsks_wrapper(int m, int n, int k, float *u, int *umap, float *XA, float *XA2,
+                   int *alpha, float *XB, float *XB2, int *beta, float *w, int *omega,
+                   int type, double scal, double cons, double powe, double *h);
 
 /* Squared row norms X2[i] = sum_p X[i*k+p]^2 on the device (the norm
  * precompute of frame/omp_dgsks_list.cpp:75-81), enqueued on `stream`

@@ -48,6 +48,11 @@ _SIGS = {
     "ks_get_error_mode": (ctypes.c_int, []),
     "dgsks_device": (ctypes.c_int, [ctypes.POINTER(KsT), ctypes.c_int, ctypes.c_int, ctypes.c_int] +
                      [_vp] * 10 + [ctypes.c_int, _vp]),
+    "sgsks_device": (ctypes.c_int, [ctypes.POINTER(KsT), ctypes.c_int, ctypes.c_int, ctypes.c_int] +
+                     [_vp] * 10 + [ctypes.c_int, _vp]),
+    "sgsks": (None, [ctypes.POINTER(KsT), ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_vp] * 10),
+    "sgsks_wrapper": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_vp] * 10 +
+                      [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp]),
     "ks_sqnorm_device": (ctypes.c_int, [_vp, ctypes.c_long, ctypes.c_int, _vp, _vp]),
     "dgsks_list": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(KsT), ctypes.c_int, _vp,
                                   ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

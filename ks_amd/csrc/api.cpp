@@ -77,6 +77,8 @@ struct Scratch {
   size_t partial_elems = 0;
   unsigned *counters = nullptr;
   size_t counter_elems = 0;
+  char *aux = nullptr;
+  size_t aux_bytes = 0;
 };
 static std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
 
@@ -144,6 +146,27 @@ int get_scratch(hipStream_t s, size_t partial_elems, size_t counter_elems, doubl
   }
   *partial = sc.partial;
   *counters = sc.counters;
+  return KS_OK;
+}
+
+int get_aux(hipStream_t s, size_t bytes, char **base) {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  std::lock_guard<std::mutex> lk(g_mu);
+  Scratch &sc = g_scratch[{d, s}];
+  if (sc.aux_bytes < bytes) {
+    hipError_t se = hipStreamSynchronize(s);
+    if (se != hipSuccess) return hip_failure(se, "get_aux: hipStreamSynchronize");
+    if (sc.aux) (void)hipFree(sc.aux);
+    sc.aux = nullptr;
+    size_t want = bytes + bytes / 4 + 256;
+    if (hipMalloc(&sc.aux, want) != hipSuccess) {
+      sc.aux_bytes = 0;
+      return KS_ERR_ALLOC;
+    }
+    sc.aux_bytes = want;
+  }
+  *base = sc.aux;
   return KS_OK;
 }
 
@@ -339,8 +362,103 @@ int dgsks_device(const ks_t *kernel, int m, int n, int k, double *u, const int *
   return KS_OK;
 }
 
+// Checks shared by sgsks_device() and sgsks() (the host path checks its arrays itself).
+static int single_precision_ok(const char *name, int type, int k, int precision) {
+  if (precision == KS_PREC_FP64) return KS_OK;
+  if (precision != KS_PREC_MIXED_FP32 && precision != KS_PREC_MIXED_BF16X3)
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): unknown precision\n", name);
+  if (type != KS_GAUSSIAN && type != KS_GAUSSIAN_VAR_BANDWIDTH)
+    return raise_error(KS_ERR_INVALID_ARG, false,
+                       "%s(): mixed precision supports the Gaussian and variable-bandwidth "
+                       "kernels only\n", name);
+  if (k > 128)
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): mixed precision supports k <= 128\n",
+                       name);
+  return KS_OK;
+}
+
+int sgsks_device(const ks_t *kernel, int m, int n, int k, float *u, const int *umap,
+                 const float *XA, const float *XA2, const int *amap, const float *XB,
+                 const float *XB2, const int *bmap, const float *w, const int *wmap,
+                 int precision, void *stream) {
+  if (!kernel || m < 0 || n < 0 || k < 0)
+    return raise_error(KS_ERR_INVALID_ARG, false, "sgsks_device(): invalid argument\n");
+  if (m == 0 || n == 0 || k == 0) return KS_OK;
+  if (!u || !umap || !XA || !amap || !XB || !bmap || !w || !wmap)
+    return raise_error(KS_ERR_INVALID_ARG, false, "sgsks_device(): NULL array\n");
+  int rc = ensure_device();
+  if (rc != KS_OK) return raise_error(rc, false, "sgsks_device(): no GPU available\n");
+  SingleProblem p;
+  rc = make_params(kernel, k, p.P, true);
+  if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return rc;
+  rc = single_precision_ok("sgsks_device", (int)kernel->type, k, precision);
+  if (rc != KS_OK) return rc;
+  const bool varbw = kernel->type == KS_GAUSSIAN_VAR_BANDWIDTH;
+  p.m = m;
+  p.n = n;
+  p.k = k;
+  p.type = (int)kernel->type;
+  p.u = u;
+  p.umap = umap;
+  p.XA = XA;
+  p.XA2 = XA2;
+  p.amap = amap;
+  p.XB = XB;
+  p.XB2 = XB2;
+  p.bmap = bmap;
+  p.w = w;
+  p.wmap = wmap;
+  p.hi = varbw ? kernel->hi : nullptr;
+  p.hj = varbw ? kernel->hj : nullptr;
+  const void *arrays[] = {u, umap, XA, XA2, amap, XB, XB2, bmap, w, wmap, p.hi, p.hj};
+  for (const void *a : arrays)
+    if (a && !device_ptr_cached(a))
+      return raise_error(KS_ERR_MIXED_POINTERS, false,
+                         "sgsks_device(): every array must be device memory\n");
+  const int r = enqueue_single(p, precision, (hipStream_t)stream);
+  if (r != KS_OK) return raise_error(r, false, "sgsks_device(): launch failed (%s)\n", err_string(r));
+  return KS_OK;
+}
+
+void sgsks(ks_t *kernel, int m, int n, int k, float *u, int *umap, float *XA, float *XA2,
+           int *amap, float *XB, float *XB2, int *bmap, float *w, int *wmap) {
+  set_last_error(KS_OK);
+  // the early return and the Laplace side effect of dgsks(), frame/dgsks.c:394-397,445-447
+  if (m == 0 || n == 0 || k == 0) {
+    if (g_errmode.load() == KS_ERRMODE_EXIT) printf("sgsks(): early return\n");
+    return;
+  }
+  if (!kernel || m < 0 || n < 0 || k < 0) {
+    raise_error(KS_ERR_INVALID_ARG, false, "sgsks(): invalid argument\n");
+    return;
+  }
+  KerParams P;
+  int rc = make_params(kernel, k, P, true);
+  if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return;
+  if (kernel->type == KS_LAPLACE) {
+    kernel->powe = P.powe;
+    kernel->scal = P.scal;
+  }
+  run_staged_single(kernel, m, n, k, u, umap, XA, XA2, amap, XB, XB2, bmap, w, wmap);
+}
+
+void sgsks_wrapper(int m, int n, int k, float *u, int *umap, float *XA, float *XA2, int *alpha,
+                   float *XB, float *XB2, int *beta, float *w, int *omega, int type, double scal,
+                   double cons, double powe, double *h) {
+  ks_t kernel;
+  memset(&kernel, 0, sizeof(kernel));
+  kernel.type = (ks_type)type;
+  kernel.scal = scal;
+  kernel.cons = cons;
+  kernel.powe = powe;
+  kernel.hi = h;
+  kernel.hj = h;
+  sgsks(&kernel, m, n, k, u, umap, XA, XA2, alpha, XB, XB2, beta, w, omega);
+}
+
 void dgsks(ks_t *kernel, int m, int n, int k, double *u, int *umap, double *XA, double *XA2,
            int *amap, double *XB, double *XB2, int *bmap, double *w, int *wmap) {
+  set_last_error(KS_OK);
   // frame/dgsks.c:394-397
   if (m == 0 || n == 0 || k == 0) {
     if (g_errmode.load() == KS_ERRMODE_EXIT) printf("dgsks(): early return\n");
@@ -364,6 +482,7 @@ void dgsks(ks_t *kernel, int m, int n, int k, double *u, int *umap, double *XA, 
 void dgsks_ref(ks_t *kernel, int m, int n, int k, double *u, int *umap, double *XA,
                double *XA2, int *amap, double *XB, double *XB2, int *bmap, double *w,
                int *wmap) {
+  set_last_error(KS_OK);
   if (!kernel || m <= 0 || n <= 0 || k <= 0) return;  // frame/dgsks_ref.c has no early-return print
   KerParams P;
   int rc = make_params(kernel, k, P, true);

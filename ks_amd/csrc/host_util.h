@@ -16,6 +16,9 @@ int ensure_device();
 hipStream_t thread_stream();
 int get_scratch(hipStream_t s, size_t partial_elems, size_t counter_elems, double **partial,
                 unsigned **counters);
+// Grow-only byte arena of one (device, stream), separate from the scratch above,
+// for front ends that re-lay out their inputs before a summation.
+int get_aux(hipStream_t s, size_t bytes, char **base);
 void laplace_params(int k, double *powe, double *scal);
 int make_params(const ks_t *ker, int k, KerParams &P, bool warn);
 
@@ -28,12 +31,39 @@ int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s);
 // Mixed-precision summation (KS_PREC_MIXED_*).
 int enqueue_mixed(const Problem &p, int precision, hipStream_t s);
 
+// Single-precision problem (sgsks / sgsks_device): float arrays, FP64 bandwidths.
+struct SingleProblem {
+  int m, n, k;
+  int type;
+  KerParams P;
+  float *u;
+  const int *umap;
+  const float *XA, *XA2;  // XA2 / XB2 may be NULL (norms computed in FP64)
+  const int *amap;
+  const float *XB, *XB2;
+  const int *bmap;
+  const float *w;
+  const int *wmap;
+  const double *hi, *hj;
+};
+// Widen the selected rows to FP64, sum with the FP64 (precision KS_PREC_FP64)
+// or mixed kernels, and add the row sums into the float u, on stream s.
+int enqueue_single(const SingleProblem &p, int precision, hipStream_t s);
+// The precision sgsks() uses: mixed BF16x3 for the Gaussian kernels at
+// 16 <= k <= 128 with m, n >= 16384, FP64 otherwise.
+int single_default_precision(int type, int m, int n, int k);
+
 // Host entry behind dgsks()/dgsks_ref(): all-device pointers run in place,
 // all-host pointers are staged through HBM, then the call synchronises.
 void run_staged(const ks_t *kernel, int m, int n, int k, double *u, const int *umap,
                 const double *XA, const double *XA2, const int *amap, const double *XB,
                 const double *XB2, const int *bmap, const double *w, const int *wmap,
                 bool reference_algorithm);
+
+// Host entry behind sgsks(): as run_staged, with float arrays.
+void run_staged_single(const ks_t *kernel, int m, int n, int k, float *u, const int *umap,
+                       const float *XA, const float *XA2, const int *amap, const float *XB,
+                       const float *XB2, const int *bmap, const float *w, const int *wmap);
 
 // True when p is device (or managed) memory that kernels can dereference.
 bool is_device_ptr(const void *p);

@@ -4,6 +4,7 @@ arguments and option names, same shape/type assertions.
 
     dgsks(d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, **options)
     dgsks_ref(d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, **options)
+    sgsks(d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, **options)   # float32
 
 u[mu[i]] += sum_j K(XA[alpha[i]], XB[beta[j]]) * w[omega[j]], in place.
 options: type in {"Gaussian", "Polynomial", "Laplace", "Variable_Bandwidth",
@@ -55,30 +56,31 @@ def _size(x) -> int:
     return int(x.numel()) if _is_torch(x) else int(np.asarray(x).size)
 
 
-def _check_args(d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega):
-    """The range and type checks of python/gsks.py:19-36."""
+def _check_args(d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, single=False):
+    """The range and type checks of python/gsks.py:19-36 (float32 arrays and
+    optional norms for the single-precision call)."""
     M, N = XA.shape[0], XB.shape[0]
     m, n = _size(alpha), _size(beta)
     _require(XA.shape[1] == XB.shape[1], "XA.shape[1] == XB.shape[1]")
     _require(M >= m, "M >= m")
     _require(N >= n, "N >= n")
-    _require(_size(XA2) == M, "XA2.size == M")
-    _require(_size(XB2) == N, "XB2.size == N")
+    _require((single and XA2 is None) or _size(XA2) == M, "XA2.size == M")
+    _require((single and XB2 is None) or _size(XB2) == N, "XB2.size == N")
+    real = [("XA", XA), ("XB", XB), ("XA2", XA2), ("XB2", XB2), ("u", u), ("w", w)]
+    ints = [("alpha", alpha), ("beta", beta), ("mu", mu), ("omega", omega)]
     if _is_torch(XA):
         import torch
-        for name, t, dt in (("XA", XA, torch.float64), ("XB", XB, torch.float64),
-                            ("XA2", XA2, torch.float64), ("XB2", XB2, torch.float64),
-                            ("u", u, torch.float64), ("w", w, torch.float64),
-                            ("alpha", alpha, torch.int32), ("beta", beta, torch.int32),
-                            ("mu", mu, torch.int32), ("omega", omega, torch.int32)):
+        fdt = torch.float32 if single else torch.float64
+        for (name, t), dt in [(r, fdt) for r in real] + [(i, torch.int32) for i in ints]:
+            if t is None:
+                continue
             _require(t.dtype == dt, f"{name}.dtype == {dt}")
             _require(t.is_contiguous(), f"{name} is contiguous")
     else:
-        for name, a, dt in (("XA", XA, np.float64), ("XB", XB, np.float64),
-                            ("XA2", XA2, np.float64), ("XB2", XB2, np.float64),
-                            ("u", u, np.float64), ("w", w, np.float64),
-                            ("alpha", alpha, np.int32), ("beta", beta, np.int32),
-                            ("mu", mu, np.int32), ("omega", omega, np.int32)):
+        fdt = np.float32 if single else np.float64
+        for (name, a), dt in [(r, fdt) for r in real] + [(i, np.int32) for i in ints]:
+            if a is None:
+                continue
             _require(a.dtype == dt, f"{name}.dtype == {np.dtype(dt).name}")
             _require(a.flags.c_contiguous, f"{name} is C-contiguous")
     _require(_size(mu) == m, "mu.size == alpha.size")
@@ -112,7 +114,8 @@ def make_kernel(options) -> KsT | None:
 
 
 def _call(fn_name, d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, options):
-    m, n = _check_args(d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega)
+    single = fn_name == "sgsks"
+    m, n = _check_args(d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, single)
     kt = make_kernel(options)
     if kt is None:
         # python/gsks.py:80-82
@@ -120,17 +123,21 @@ def _call(fn_name, d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, options):
         return
     L = _lib.lib()
     args = [_ptr(a) for a in (u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega)]
-    precision = PRECISIONS[options.get("precision", "fp64")]
+    given = options.get("precision")
+    precision = PRECISIONS[given or "fp64"]
     if _is_torch(u) and u.is_cuda:
         import torch
         stream = ctypes.c_void_p(torch.cuda.current_stream(u.device).cuda_stream)
-        if fn_name == "dgsks":
-            rc = L.dgsks_device(ctypes.byref(kt), m, n, int(d), *args, precision, stream)
-            _lib.check(rc, "dgsks_device")
+        if fn_name == "dgsks" or (single and given is not None):
+            rc = getattr(L, fn_name + "_device")(ctypes.byref(kt), m, n, int(d), *args,
+                                                 precision, stream)
+            _lib.check(rc, fn_name + "_device")
             return
         torch.cuda.current_stream(u.device).synchronize()
-    _require(precision == _lib.KS_PREC_FP64 or fn_name != "dgsks",
-             "mixed precision needs device tensors")
+    if single:  # the host sgsks() picks its precision itself
+        _require(given is None, "sgsks: precision= needs device tensors")
+    _require(given is None or precision == _lib.KS_PREC_FP64 or fn_name == "dgsks_ref",
+             "an explicit mixed precision needs device tensors")
     getattr(L, fn_name)(ctypes.byref(kt), m, n, int(d), *args)
     code = L.ks_last_error()
     if code == _lib.KS_ERR_LAPLACE_K:  # the reference prints and carries on
@@ -149,6 +156,15 @@ def dgsks_ref(d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, **options):
     """The reference algorithm (unfused: cross-term GEMM panel, elementwise
     kernel, GEMV) on the GPU (python/gsks.py:18-105)."""
     _call("dgsks_ref", d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, options)
+
+
+def sgsks(d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, **options):
+    """Single precision: float32 arrays (XA2 / XB2 may be None: norms computed
+    in FP64), summed on the GPU after an exact widening to FP64.  Without
+    precision= the library picks mixed BF16x3 for the Gaussian kernels at
+    16 <= d <= 128 with at least 16384 targets and sources, FP64 otherwise;
+    precision= (device tensors) forces one."""
+    _call("sgsks", d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, options)
 
 
 def sqnorm(X):

@@ -49,11 +49,13 @@ double orc_pair(const orc_kernel *ker, double c, double aa, double bb, double hi
     case ORC_GAUSSIAN: /* :254-270 */
       C += aa;
       C += bb;
+      if (ker->clamp && C < 0.0) C = 0.0;
       C *= ker->scal;
       return exp(C);
     case ORC_GAUSSIAN_VAR_BANDWIDTH: /* :271-288 */
       C += aa;
       C += bb;
+      if (ker->clamp && C < 0.0) C = 0.0;
       C *= -0.5;
       C *= hi_i;
       C *= hj_j;
@@ -80,6 +82,7 @@ double orc_pair(const orc_kernel *ker, double c, double aa, double bb, double hi
     case ORC_QUARTIC: /* :379-394 */
       C += aa;
       C += bb;
+      if (ker->clamp && C < 0.0) C = 0.0;
       if (C < 1.0) {
         C = 1.0 - C;
         return (15.0 / 16.0) * C * C;
@@ -88,10 +91,12 @@ double orc_pair(const orc_kernel *ker, double c, double aa, double bb, double hi
     case ORC_MULTIQUADRATIC: /* :395-404 */
       C += aa;
       C += bb;
+      if (ker->clamp && C < 0.0) C = 0.0;
       return C + ker->cons;
     case ORC_EPANECHNIKOV: /* :405-420 */
       C += aa;
       C += bb;
+      if (ker->clamp && C < 0.0) C = 0.0;
       if (C < 1.0) return (3.0 / 4.0) * (1.0 - C);
       return 0.0;
     default:

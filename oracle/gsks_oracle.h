@@ -45,6 +45,13 @@ typedef struct {
   double cons;
   const double *hi; /* indexed by global point id (VAR_BANDWIDTH only) */
   const double *hj;
+  /* 0: dgsks_ref's semantics (no clamp).  1: the fused micro-kernels' --
+   * d^2 = max(d^2, 0) before the Gaussian, variable-bandwidth, quartic,
+   * multiquadric and Epanechnikov kernels
+   * (micro_kernel/x86_64/haswell/sq2nrm_int_d8x6.h:64-78), which is what
+   * dgsks() computes; Laplace is unaffected (d^2 < 1e-15 is replaced either
+   * way). */
+  int clamp;
 } orc_kernel;
 
 /* Laplace parameters the reference writes into ks_t (frame/dgsks_ref.c:146-148). */

@@ -34,7 +34,7 @@ _lp = ctypes.POINTER(ctypes.c_long)
 
 class OrcKernel(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int), ("powe", ctypes.c_double), ("scal", ctypes.c_double),
-                ("cons", ctypes.c_double), ("hi", _dp), ("hj", _dp)]
+                ("cons", ctypes.c_double), ("hi", _dp), ("hj", _dp), ("clamp", ctypes.c_int)]
 
 
 class RefKsT(ctypes.Structure):
@@ -150,8 +150,8 @@ def sqnorm(X):
     return out
 
 
-def _kernel(ktype, scal=0.0, cons=0.0, powe=0.0, hi=None, hj=None):
-    kr = OrcKernel(int(ktype), float(powe), float(scal), float(cons), None, None)
+def _kernel(ktype, scal=0.0, cons=0.0, powe=0.0, hi=None, hj=None, clamp=False):
+    kr = OrcKernel(int(ktype), float(powe), float(scal), float(cons), None, None, int(clamp))
     keep = []
     if hi is not None:
         hi = _f64(hi)
@@ -164,16 +164,20 @@ def _kernel(ktype, scal=0.0, cons=0.0, powe=0.0, hi=None, hj=None):
     return kr, keep
 
 
-def pair(ktype, c, aa, bb, hi=0.0, hj=0.0, *, scal=0.0, cons=0.0, powe=0.0):
-    """K of one pair from c = a.b, |a|^2, |b|^2 (frame/dgsks_ref.c:253-424)."""
-    kr, _ = _kernel(ktype, scal, cons, powe)
+def pair(ktype, c, aa, bb, hi=0.0, hj=0.0, *, scal=0.0, cons=0.0, powe=0.0, clamp=False):
+    """K of one pair from c = a.b, |a|^2, |b|^2 (frame/dgsks_ref.c:253-424);
+    clamp=True: d^2 clamped at 0 as the fused micro-kernels do
+    (micro_kernel/x86_64/haswell/sq2nrm_int_d8x6.h:64-78)."""
+    kr, _ = _kernel(ktype, scal, cons, powe, clamp=clamp)
     return lib().orc_pair(ctypes.byref(kr), float(c), float(aa), float(bb), float(hi), float(hj))
 
 
 def dgsks_rows(ktype, XA, XA2, amap, XB, XB2, bmap, w, wmap, u, umap, rows=None, *,
                scal=0.0, cons=0.0, powe=0.0, hi=None, hj=None, nthreads=None,
-               apply_laplace=True):
-    """out[r] = u[umap[rows[r]]] + sum_j K(rows[r], j) w[wmap[j]] (rows=None: all)."""
+               apply_laplace=True, clamp=False):
+    """out[r] = u[umap[rows[r]]] + sum_j K(rows[r], j) w[wmap[j]] (rows=None: all).
+    clamp=True evaluates K with dgsks()'s d^2 >= 0 clamp instead of dgsks_ref's
+    unclamped form."""
     XA, XA2, XB, XB2, w, u = map(_f64, (XA, XA2, XB, XB2, w, u))
     amap, bmap, wmap, umap = map(_i32, (amap, bmap, wmap, umap))
     m, n = amap.size, bmap.size
@@ -182,7 +186,7 @@ def dgsks_rows(ktype, XA, XA2, amap, XB, XB2, bmap, w, wmap, u, umap, rows=None,
         powe, scal = laplace_params(k)
     rows = np.arange(m, dtype=np.int32) if rows is None else _i32(rows)
     out = np.empty(rows.size, dtype=np.float64)
-    kr, keep = _kernel(ktype, scal, cons, powe, hi, hj)
+    kr, keep = _kernel(ktype, scal, cons, powe, hi, hj, clamp)
     lib().orc_dgsks_rows(ctypes.byref(kr), m, n, k, _d(u), _i(umap), _d(XA), _d(XA2), _i(amap),
                          _d(XB), _d(XB2), _i(bmap), _d(w), _i(wmap), rows.size, _i(rows),
                          _d(out), _nthreads(nthreads))

@@ -82,6 +82,13 @@ def test_error_strings_and_modes(L):
     old = L.ks_get_error_mode()
     L.ks_set_error_mode(1)
     assert L.ks_get_error_mode() == 1
+    # a failed device call leaves its code; a later void entry point clears it
+    # at its start (here an early return: m == 0), so nothing stale is reported
+    assert L.sgsks_device(None, 1, 1, 1, *([None] * 10), 0, None) == 2
+    for fn in (L.dgsks, L.dgsks_ref, L.sgsks):
+        L.dgsks_device(None, 1, 1, 1, *([None] * 10), 0, None)
+        fn(None, 0, 1, 1, *([None] * 10))
+        assert L.ks_last_error() == 0
     L.ks_set_error_mode(old)
 
 
@@ -117,6 +124,17 @@ def test_no_gpu_fails_loudly(L):
     from ks_amd import gsks
     with pytest.raises(_lib.KsError):
         gsks.dgsks(3, u, idx, X, X2, idx, X, X2, idx, w, idx, type="Gaussian", scal=-0.5)
+    # single precision: the same, for the device entry point and the host one
+    Xf, uf, wf = X.astype(np.float32), u.astype(np.float32), w.astype(np.float32)
+    rc = L.sgsks_device(ctypes.byref(kt), 8, 8, 3, ptr(uf), ptr(idx), ptr(Xf), None, ptr(idx),
+                        ptr(Xf), None, ptr(idx), ptr(wf), ptr(idx), 0, None)
+    assert rc == _lib.KS_ERR_DEVICE
+    L.sgsks(ctypes.byref(kt), 8, 8, 3, ptr(uf), ptr(idx), ptr(Xf), None, ptr(idx), ptr(Xf), None,
+            ptr(idx), ptr(wf), ptr(idx))
+    assert L.ks_last_error() == _lib.KS_ERR_DEVICE
+    assert np.all(uf == 0)
+    with pytest.raises(_lib.KsError):
+        gsks.sgsks(3, uf, idx, Xf, None, idx, Xf, None, idx, wf, idx, type="Gaussian", scal=-0.5)
 
 
 def test_python_argument_checks(L):
@@ -132,6 +150,17 @@ def test_python_argument_checks(L):
         gsks.dgsks(3, u, idx, X, X2[:5], idx, X, X2, idx, w, idx, type="Gaussian")
     with pytest.raises(AssertionError):
         gsks.dgsks(3, u, idx, X, X2, idx, np.zeros((10, 4)), X2, idx, w, idx, type="Gaussian")
+    # sgsks: float32 arrays, norms optional, no explicit precision for host arrays
+    Xf, uf, wf = X.astype(np.float32), u.astype(np.float32), w.astype(np.float32)
+    with pytest.raises(AssertionError):
+        gsks.sgsks(3, u, idx, Xf, None, idx, Xf, None, idx, wf, idx, type="Gaussian")
+    with pytest.raises(AssertionError):
+        gsks.sgsks(3, uf, idx, Xf, X2, idx, Xf, None, idx, wf, idx, type="Gaussian")
+    with pytest.raises(AssertionError):
+        gsks.sgsks(3, uf, idx, Xf, None, idx, Xf, None, idx, wf, idx, type="Gaussian",
+                   precision="fp64")
+    with pytest.raises(AssertionError):
+        gsks.dgsks(3, u, idx, X, None, idx, X, X2, idx, w, idx, type="Gaussian")
 
 
 def test_unsupported_type_prints_and_returns(L, capsys):

@@ -1,7 +1,7 @@
 """GPU parity of the FP64 fused summation (dgsks through the C ABI) against the
 oracle and the reference's golden outputs.  Tolerance: relative L2 <= 1e-12
 over the updated rows (the reference's error metric, test/test_dgsks.c:43-79;
-the bar BASELINE.json sets for the FP64 drop-in)."""
+the accuracy bar of the FP64 drop-in)."""
 import ctypes
 
 import numpy as np
@@ -141,6 +141,44 @@ def test_distinct_tables_and_bandwidths(G, oracle, torch_cuda):
     torch.cuda.synchronize()
     ref = oracle.dgsks_rows(3, A, A2, amap, B, B2, bmap, w, wmap, u0, umap, hi=hi, hj=hj)
     assert oracle.rel_l2(du.cpu().numpy()[umap], ref) <= TOL
+
+
+@pytest.mark.parametrize("k", [3, 16, 64])
+def test_clamp_semantics_with_inconsistent_norms(G, oracle, torch_cuda, k):
+    """Norms lowered by 1e-7 (relative) and every target also a source: d^2 of
+    each coincident pair is negative.  dgsks clamps it at 0 like the reference's
+    fused micro-kernels (the oracle's clamp mode), dgsks_ref does not (its
+    default mode); both to 1e-12, on the general, symmetric (k = 3, one map
+    tensor) and matrix-core paths, and the mixed form to 1e-6."""
+    torch = torch_cuda
+    rng = np.random.default_rng(500 + k)
+    nx, m = 3000, 2500
+    X = rng.random((nx, k)) * (1.2 / np.sqrt(k))
+    X2 = oracle.sqnorm(X) * (1.0 - 1e-7)
+    amap = rng.permutation(nx)[:m].astype(np.int32)
+    umap = rng.permutation(nx)[:m].astype(np.int32)
+    wmap = rng.permutation(nx)[:m].astype(np.int32)
+    w = rng.standard_normal(nx)
+    u0 = rng.standard_normal(nx)
+    hv = rng.uniform(1.0, 3.0, nx)
+    dX, dX2, damap, dumap, dwmap, dw, dh = _dev(torch, X, X2, amap, umap, wmap, w, hv)
+    for t in (0, 3, 5, 6, 7):
+        prm = PARAMS[t]
+        h = hv if t == 3 else None
+        ref = {c: oracle.dgsks_rows(t, X, X2, amap, X, X2, amap, w, wmap, u0, umap, hi=h, hj=h,
+                                    clamp=c, **prm) for c in (True, False)}
+        assert oracle.rel_l2(ref[False], ref[True]) > 1e-10  # the case discriminates
+        runs = [("dgsks", damap, "fp64", True, TOL), ("dgsks", damap.clone(), "fp64", True, TOL),
+                ("dgsks_ref", damap, "fp64", False, TOL)]
+        if t in (0, 3):
+            runs.append(("dgsks", damap, "mixed_bf16x3", True, 1e-6))
+        for fn, dbmap, prec, clamp, tol in runs:
+            du = _dev(torch, u0)[0]
+            getattr(G, fn)(k, du, dumap, dX, dX2, damap, dX, dX2, dbmap, dw, dwmap,
+                           precision=prec, **_opts(t, h=dh if t == 3 else None, **prm))
+            torch.cuda.synchronize()
+            err = oracle.rel_l2(du.cpu().numpy()[umap], ref[clamp])
+            assert err <= tol, (t, k, fn, prec, err)
 
 
 # ------------------------------------------------------------------ benchmark shapes

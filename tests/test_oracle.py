@@ -99,6 +99,55 @@ def test_oracle_bitexact_vs_compiled_reference(oracle, shape):
         assert np.array_equal(a, b), (t, shape)
 
 
+@pytest.mark.parametrize("t", [0, 3, 5, 6, 7])
+def test_clamp_mode_vs_compiled_reference_dgsks(oracle, t):
+    """Norms lowered by 1e-7 (relative) make d^2 of every coincident pair
+    negative.  The fused dgsks clamps d^2 at 0 before these kernels
+    (micro_kernel/x86_64/sandybridge/ks_gaussian_int_d8x4.c:169 and its
+    siblings), dgsks_ref does not: the oracle's clamp mode must follow the
+    compiled dgsks, its default mode must visibly differ from it."""
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(40 + t)
+    m, k = 40, 8
+    X, amap, umap, _, wmap, w, u0, h = _problem(rng, m, m, k)
+    bmap = amap.copy()  # every target meets itself
+    X2 = oracle.sqnorm(X) * (1.0 - 1e-7)
+    prm = _params(t, k)
+    hh = h if t == oracle.VAR_BANDWIDTH else None
+    ref = u0.copy()
+    oracle.ref_call("dgsks", t, ref, umap, X, X2, amap, X, X2, bmap, w, wmap, hi=hh, hj=hh, **prm)
+    rows = {c: oracle.dgsks_rows(t, X, X2, amap, X, X2, bmap, w, wmap, u0, umap, hi=hh, hj=hh,
+                                 clamp=c, **prm) for c in (True, False)}
+    assert oracle.rel_l2(rows[True], ref[umap]) <= 1e-13
+    assert oracle.rel_l2(rows[False], ref[umap]) > 1e-9
+
+
+def test_clamp_mode_closed_forms(oracle):
+    """K at a negative d^2 under the clamp is K at d^2 = 0, for every type it
+    applies to; without it the unclamped formula is evaluated; Laplace is the
+    same either way (its d^2 < 1e-15 branch)."""
+    aa = bb = 0.5
+    c = 0.5 + 1e-6  # d^2 = aa + bb - 2c = -2e-6
+    P = oracle.pair
+    assert P(0, c, aa, bb, scal=-3.0, clamp=True) == 1.0
+    assert P(0, c, aa, bb, scal=-3.0) == math.exp(-3.0 * (aa + bb - 2 * c))
+    assert P(3, c, aa, bb, 2.0, 1.5, clamp=True) == 1.0
+    assert P(5, c, aa, bb, clamp=True) == 15.0 / 16.0
+    assert P(5, c, aa, bb) > 15.0 / 16.0
+    assert P(6, c, aa, bb, cons=0.25, clamp=True) == 0.25
+    assert P(6, c, aa, bb, cons=0.25) < 0.25
+    assert P(7, c, aa, bb, clamp=True) == 0.75
+    assert P(7, c, aa, bb) > 0.75
+    powe, scal = oracle.laplace_params(3)
+    assert P(2, c, aa, bb, powe=powe, scal=scal, clamp=True) == P(2, c, aa, bb, powe=powe,
+                                                                 scal=scal)
+    # non-negative d^2: the clamp changes nothing
+    for t in (0, 5, 6, 7):
+        assert P(t, 0.1, aa, bb, scal=-3.0, cons=0.25, clamp=True) == \
+            P(t, 0.1, aa, bb, scal=-3.0, cons=0.25)
+
+
 # ------------------------------------------------------------------ independent restatements
 
 @pytest.mark.parametrize("t", TYPES)
