@@ -1,18 +1,26 @@
 // Mixed-precision fused summation (Gaussian and variable-bandwidth Gaussian):
 //   u[umap[i]] += sum_j exp(x_ij) w[wmap[j]],  x = scal*d^2  or  -0.5 d^2 hi hj.
 //
-// Every pair is evaluated; what changes against the FP64 drop-in is where:
+// Every pair that can matter is evaluated; what changes against the FP64
+// drop-in is where:
 //   1. an estimate of every cross term a.b runs on the BF16 matrix cores as a
 //      split-BF16 three-product (a ~= ah + al, b ~= bh + bl, bf16 each;
 //      a.b ~= ah.bh + ah.bl + al.bh accumulated in FP32 by
 //      v_mfma_f32_16x16x32_bf16, 3 instructions per 32 coordinates of a 16 x 16
 //      pair block: KS_PREC_MIXED_BF16X3), or on the FP32 matrix cores
 //      (v_mfma_f32_16x16x4f32, 8 per 32 coordinates: KS_PREC_MIXED_FP32);
-//   2. a 16 x 16 block whose estimated exponents all lie below -kFarT
+//   2. BF16x3 Gaussian only: the first product ah.bh (one instruction per 32
+//      coordinates) bounds the block before the other two run.  With a
+//      rigorous margin for what the split and FP32 accumulation can change
+//      (2^-7 |a||b| from the BF16 roundings of both operands plus the FP32
+//      sums, using the largest |a| of the wave's rows and each source's |b|),
+//      a block whose every exponent is below kSkipT2 (K < 2^-60 for every
+//      pair) is skipped: a target loses at most n 2^-60 max|w| in total;
+//   3. a 16 x 16 block whose estimated exponents all lie below -kFarT
 //      (K < ~1e-7 for every pair) is "far": its kernel values are taken in FP32
 //      (v_exp_f32) from the estimate, whose error is bounded by the estimate's
 //      error times K, far below 1e-6 of u;
-//   3. any other block is "near": its cross terms are recomputed exactly on the
+//   4. any other block is "near": its cross terms are recomputed exactly on the
 //      FP64 matrix cores (v_mfma_f64_16x16x4f64 from FP64 coordinates) and its
 //      kernel values evaluated in FP64 with FP64 norms, so the |a|^2 + |b|^2 -
 //      2 a.b cancellation of close pairs never meets an FP32 rounding.
@@ -51,13 +59,22 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr float kFarT = 16.0f;          // far: estimated x < -16 (natural-log units)
+constexpr float kSkipT2 = -60.0f;       // skipped: every exponent < -60 (log2 units)
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr int XWG = 256;                // 4 waves
 constexpr int kPivots = 128;            // bucketing pivots (at most)
 constexpr int kSortMin = 16384;         // bucket only when m and n are both at least this
+constexpr int kXcds = 8;                // L2 domains; workgroups go to them round robin
+constexpr long kSliceHiBytes = 2l << 20;  // BF16 hi halves of one source slice: 2 MiB
                                         // (its sorts cost ~15 launches per call)
 constexpr int kMixRB = 2;               // 16-row blocks per wave
 constexpr int kMixRowsPerWG = 4 * kMixRB * 16;
+
+// The skip tier runs for the BF16x3 Gaussian only: with a bandwidth per point,
+// one small hi hj in a 16 x 16 block keeps it from being skipped, and the
+// two-stage test then only adds work.
+template <int PREC, int TYPE>
+constexpr bool kSkipTier = PREC == KS_PREC_MIXED_BF16X3 && TYPE == GAUSSIAN;
 
 // ------------------------------------------------------------------ pre-pass
 
@@ -66,7 +83,7 @@ struct MixPack {
   uint4 *Bp;        // BF16 / FP32 source operand [np/16][nc][2 parts][64 lanes]
   double4 *Bd;      // FP64 source operand [np/16][2 nc][64 lanes]
   float2 *ra;       // per target: {-|a|^2 / 2, -hi log2(e) / 2 (VarBW)}
-  float4 *cb;       // per source: {xs |b|^2 (Gaussian) or |b|^2, hj, w, 0}
+  float4 *cb;       // per source: {xs |b|^2 (Gaussian) or |b|^2, hj, w, |b| (Gaussian)}
   double2 *rd;      // per target: {|a|^2, -hi / 2}
   double4 *cd;      // per source: {|b|^2, w, hj, 0}
   const int *prow;  // sorted target position -> target index of the call
@@ -207,8 +224,9 @@ __global__ void mixed_pack_kernel(const Problem p, int nc, float xs, long mp, lo
       const double bb = p.XB2[bj];
       const double w = p.w[p.wmap[jj]];
       const double hj = (TYPE == VAR_BANDWIDTH) ? p.hj[bj] : 0.0;
-      if (TYPE == GAUSSIAN) pk.cb[j] = make_float4((float)(bb * (double)xs), 0.0f, (float)w, 0.0f);
-      else pk.cb[j] = make_float4((float)bb, (float)hj, (float)w, 0.0f);
+      const float nb = (float)sqrt(bb);
+      if (TYPE == GAUSSIAN) pk.cb[j] = make_float4((float)(bb * (double)xs), 0.0f, (float)w, nb);
+      else pk.cb[j] = make_float4((float)bb, (float)hj, (float)w, nb);
       pk.cd[j] = make_double4(bb, w, hj, 0.0);
     } else {  // padding source: exponent -inf, weight 0
       pack_point<PREC>(nullptr, p.k, nc, j, pk.Bp, pk.Bd);
@@ -289,7 +307,8 @@ __device__ __forceinline__ double row_sum16(double x) {
 constexpr int kFlush = 8;  // far-pair FP32 partials go to FP64 every 8 source blocks
 
 template <int PREC, int TYPE, int NC>
-__global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, float xs) {
+__global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A, MixPack pk,
+                                                                  float xs) {
   constexpr int RB = kMixRB;
   constexpr int KC = 2 * NC;  // 16-coordinate chunks of the FP64 operand
   const Problem &p = A.p;
@@ -298,8 +317,14 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
   const int wave = threadIdx.x >> 6;
   const int p16 = lane & 15;
   const int g = lane >> 4;
-  const int rblk = blockIdx.x;
-  const int split = blockIdx.y;
+  // XCD-aware order: consecutive workgroups go to the XCDs in turn, so XCD x
+  // runs workgroups x, x + 8, ...; give it the source slices x, x + 8, ... and
+  // let it sweep every row block of one slice before the next, so that the
+  // workgroups sharing its L2 read the same slice (nsplit is a multiple of 8).
+  const int row_blocks = (m + 4 * RB * 16 - 1) / (4 * RB * 16);
+  const int xcd = (int)(blockIdx.x % kXcds), loc = (int)(blockIdx.x / kXcds);
+  const int rblk = loc % row_blocks;
+  const int split = xcd + kXcds * (loc / row_blocks);
   constexpr int WROWS = RB * 16;
   const int wg_rows = 4 * WROWS;
   const int row0 = rblk * wg_rows + wave * WROWS;  // sorted target position
@@ -310,6 +335,8 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
   constexpr int KS = 32 * NC + 2;
   __shared__ double s_rows[4][WROWS * KS];
   __shared__ double s_tot[4][WROWS];
+  // FP64 row factors {|a|^2, -hi/2}, read by near blocks only
+  __shared__ double2 s_rd[4][WROWS];
   double *arows = s_rows[wave];
   for (int e = lane; e < WROWS * 32 * NC; e += 64) {
     const int rl = e / (32 * NC), q = e - rl * (32 * NC);
@@ -323,7 +350,8 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
   Frag<PREC, NC> af[RB];
   f4 cinit[RB];
   float ra_h[RB][4], facc[RB][4];
-  double racc[RB][4], aa[RB][4], hi[RB][4];
+  double racc[RB][4];
+  float mA = 0.0f;  // largest |a| of the lane's rows
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) {
     load_frag_packed<PREC, NC>(pk.Ap + (long)((row0 >> 4) + rb) * NC * 128, lane, af[rb]);
@@ -334,92 +362,127 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
       const double2 d = pk.rd[t];
       cinit[rb][v] = f.x;
       ra_h[rb][v] = f.y;
-      aa[rb][v] = d.x;
-      hi[rb][v] = d.y;
+      if (p16 == 0) s_rd[wave][rb * 16 + out_row(g, v)] = d;
+      mA = fmaxf(mA, (float)sqrt(d.x));
       racc[rb][v] = 0.0;
       facc[rb][v] = 0.0f;
     }
   }
   __syncthreads();  // the staged rows are visible to every lane of the wave
   const float m2xs = -2.0f * xs;
+
+  // Skip-test margin (Gaussian BF16x3).  After the first product,
+  // acc = -|a|^2/2 + ah.bh in FP32.  With |ah - a| <= 2^-8 |a| per coordinate
+  // (and the same for b), |ah.bh - a.b| <= (2^-7 + 2^-16) |a||b|; the FP32 sums
+  // of kp + 1 terms add at most (kp + 3) 2^-24 (|a|^2/2 + |a||b|).  With A the
+  // largest |a| of the wave's rows and B = |b| of the lane's source, the
+  // exponent x2 = -2 xs acc + xs |b|^2 is off by at most
+  //   |2 xs| (2^-7 1.01 A B + (kp + 3) 2^-23 (A^2/2 + A B)) + 2^-23 |xs| B^2
+  // (doubled rounding terms cover the FP32 norms and the final FMA); the test
+  // also shrinks the exponent by 2^-18 of itself for the FP32 roundings that
+  // scale with it (xs, the FMA).
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) mA = fmaxf(mA, __shfl_xor(mA, o));
+  mA *= 1.0001f;  // the FP32 rounding of sqrt
+  constexpr float kAcc = (32.0f * NC + 3.0f) * 0x1p-23f;
+  const float mk1 = fabsf(2.0f * xs) * (0x1p-7f * 1.01f * mA + kAcc * mA);  // times B
+  const float mk0 = fabsf(2.0f * xs) * (kAcc * mA * mA * 0.5f);           // constant
+  const float mk2 = fabsf(xs) * 0x1p-23f;                                  // times B^2
+  constexpr float kShrink = 1.0f - 0x1p-18f;
   const float thr = -kFarT * kLog2e;
   const KerParams P = p.P;
   const int j0 = split * A.cols_per_split;
   const int j1 = min(p.n, j0 + A.cols_per_split);
 
-  // FP64 source fragment of the block being (or about to be) recomputed exactly
-  double bd[KC][4];
-  double4 cdv = make_double4(0.0, 0.0, 0.0, 0.0);
-  int bd_block = -1;  // source block whose FP64 fragment bd holds (or has in flight)
-  auto fetch64 = [&](int jb) {
+  // FP64 source fragment of a block being (or about to be) recomputed exactly
+  struct F64Frag {
+    double bd[KC][4];
+    double4 cdv;
+    int block;  // the source block it holds (or has in flight), -1 for none
+  };
+  auto fetch64 = [&](F64Frag &f, int jb) {
     const double4 *src = pk.Bd + (long)(jb >> 4) * KC * 64 + lane;
 #pragma unroll
     for (int c = 0; c < KC; ++c) {
       const double4 q = src[c * 64];
-      bd[c][0] = q.x;
-      bd[c][1] = q.y;
-      bd[c][2] = q.z;
-      bd[c][3] = q.w;
+      f.bd[c][0] = q.x;
+      f.bd[c][1] = q.y;
+      f.bd[c][2] = q.z;
+      f.bd[c][3] = q.w;
     }
-    cdv = pk.cd[jb + p16];
-    bd_block = jb;
+    f.cdv = pk.cd[jb + p16];
+    f.block = jb;
   };
 
   // source blocks start on multiples of 16 and a slice ends on one or at n, so
-  // a block's columns past j1 are padding of the packed arrays
+  // a block's columns past j1 are padding of the packed arrays.  The skip tier
+  // loads the hi halves ahead of time and the lo halves (fetch_lo) only for
+  // blocks that pass its test.
   auto fetch = [&](int jb, Frag<PREC, NC> &f, float4 &c4) {
-    load_frag_packed<PREC, NC>(pk.Bp + (long)(jb >> 4) * NC * 128, lane, f);
+    const uint4 *blk = pk.Bp + (long)(jb >> 4) * NC * 128 + lane;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      f.v[c][0] = blk[(c * 2) * 64];
+      f.v[c][1] = blk[(c * 2 + 1) * 64];
+    }
     c4 = pk.cb[jb + p16];
+  };
+  uint4 blo[NC];     // lo halves of the source block lo_block (BF16x3)
+  int lo_block = -1;
+  auto fetch_lo = [&](int jb) {
+    const uint4 *blk = pk.Bp + (long)(jb >> 4) * NC * 128 + lane;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) blo[c] = blk[(c * 2 + 1) * 64];
+    lo_block = jb;
   };
 
   // One 16-source block against the wave's RB row blocks.
   int nblk = 0;
-  auto block = [&](int jb, int jnext, const Frag<PREC, NC> &bf, const float4 &cb) {
-    bool used64 = false;
+  // Far / near evaluation of row block rb against source block jb, given its
+  // full FP32 cross-term estimate c (= a.b - |a|^2/2).
+  auto evaluate = [&](int rb, int jb, const f4 &c, const float4 &cb, F64Frag &f64, bool &used64) {
+    float x2[4];
+    bool nr = false;
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
-      const f4 c = cross_block<PREC, NC>(af[rb], bf, cinit[rb]);
-      float x2[4];
-      bool nr = false;
+    for (int v = 0; v < 4; ++v) {
+      if constexpr (TYPE == GAUSSIAN) {
+        x2[v] = fmaf(m2xs, c[v], cb.x);
+      } else {
+        x2[v] = fmaxf(fmaf(-2.0f, c[v], cb.x), 0.0f) * (ra_h[rb][v] * cb.y);
+      }
+      nr = nr || (x2[v] > thr);
+    }
+    if (__ballot(nr) == 0ull) {
+      // far block: FP32 kernel values from the estimate
+#pragma unroll
+      for (int v = 0; v < 4; ++v) facc[rb][v] = fmaf(__builtin_amdgcn_exp2f(x2[v]), cb.z, facc[rb][v]);
+    } else {
+      // near block: exact FP64 cross terms on the matrix cores, FP64 kernel values
+      if (f64.block != jb) fetch64(f64, jb);
+      used64 = true;
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+      const double *ar = arows + (rb * 16 + f64_a_row(p16)) * KS + 4 * g;
+#pragma unroll
+      for (int cc = 0; cc < KC; ++cc) {
+        const double2 a01 = *reinterpret_cast<const double2 *>(ar + cc * 16);
+        const double2 a23 = *reinterpret_cast<const double2 *>(ar + cc * 16 + 2);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.x, f64.bd[cc][0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.y, f64.bd[cc][1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.x, f64.bd[cc][2], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.y, f64.bd[cc][3], acc, 0, 0, 0);
+      }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        if constexpr (TYPE == GAUSSIAN) {
-          x2[v] = fmaf(m2xs, c[v], cb.x);
-        } else {
-          x2[v] = fmaxf(fmaf(-2.0f, c[v], cb.x), 0.0f) * (ra_h[rb][v] * cb.y);
-        }
-        nr = nr || (x2[v] > thr);
-      }
-      if (__ballot(nr) == 0ull) {
-        // far block: FP32 kernel values from the estimate
-#pragma unroll
-        for (int v = 0; v < 4; ++v) facc[rb][v] = fmaf(__builtin_amdgcn_exp2f(x2[v]), cb.z, facc[rb][v]);
-      } else {
-        // near block: exact FP64 cross terms on the matrix cores, FP64 kernel values
-        if (bd_block != jb) fetch64(jb);
-        used64 = true;
-        d4 acc = {0.0, 0.0, 0.0, 0.0};
-        const double *ar = arows + (rb * 16 + f64_a_row(p16)) * KS + 4 * g;
-#pragma unroll
-        for (int cc = 0; cc < KC; ++cc) {
-          const double2 a01 = *reinterpret_cast<const double2 *>(ar + cc * 16);
-          const double2 a23 = *reinterpret_cast<const double2 *>(ar + cc * 16 + 2);
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.x, bd[cc][0], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.y, bd[cc][1], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.x, bd[cc][2], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.y, bd[cc][3], acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const double K = kval_f64<TYPE>(acc[v], aa[rb][v], cdv.x, hi[rb][v], cdv.z, P);
-          racc[rb][v] = fma(K, cdv.y, racc[rb][v]);
-        }
+        const double2 rd = s_rd[wave][rb * 16 + out_row(g, v)];
+        const double K = kval_f64<TYPE>(acc[v], rd.x, f64.cdv.x, rd.y, f64.cdv.z, P);
+        racc[rb][v] = fma(K, f64.cdv.y, racc[rb][v]);
       }
     }
-    // near blocks come in runs once the points are bucketed: after one, start
-    // the next block's FP64 fragment loads
-    if (used64 && jnext != jb) fetch64(jnext);
-    if (++nblk == kFlush) {
+  };
+  // FP32 far partials go to the FP64 row sums every kFlush source blocks
+  auto tick = [&](int nb) {
+    nblk += nb;
+    if (nblk >= kFlush) {
       nblk = 0;
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb)
@@ -430,23 +493,148 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
         }
     }
   };
+  const int jlast = j0 + ((j1 - j0 - 1) >> 4) * 16;  // start of the slice's last block
 
-  // two fragment buffers in turn: the next block's loads are in flight while
-  // this block computes, with no register copies between them (the last
-  // block's "next" is itself again, an in-bounds re-read)
-  Frag<PREC, NC> b0, b1;
-  float4 c0, c1;
-  const int jlast = j0 + ((j1 - j0 - 1) >> 4) * 16;
-  if (j0 < j1) {
-    fetch(j0, b0, c0);
-    for (int jb = j0;; jb += 32) {
-      const int jn1 = min(jb + 16, jlast), jn2 = min(jb + 32, jlast);
-      fetch(jn1, b1, c1);
-      block(jb, jn1, b0, c0);
-      if (jb + 16 >= j1) break;
-      fetch(jn2, b0, c0);
-      block(jb + 16, jn2, b1, c1);
-      if (jb + 32 >= j1) break;
+  if constexpr (kSkipTier<PREC, TYPE>) {
+    // Two source blocks x RB row blocks per step: their 2 RB first products
+    // are independent and issue back to back, their skip tests are four
+    // instructions each, and one ballot lets the usual all-skipped step
+    // through.  The hi halves of the next two blocks load meanwhile.
+    struct HB {
+      uint4 h[NC];
+      float4 cb;
+    };
+    auto fetch_h = [&](int jb, HB &b) {
+      const uint4 *blk = pk.Bp + (long)(jb >> 4) * NC * 128 + lane;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) b.h[c] = blk[(c * 2) * 64];
+      b.cb = pk.cb[jb + p16];
+    };
+    auto est = [&](int rb, const HB &b) {
+      f4 e = cinit[rb];
+#pragma unroll
+      for (int cc = 0; cc < NC; ++cc)
+        e = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf8 *>(&af[rb].v[cc][0]),
+                                                    *reinterpret_cast<const bf8 *>(&b.h[cc]), e, 0,
+                                                    0, 0);
+      return e;
+    };
+    // For a decaying kernel (xs < 0) the largest exponent of a lane's four
+    // pairs belongs to its largest accumulator: x2 = -2 xs acc + xs |b|^2.
+    const bool skip_on = xs < 0.0f;
+    const float m2s = m2xs * kShrink;
+    auto tval = [&](const HB &b) {
+      return fmaf(b.cb.x, kShrink, fmaf(fmaf(mk2, b.cb.w, mk1), b.cb.w, mk0));
+    };
+    auto hot = [&](const f4 &e, float t) {
+      const float mx = fmaxf(fmaxf(e[0], e[1]), fmaxf(e[2], e[3]));
+      return !skip_on || fmaf(m2s, mx, t) > kSkipT2;
+    };
+    // the row blocks of one source block that passed: the two correction
+    // products complete the split-BF16 cross term, then far / near
+    auto finish = [&](int jb, int jnext, const HB &b, const f4 *e, const bool *h) {
+      bool used64 = false, usedlo = false;
+      F64Frag f64;  // live within this block only: loaded by its first near row block
+      f64.block = -1;
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        if (!h[rb]) continue;  // wave-uniform
+        if (lo_block != jb) fetch_lo(jb);
+        usedlo = true;
+        f4 c = e[rb];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) {
+          const bf8 ah = *reinterpret_cast<const bf8 *>(&af[rb].v[cc][0]);
+          const bf8 al = *reinterpret_cast<const bf8 *>(&af[rb].v[cc][1]);
+          const bf8 bh = *reinterpret_cast<const bf8 *>(&b.h[cc]);
+          const bf8 bl = *reinterpret_cast<const bf8 *>(&blo[cc]);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+        }
+        evaluate(rb, jb, c, b.cb, f64, used64);
+      }
+      // blocks that pass come in runs once the points are bucketed: after one,
+      // start the next block's lo loads
+      if (usedlo && jnext != jb) fetch_lo(jnext);
+    };
+    auto step = [&](int jb, const HB &b0, const HB &b1) {
+      const bool two = jb + 16 < j1;  // the second block lies in the slice
+      f4 e0[RB], e1[RB];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) e0[rb] = est(rb, b0);
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) e1[rb] = est(rb, b1);
+      const float t0 = tval(b0), t1 = tval(b1);
+      bool h0[RB], h1[RB], any = false;
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        h0[rb] = hot(e0[rb], t0);
+        h1[rb] = two && hot(e1[rb], t1);
+        any = any || h0[rb] || h1[rb];
+      }
+      if (__ballot(any) != 0ull) {
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+          h0[rb] = __ballot(h0[rb]) != 0ull;
+          h1[rb] = __ballot(h1[rb]) != 0ull;
+        }
+        finish(jb, min(jb + 16, jlast), b0, e0, h0);
+        if (two) finish(jb + 16, min(jb + 32, jlast), b1, e1, h1);
+      }
+      tick(two ? 2 : 1);
+    };
+    // three pairs of buffers in turn, two pairs ahead (blocks past the slice
+    // re-read its last block, in bounds, and are not evaluated)
+    HB pa0, pa1, pb0, pb1, pc0, pc1;
+    auto fetch_pair = [&](int jb, HB &x0, HB &x1) {
+      fetch_h(min(jb, jlast), x0);
+      fetch_h(min(jb + 16, jlast), x1);
+    };
+    if (j0 < j1) {
+      fetch_pair(j0, pa0, pa1);
+      fetch_pair(j0 + 32, pb0, pb1);
+      for (int jb = j0;; jb += 96) {
+        fetch_pair(jb + 64, pc0, pc1);
+        step(jb, pa0, pa1);
+        if (jb + 32 >= j1) break;
+        fetch_pair(jb + 96, pa0, pa1);
+        step(jb + 32, pb0, pb1);
+        if (jb + 64 >= j1) break;
+        fetch_pair(jb + 128, pb0, pb1);
+        step(jb + 64, pc0, pc1);
+        if (jb + 96 >= j1) break;
+      }
+    }
+  } else {
+    F64Frag f64p;  // persists across blocks: prefetched after a near block
+    f64p.block = -1;
+    // One source block against the wave's RB row blocks.
+    auto block = [&](int jb, int jnext, const Frag<PREC, NC> &bf, const float4 &cb) {
+      bool used64 = false;
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+        evaluate(rb, jb, cross_block<PREC, NC>(af[rb], bf, cinit[rb]), cb, f64p, used64);
+      // near blocks come in runs once the points are bucketed: after one,
+      // start the next block's FP64 fragment loads
+      if (used64 && jnext != jb) fetch64(f64p, jnext);
+      tick(1);
+    };
+    // two fragment buffers in turn: the next block's loads are in flight while
+    // this block computes, with no register copies between them (the last
+    // block's "next" is itself again, an in-bounds re-read)
+    Frag<PREC, NC> b0, b1;
+    float4 c0, c1;
+    if (j0 < j1) {
+      fetch(j0, b0, c0);
+      for (int jb = j0;; jb += 32) {
+        const int jn1 = min(jb + 16, jlast), jn2 = min(jb + 32, jlast);
+        fetch(jn1, b1, c1);
+        block(jb, jn1, b0, c0);
+        if (jb + 16 >= j1) break;
+        fetch(jn2, b0, c0);
+        block(jb + 16, jn2, b1, c1);
+        if (jb + 32 >= j1) break;
+      }
     }
   }
 
@@ -493,7 +681,7 @@ __global__ __launch_bounds__(XWG) void mixed_kernel(FusedArgs A, MixPack pk, flo
 template <int PREC, int TYPE, int NC>
 static hipError_t mixed_go(const FusedArgs &A, const MixPack &pk, float xs, int row_blocks,
                            hipStream_t s) {
-  dim3 grid(row_blocks, A.nsplit);
+  dim3 grid((unsigned)row_blocks * (unsigned)A.nsplit);
   hipLaunchKernelGGL((mixed_kernel<PREC, TYPE, NC>), grid, dim3(XWG), 0, s, A, pk, xs);
   return hipGetLastError();
 }
@@ -552,9 +740,19 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
     return raise_error(KS_ERR_INVALID_ARG, false,
                        "dgsks_device(): mixed precision supports k <= 128\n");
   FusedPlan plan = plan_split(p.m, p.n, kMixRowsPerWG, 2048, 128);
-  // source slices start on 16-point blocks of the packed layout
-  plan.cols_per_split = (plan.cols_per_split + 15) / 16 * 16;
-  plan.nsplit = (p.n + plan.cols_per_split - 1) / plan.cols_per_split;
+  // Source slices: a multiple of the 8 XCDs (the kernel gives XCD x the slices
+  // x, x + 8, ...), each with at most kSliceHiBytes of packed hi halves so that
+  // one slice stays in an XCD's L2, and at least as many as plan_split wants
+  // for parallelism.  Slices start on 16-point blocks of the packed layout;
+  // trailing ones may be empty.
+  {
+    const long max_src = kSliceHiBytes / (64l * nc);
+    long ns = kXcds * ((p.n + kXcds * max_src - 1) / (kXcds * max_src));
+    const long want = (plan.nsplit + kXcds - 1) / kXcds * kXcds;
+    if (ns < want) ns = want;
+    plan.nsplit = (int)ns;
+    plan.cols_per_split = (int)(((p.n + ns - 1) / ns + 15) / 16 * 16);
+  }
   const bool bucket = p.m >= kSortMin && p.n >= kSortMin;
   const int P = bucket ? kPivots : 0;
   const int kp = 32 * nc;
