@@ -133,22 +133,26 @@ __device__ double tile_valu(const ListArgs &L, const DevTask &T, int pos, int c0
   return acc;
 }
 
-// Sources staged per wave in LDS: kStageDoubles doubles (a chunk of
-// kStageDoubles / K source rows of K padded coordinates), then 3 doubles per
-// source of the chunk (|b|^2, w, hj).
+// Sources staged per wave in LDS: a chunk of stage_rows(K) source rows of K
+// padded coordinates (at most kStageDoubles doubles), then 3 doubles per source
+// of the chunk (|b|^2, w, hj).
 constexpr int kStageDoubles = 1024;
 constexpr int kStageSide = 3 * 64;
+constexpr int stage_rows(int K) { return kStageDoubles / K < 64 ? kStageDoubles / K : 64; }
+constexpr int stage_doubles(int K) { return (stage_rows(K) * K + 63) / 64 * 64; }
 
-// lane = row, k <= K (K in {16, 32, 64}, coordinates past k are zero): the
-// row stays in VGPRs; the sources go through the wave's LDS slot in chunks of
-// kStageDoubles / K rows -- loaded with coalesced vector loads (consecutive
-// lanes read consecutive doubles of a row) and read back as wave-wide
-// broadcasts -- so every source row is fetched once per tile.  Zero padding
-// keeps the Laplace rank-k sum exact: the extra terms add +0.
+// lane = row, k <= K (K in {3, 8, 16, 32, 64}, coordinates past k are zero):
+// the row stays in VGPRs; the sources go through the wave's LDS slot in chunks
+// of stage_rows(K) rows -- loaded with vector loads that are all in flight at
+// once (consecutive lanes read consecutive doubles of a row) and read back as
+// wave-wide broadcasts -- so every source row is fetched once per tile and no
+// load waits on the one before it.  Zero padding keeps the Laplace rank-k sum
+// exact: the extra terms add +0.
 template <int TYPE, int K>
 __device__ double tile_valu_lds(const ListArgs &L, const DevTask &T, int pos, int c0, int c1,
                                 double *stage) {
-  constexpr int CH = kStageDoubles / K;
+  constexpr int CH = stage_rows(K);
+  constexpr int NV = stage_doubles(K) / 64;  // staged doubles per lane
   const int k = L.k;
   const int lane = threadIdx.x & 63;
   const int ai = L.amap[T.a0 + (pos < T.m ? pos : T.m - 1)];
@@ -159,13 +163,13 @@ __device__ double tile_valu_lds(const ListArgs &L, const DevTask &T, int pos, in
   const double aa = L.XA2[ai];
   const double hi = (TYPE == VAR_BANDWIDTH) ? -0.5 * T.hi[ai] : 0.0;
   const KerParams P = T.P;
-  double *side = stage + kStageDoubles;  // [CH][3]: |b|^2, w, hj
+  double *side = stage + stage_doubles(K);  // [CH][3]: |b|^2, w, hj
   double acc = 0.0;
   for (int j0 = c0; j0 < c1; j0 += CH) {
     const int nch = min(CH, c1 - j0);
-    double v[kStageDoubles / 64];
+    double v[NV];
 #pragma unroll
-    for (int i = 0; i < kStageDoubles / 64; ++i) {
+    for (int i = 0; i < NV; ++i) {
       const int e = lane + 64 * i, r = e / K, p = e % K;
       const int rr = r < nch ? r : nch - 1;
       const double x = L.XB[(long)L.bmap[T.b0 + j0 + rr] * k + (p < k ? p : k - 1)];
@@ -181,7 +185,7 @@ __device__ double tile_valu_lds(const ListArgs &L, const DevTask &T, int pos, in
     }
     __builtin_amdgcn_wave_barrier();  // the previous chunk's broadcasts are done
 #pragma unroll
-    for (int i = 0; i < kStageDoubles / 64; ++i) stage[lane + 64 * i] = v[i];
+    for (int i = 0; i < NV; ++i) stage[lane + 64 * i] = v[i];
     if (lane < CH) {
       side[3 * lane] = sb;
       side[3 * lane + 1] = sw;
@@ -336,7 +340,7 @@ __device__ double run_tile(const ListArgs &L, const DevTask &T, int row0, int c0
       if (L.k <= 48) return tile_mfma<TYPE, 3>(L, T, row0, c0, c1, red);
       return tile_mfma<TYPE, 4>(L, T, row0, c0, c1, red);
     }
-  } else if constexpr (K >= 16) {
+  } else if constexpr (K > 0) {
     return tile_valu_lds<TYPE, K>(L, T, row0 + (threadIdx.x & 63), c0, c1, stage);
   } else {
     return tile_valu<TYPE, K>(L, T, row0 + (threadIdx.x & 63), c0, c1);
@@ -352,7 +356,7 @@ __global__ __launch_bounds__(256) void list_kernel(ListArgs L) {
   const int wave = threadIdx.x >> 6;
   __shared__ double s_red[4][64];
   double *red = s_red[wave];
-  constexpr int NST = (K >= 16 && MF == 0) ? kStageDoubles + kStageSide : 1;
+  constexpr int NST = (K > 0 && MF == 0) ? stage_doubles(K > 0 ? K : 1) + kStageSide : 1;
   __shared__ double s_stage[4][NST];
   double *stage = s_stage[wave];
   while (true) {

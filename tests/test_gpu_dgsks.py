@@ -143,6 +143,42 @@ def test_distinct_tables_and_bandwidths(G, oracle, torch_cuda):
     assert oracle.rel_l2(du.cpu().numpy()[umap], ref) <= TOL
 
 
+@pytest.mark.parametrize("k", [3, 16])
+def test_gaussian_exp_ulps(G, torch_cuda, k):
+    """The FP64 kernels' exp (the device library's) against numpy's (correctly rounded in practice)
+    over [-745.5, 0] and near its edges: one source at the origin with weight 1
+    and targets whose norms are x, so u_i = exp(-x_i) exactly (d^2 = x_i; the
+    coordinates are zero).  k = 3 runs the vector kernel, k = 16 the matrix-core
+    one.  Bar: 2 ulps in the normal range, 1 ulp of the smallest subnormal
+    below it, exact 1 at x = 0 and 0 past the underflow."""
+    torch = torch_cuda
+    rng = np.random.default_rng(11 + k)
+    x = np.concatenate([[0.0, 5e-324, 1e-300, 1e-17, 0.5, 1.0, np.log(2), 708.0, 708.4, 709.0,
+                         744.0, 745.1, 745.2, 800.0, 1e6],
+                        rng.uniform(0, 1, 4000), rng.uniform(0, 50, 4000),
+                        rng.uniform(0, 745.2, 4000)])
+    m = x.size
+    XA = np.zeros((m, k))
+    XB = np.zeros((1, k))
+    dXA, dXA2, dXB, dXB2 = _dev(torch, XA, x, XB, np.zeros(1))
+    idx = np.arange(m, dtype=np.int32)
+    dm, dz, dw = _dev(torch, idx, np.zeros(1, dtype=np.int32), np.ones(1))
+    du = torch.zeros(m, dtype=torch.float64, device="cuda")
+    G.dgsks(k, du, dm, dXA, dXA2, dm, dXB, dXB2, dz, dw, dz, type="Gaussian", scal=-1.0)
+    torch.cuda.synchronize()
+    got = du.cpu().numpy()
+    want = np.exp(-x)
+    assert got[0] == 1.0 and got[x > 745.2].max() == 0.0
+    tiny = np.finfo(float).tiny
+    ulp = np.spacing(np.maximum(np.abs(want), tiny))
+    err = np.abs(got - want) / ulp
+    normal = want >= tiny
+    print(f"k={k}: max ulp error {err[normal].max():.2f} (normal), "
+          f"{np.abs(got - want)[~normal].max() / 5e-324:.2f} x 2^-1074 (subnormal)")
+    assert err[normal].max() <= 2.0
+    assert np.abs(got - want)[~normal].max() <= 5e-324
+
+
 @pytest.mark.parametrize("k", [3, 16, 64])
 def test_clamp_semantics_with_inconsistent_norms(G, oracle, torch_cuda, k):
     """Norms lowered by 1e-7 (relative) and every target also a source: d^2 of
