@@ -51,6 +51,13 @@ def build_workload(name):
         d = W.mixture_k64()
         d["kernel"] = dict(d["kernel"])
         d["label"] = "mixture_k64 gaussian h=0.3 m=n=131072 k=64"
+    elif name == "mixture_k64_sym":
+        # targets are the sources: one map for both, so the FP64 drop-in takes
+        # its symmetric path (each pair of different row blocks once)
+        d = W.mixture_k64()
+        d["kernel"] = dict(d["kernel"])
+        d["bmap"] = d["amap"]
+        d["label"] = "mixture_k64 gaussian h=0.3 m=n=131072 k=64, one map (symmetric)"
     elif name == "mixture_k64_varbw":
         d = W.mixture_k64()
         d["kernel"] = dict(type=W.VAR_BANDWIDTH)
@@ -354,11 +361,16 @@ def main():
 
     evals = float(m) * float(n)
     flops_per_eval = 2 * k + FLOP_C[kd["type"]]
-    # A symmetric sum (one map and one table for targets and sources, k <= 8)
-    # evaluates each pair of different 1024-row blocks once for both terms: the
-    # roofline counts the evaluations the kernel performs.
-    symmetric = bmap is amap and m == n and k in (2, 3, 4, 8) and m >= 2048
-    B = 1024
+    # A symmetric sum (one map and one table for targets and sources) evaluates
+    # each pair of different row blocks once for both terms -- row blocks of
+    # 1024 on the vector units (k in {2, 3, 4, 8}), of 256 on the matrix cores
+    # (9 <= k <= 64, not Laplace): the roofline counts the evaluations the
+    # kernel performs.
+    one_map = bmap is amap and m == n
+    sym_valu = one_map and k in (2, 3, 4, 8) and m >= 2048
+    sym_mfma = one_map and 9 <= k <= 64 and kd["type"] != 2 and m >= 256
+    symmetric = sym_valu or sym_mfma
+    B = 1024 if sym_valu else 128
     evaluated = (sum(min(B, m - i) * (m - i) for i in range(0, m, B)) if symmetric
                  else evals)
 
@@ -369,7 +381,8 @@ def main():
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": TRAFFIC.get((args.workload, 0)),
                 "flops_per_eval": flops_per_eval, "evaluated_pairs": evaluated,
                 "flop_model": "2k + c_type, test/test_dgsks.c:254-283", "kernel_ms": kernel_ms,
-                "kernel": ("gsks::fused_f64_sym_kernel (+ sym_colsum_kernel)" if symmetric
+                "kernel": ("gsks::fused_f64_sym_kernel (+ sym_colsum_kernel)" if sym_valu
+                           else "gsks::fused_f64_sym_mfma_kernel (+ sym_colsum_kernel)" if sym_mfma
                            else "gsks::fused_f64_mfma_kernel" if 9 <= k <= 64 and kd["type"] != 2
                            else "gsks::fused_f64_kernel")}
 

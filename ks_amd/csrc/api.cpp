@@ -245,9 +245,12 @@ constexpr int kMinCols = 128;     // keep each workgroup's source slice >= 128 p
 constexpr size_t kSymMaxColPartials = (size_t)1 << 29;
 
 int enqueue_fused(const Problem &p, hipStream_t s) {
-  if (sym_eligible(p)) {
-    // sources are the targets: each pair of different row blocks evaluated once
-    const SymPlan pl = plan_sym(p.m, kTargetWGs);
+  // sources are the targets: each pair of different row blocks evaluated once,
+  // on the vector units (k <= 8) or the matrix cores (9 <= k <= 64, not Laplace)
+  const bool sym_valu = sym_eligible(p), sym_mfma = !sym_valu && sym_mfma_eligible(p);
+  if (sym_valu || sym_mfma) {
+    const SymPlan pl = sym_valu ? plan_sym(p.m, kTargetWGs)
+                                : plan_sym_rows(p.m, kTargetWGs, sym_mfma_rows());
     const size_t ncol = (size_t)pl.row_blocks * p.m;
     if (ncol <= kSymMaxColPartials) {
       double *partial = nullptr;
@@ -255,9 +258,10 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
       int rc = get_scratch(s, (size_t)pl.nsplit * p.m + ncol, (size_t)pl.row_blocks, &partial,
                            &counters);
       if (rc != KS_OK) return rc;
-      hipError_t e = launch_fused_f64_sym(p, pl, partial, counters,
-                                          partial + (size_t)pl.nsplit * p.m, s);
-      return e == hipSuccess ? KS_OK : hip_failure(e, "fused_f64_sym launch");
+      double *colp = partial + (size_t)pl.nsplit * p.m;
+      hipError_t e = sym_valu ? launch_fused_f64_sym(p, pl, partial, counters, colp, s)
+                              : launch_fused_f64_sym_mfma(p, pl, partial, counters, colp, s);
+      return e == hipSuccess ? KS_OK : hip_failure(e, "symmetric launch");
     }
   }
   const bool mfma = mfma_f64_eligible(p.type, p.k);

@@ -187,6 +187,205 @@ __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
   if (threadIdx.x == 0) A.counters[rblk] = 0u;
 }
 
+// ------------------------------------------------------------------ symmetric sums
+
+// Sources are the targets (one map, one table, one norm vector, hi == hj):
+// K_ij = K_ji, so each pair of different row blocks is evaluated once for both
+// of its terms, as in fused_f64_sym.hip, with the cross terms on the matrix
+// cores.  A workgroup of 8 waves owns a row block of 256 targets (RB 16-row
+// blocks per wave) and walks the sources j >= its first row: its own block
+// ("diagonal", every pair, row sums only), then the later blocks ("upper", row
+// sums and column sums).  A lane's column sum sum_v K[v] w_row[v] covers its
+// 4 RB rows of its column; two lane exchanges give the wave's 32 rows, and the
+// 8 waves meet in LDS once per 64 sources, in wave order, giving one value per
+// (row block, source) in colp[I][j].  Row sums use the split partials of the
+// general kernel (splits before a block's diagonal hold no work for it), and
+// sym_colsum_kernel adds sum_{I < J} colp[I][j] in block order: deterministic.
+constexpr int SMWG = 256;  // 4 waves
+
+struct SymMArgs {
+  Problem p;
+  int cols_per_split;  // a multiple of the row block
+  int nsplit;
+  double *partial;     // [nsplit][m] row partials
+  unsigned *counters;  // [row blocks], zero between launches
+  double *colp;        // [row blocks][m] column partials of the upper sources
+};
+
+template <int TYPE, int KC, int RB, bool EXACT>
+__global__ __launch_bounds__(SMWG) void fused_f64_sym_mfma_kernel(SymMArgs A) {
+  const Problem &p = A.p;
+  const int m = p.m, k = p.k;  // m == n
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int p16 = lane & 15, g = lane >> 4;
+  const int rblk = blockIdx.x, split = blockIdx.y;
+  constexpr int B = (SMWG / 64) * RB * 16;
+  const int row0 = rblk * B;
+  const int first = row0 / A.cols_per_split;  // the split holding this block's diagonal
+  if (split < first) return;                  // earlier sources are other blocks' upper part
+  const int wrow0 = row0 + wave * (RB * 16);
+
+  // ---- A fragments; norms, bandwidths and weights of the rows of this lane's results
+  double af[RB][KC][4];
+  double aa[RB][4], hi[RB][4], wr[RB][4], racc[RB][4];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) {
+    const int r = wrow0 + rb * 16 + p16;
+    const int ai = p.amap[r < m ? r : m - 1];
+    load_frag<KC, EXACT>(p.XA, (long)ai * k, k, g, af[rb]);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int ro = wrow0 + rb * 16 + g + 4 * v;
+      const int ro_ = ro < m ? ro : m - 1;
+      const int ao = p.amap[ro_];
+      aa[rb][v] = p.XA2[ao];
+      hi[rb][v] = (TYPE == VAR_BANDWIDTH) ? -0.5 * p.hi[ao] : 0.0;
+      wr[rb][v] = ro < m ? p.w[p.wmap[ro_]] : 0.0;  // padding rows add nothing to a column
+      racc[rb][v] = 0.0;
+    }
+  }
+  const int c0 = split * A.cols_per_split;
+  const int c1 = min(m, c0 + A.cols_per_split);
+  const KerParams P = p.P;
+
+  double bf0[KC][4], bf1[KC][4];
+  double bb0 = 0.0, wj0 = 0.0, hj0 = 0.0, bb1 = 0.0, wj1 = 0.0, hj1 = 0.0;
+  auto fetch = [&](int jb, double (&f)[KC][4], double &b2, double &ww, double &hh) {
+    const int col = jb + p16;
+    const bool ok = col < c1;
+    const int bj = p.bmap[ok ? col : c1 - 1];
+    load_frag<KC, EXACT>(p.XB, (long)bj * k, k, g, f);
+    b2 = p.XB2[bj];
+    ww = ok ? p.w[p.wmap[col]] : 0.0;
+    hh = (TYPE == VAR_BANDWIDTH) ? p.hj[bj] : 0.0;
+  };
+  // one 16-source block: row sums, and (upper) the lane's column sum
+  auto block = [&](const double (&f)[KC][4], double bb, double wj, double hj, bool upper) {
+    double cs = 0.0;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int c = 0; c < KC; ++c) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[rb][c][t], f[c][t], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double K = kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P);
+        racc[rb][v] = fma(K, wj, racc[rb][v]);
+        if (upper) cs = fma(K, wr[rb][v], cs);
+      }
+    }
+    return cs;
+  };
+
+  // ---- diagonal sources: every pair, row sums only
+  {
+    const int lo = max(c0, row0), hi_ = min(c1, row0 + B);
+    if (lo < hi_) {
+      fetch(lo, bf0, bb0, wj0, hj0);
+      for (int jb = lo; jb < hi_; jb += 32) {
+        if (jb + 16 < hi_) fetch(jb + 16, bf1, bb1, wj1, hj1);
+        (void)block(bf0, bb0, wj0, hj0, false);
+        if (jb + 16 >= hi_) break;
+        if (jb + 32 < hi_) fetch(jb + 32, bf0, bb0, wj0, hj0);
+        (void)block(bf1, bb1, wj1, hj1, false);
+      }
+    }
+  }
+
+  // ---- upper sources: row sums and column sums, combined every 64 sources
+  __shared__ double s_col[SMWG / 64][64];
+  const int ulo = max(c0, row0 + B);  // a multiple of 64 (of B)
+  auto upper_block = [&](int jb, const double (&f)[KC][4], double bb, double wj, double hj) {
+    double cs = block(f, bb, wj, hj, true);
+    cs += __shfl_xor(cs, 16);
+    cs += __shfl_xor(cs, 32);
+    const int slot = (jb - ulo) & 63;  // 16 q for the q-th block of the chunk
+    if (lane < 16) s_col[wave][slot + lane] = cs;
+    if (slot == 48 || jb + 16 >= c1) {
+      __syncthreads();
+      const int jc = jb - slot;
+      if (tid < 64 && jc + tid < c1) {
+        double t = 0.0;
+#pragma unroll
+        for (int w8 = 0; w8 < SMWG / 64; ++w8) t += s_col[w8][tid];
+        A.colp[(long)rblk * m + jc + tid] = t;
+      }
+      __syncthreads();
+    }
+  };
+  if (ulo < c1) {
+    fetch(ulo, bf0, bb0, wj0, hj0);
+    for (int jb = ulo; jb < c1; jb += 32) {
+      if (jb + 16 < c1) fetch(jb + 16, bf1, bb1, wj1, hj1);
+      upper_block(jb, bf0, bb0, wj0, hj0);
+      if (jb + 16 >= c1) break;
+      if (jb + 32 < c1) fetch(jb + 32, bf0, bb0, wj0, hj0);
+      upper_block(jb + 16, bf1, bb1, wj1, hj1);
+    }
+  }
+
+  // ---- row results: the 16 column lanes of each row group, then the splits
+  // from `first` on in split order
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double s = racc[rb][v];
+      s += __shfl_xor(s, 8, 16);
+      s += __shfl_xor(s, 4, 16);
+      s += __shfl_xor(s, 2, 16);
+      s += __shfl_xor(s, 1, 16);
+      racc[rb][v] = s;
+    }
+  double mine[RB];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) {
+    double x = racc[rb][0];
+    x = (p16 == 1) ? racc[rb][1] : x;
+    x = (p16 == 2) ? racc[rb][2] : x;
+    x = (p16 == 3) ? racc[rb][3] : x;
+    mine[rb] = x;
+  }
+  const bool writer = p16 < 4;
+  const int active = A.nsplit - first;
+  if (active == 1) {
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      const int r = wrow0 + rb * 16 + g + 4 * p16;
+      if (writer && r < m) p.u[p.umap[r]] += mine[rb];
+    }
+    return;
+  }
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) {
+    const int r = wrow0 + rb * 16 + g + 4 * p16;
+    if (writer && r < m) A.partial[(long)split * m + r] = mine[rb];
+  }
+  __threadfence();
+  __syncthreads();
+  __shared__ int s_last;
+  if (tid == 0) {
+    const unsigned prev = atomicAdd(&A.counters[rblk], 1u);
+    s_last = (prev == (unsigned)(active - 1));
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  for (int q = tid; q < B; q += SMWG) {
+    const int r = row0 + q;
+    if (r < m) {
+      double s = 0.0;
+      for (int sp = first; sp < A.nsplit; ++sp) s += A.partial[(long)sp * m + r];
+      p.u[p.umap[r]] += s;
+    }
+  }
+  if (tid == 0) A.counters[rblk] = 0u;
+}
+
 // ------------------------------------------------------------------ launch
 
 constexpr int kMfmaRB = 2;
@@ -236,6 +435,66 @@ hipError_t launch_fused_f64_mfma(const Problem &p, const FusedPlan &plan, double
     case EPANECHNIKOV: return go_type<EPANECHNIKOV>(A, plan.row_blocks, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+}  // namespace gsks
+
+namespace gsks {
+
+bool sym_mfma_eligible(const Problem &p) {
+  if (!mfma_f64_eligible(p.type, p.k)) return false;
+  if (p.m != p.n || p.m < 2 * sym_mfma_rows()) return false;
+  if (p.amap != p.bmap || p.XA != p.XB || p.XA2 != p.XB2) return false;
+  if (p.type == VAR_BANDWIDTH && p.hi != p.hj) return false;
+  return true;
+}
+
+int sym_mfma_rows() { return (SMWG / 64) * kMfmaRB * 16; }
+
+template <int TYPE, int KC>
+static hipError_t sym_mfma_go(const SymMArgs &A, const SymPlan &pl, hipStream_t s) {
+  dim3 grid(pl.row_blocks, pl.nsplit);
+  const bool exact = A.p.k == 16 * KC && (uintptr_t)A.p.XA % 16 == 0;
+  if (exact)
+    hipLaunchKernelGGL((fused_f64_sym_mfma_kernel<TYPE, KC, kMfmaRB, true>), grid, dim3(SMWG), 0, s, A);
+  else
+    hipLaunchKernelGGL((fused_f64_sym_mfma_kernel<TYPE, KC, kMfmaRB, false>), grid, dim3(SMWG), 0, s,
+                       A);
+  return hipGetLastError();
+}
+
+template <int TYPE>
+static hipError_t sym_mfma_type(const SymMArgs &A, const SymPlan &pl, hipStream_t s) {
+  switch ((A.p.k + 15) / 16) {
+    case 1: return sym_mfma_go<TYPE, 1>(A, pl, s);
+    case 2: return sym_mfma_go<TYPE, 2>(A, pl, s);
+    case 3: return sym_mfma_go<TYPE, 3>(A, pl, s);
+    default: return sym_mfma_go<TYPE, 4>(A, pl, s);
+  }
+}
+
+hipError_t launch_fused_f64_sym_mfma(const Problem &p, const SymPlan &pl, double *partial,
+                                     unsigned *counters, double *colp, hipStream_t s) {
+  SymMArgs A;
+  A.p = p;
+  A.cols_per_split = pl.cols_per_split;
+  A.nsplit = pl.nsplit;
+  A.partial = partial;
+  A.counters = counters;
+  A.colp = colp;
+  hipError_t e;
+  switch (p.type) {
+    case GAUSSIAN: e = sym_mfma_type<GAUSSIAN>(A, pl, s); break;
+    case POLYNOMIAL: e = sym_mfma_type<POLYNOMIAL>(A, pl, s); break;
+    case VAR_BANDWIDTH: e = sym_mfma_type<VAR_BANDWIDTH>(A, pl, s); break;
+    case TANH: e = sym_mfma_type<TANH>(A, pl, s); break;
+    case QUARTIC: e = sym_mfma_type<QUARTIC>(A, pl, s); break;
+    case MULTIQUADRATIC: e = sym_mfma_type<MULTIQUADRATIC>(A, pl, s); break;
+    case EPANECHNIKOV: e = sym_mfma_type<EPANECHNIKOV>(A, pl, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
+  return launch_sym_colsum(p, pl.rows_per_wg, colp, s);
 }
 
 }  // namespace gsks

@@ -224,9 +224,11 @@ bool sym_eligible(const Problem &p) {
   return true;
 }
 
-SymPlan plan_sym(int m, int target_wgs) {
+SymPlan plan_sym(int m, int target_wgs) { return plan_sym_rows(m, target_wgs, SWG * 4); }
+
+SymPlan plan_sym_rows(int m, int target_wgs, int rows_per_wg) {
   SymPlan pl;
-  pl.rows_per_wg = SWG * 4;
+  pl.rows_per_wg = rows_per_wg;
   pl.row_blocks = (m + pl.rows_per_wg - 1) / pl.rows_per_wg;
   // about target_wgs working workgroups: row block I has nsplit - I*B/S splits
   long nsplit = (2L * target_wgs + pl.row_blocks - 1) / pl.row_blocks;
@@ -278,8 +280,12 @@ hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *par
     default: return hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(sym_colsum_kernel, dim3((unsigned)((p.m + 255) / 256)), dim3(256), 0, s, p,
-                     pl.rows_per_wg, (const double *)colp);
+  return launch_sym_colsum(p, pl.rows_per_wg, colp, s);
+}
+
+hipError_t launch_sym_colsum(const Problem &p, int B, const double *colp, hipStream_t s) {
+  hipLaunchKernelGGL(sym_colsum_kernel, dim3((unsigned)((p.m + 255) / 256)), dim3(256), 0, s, p, B,
+                     colp);
   return hipGetLastError();
 }
 

@@ -44,6 +44,9 @@ struct SymPlan {
 };
 bool sym_eligible(const Problem &p);
 SymPlan plan_sym(int m, int target_wgs);
+SymPlan plan_sym_rows(int m, int target_wgs, int rows_per_wg);
+// u[umap[j]] += sum_{I < J} colp[I][j] over row blocks of B targets (J = j's block)
+hipError_t launch_sym_colsum(const Problem &p, int B, const double *colp, hipStream_t s);
 // partial: [nsplit][m]; counters: [row_blocks] (zero); colp: [row_blocks][m]
 hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *partial,
                                 unsigned *counters, double *colp, hipStream_t s);
@@ -51,6 +54,11 @@ hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *par
 // FP64 cross term on the matrix cores (fused_f64_mfma.hip)
 bool mfma_f64_eligible(int type, int k);
 int mfma_rows_per_wg();
+// ... and its symmetric form (row blocks of sym_mfma_rows() targets)
+bool sym_mfma_eligible(const Problem &p);
+int sym_mfma_rows();
+hipError_t launch_fused_f64_sym_mfma(const Problem &p, const SymPlan &pl, double *partial,
+                                     unsigned *counters, double *colp, hipStream_t s);
 hipError_t launch_fused_f64_mfma(const Problem &p, const FusedPlan &plan, double *partial,
                                  unsigned *counters, hipStream_t s);
 

@@ -63,9 +63,11 @@ def _run(L, torch, kt, N, k, u, umap, X, X2, amap, bmap, w, wmap):
     torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("k", [3, 8])
+@pytest.mark.parametrize("k", [3, 8, 16, 40, 64])
 @pytest.mark.parametrize("t", list(TYPES))
 def test_symmetric_all_types(L, torch_cuda, oracle, t, k):
+    """k <= 8 runs the vector kernel, 9 <= k <= 64 the matrix-core one (not
+    Laplace, which takes the general path there)."""
     torch = torch_cuda
     N = 5000  # not a multiple of the 1024-row blocks
     X, X2, w, amap, umap, wmap, h = _problem(N, k, 700 + 10 * t + k)
