@@ -37,7 +37,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix peak
 # scaled by what the counter reads for a 1 GiB device copy (FETCH_SIZE x 2,
 # WRITE_SIZE x 1), keyed by (workload, precision); the summaries and commands
 # are in profiles/mixture_k64_fp64_and_mixed.md.
-TRAFFIC = {("mixture_k64", 0): 9.673e9,    # fused_f64_mfma_kernel
+TRAFFIC = {("mixture_k64", 0): 2.344e9,    # fused_f64_mfma_kernel
            ("mixture_k64", 2): 3.43e9}    # mixed_kernel + pack + bucketing pre-pass
 # flops per kernel evaluation in the reference's model, test/test_dgsks.c:254-283
 FLOP_C = {0: 37, 1: 6, 2: 60, 3: 35, 4: 89, 5: 8, 6: 6, 7: 7}

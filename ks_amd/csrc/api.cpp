@@ -241,6 +241,11 @@ int make_params(const ks_t *ker, int k, KerParams &P, bool warn) {
 
 constexpr int kTargetWGs = 2048;  // ~8 workgroups (32 waves) per CU on 256 CUs
 constexpr int kMinCols = 128;     // keep each workgroup's source slice >= 128 points
+// The matrix-core kernel streams 16-source blocks with a two-stage prefetch; a
+// workgroup needs about 64 blocks per wave to amortise its set-up (A fragments,
+// norms) and its split hand-off.  Measured at m = n = 8192, k = 16: 0.205 ms
+// with 256-source slices (2048 workgroups), 0.155 ms with 1024.
+constexpr int kMinColsMfma = 1024;
 // symmetric sums keep one column partial per (row block, source): at most 4 GiB
 constexpr size_t kSymMaxColPartials = (size_t)1 << 29;
 
@@ -265,7 +270,7 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
     }
   }
   const bool mfma = mfma_f64_eligible(p.type, p.k);
-  FusedPlan plan = mfma ? plan_split(p.m, p.n, mfma_rows_per_wg(), kTargetWGs, kMinCols)
+  FusedPlan plan = mfma ? plan_split(p.m, p.n, mfma_rows_per_wg(), kTargetWGs, kMinColsMfma)
                         : plan_fused(p.m, p.n, p.k, kTargetWGs, kMinCols);
   double *partial = nullptr;
   unsigned *counters = nullptr;

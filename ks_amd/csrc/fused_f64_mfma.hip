@@ -92,17 +92,23 @@ __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
   const int j1 = min(p.n, j0 + A.cols_per_split);
   const KerParams P = p.P;
 
-  // ---- stream 16-point source blocks; two register buffers used in turn
+  // ---- stream 16-point source blocks; two register buffers used in turn.  A
+  // block's map entries (bmap, wmap) load two blocks ahead and the data they
+  // address one block ahead, so no block waits on two dependent round trips.
   double bf0[KC][4], bf1[KC][4];
   double bb0 = 0.0, wj0 = 0.0, hj0 = 0.0, bb1 = 0.0, wj1 = 0.0, hj1 = 0.0;
-  auto fetch = [&](int jb, double (&f)[KC][4], double &b2, double &ww, double &hh) {
-    int col = jb + p16;
-    bool ok = col < j1;
-    int bj = p.bmap[ok ? col : j1 - 1];
-    load_frag<KC, EXACT>(p.XB, (long)bj * k, k, g, f);
-    b2 = p.XB2[bj];
-    ww = ok ? p.w[p.wmap[col]] : 0.0;
-    hh = (TYPE == VAR_BANDWIDTH) ? p.hj[bj] : 0.0;
+  int ib0 = 0, iw0 = -1, ib1 = 0, iw1 = -1;
+  auto fetch_idx = [&](int jb, int &ib, int &iw) {
+    const int col = jb + p16;
+    const bool ok = col < j1;
+    ib = p.bmap[ok ? col : j1 - 1];
+    iw = ok ? p.wmap[col] : -1;
+  };
+  auto fetch = [&](int ib, int iw, double (&f)[KC][4], double &b2, double &ww, double &hh) {
+    load_frag<KC, EXACT>(p.XB, (long)ib * k, k, g, f);
+    b2 = p.XB2[ib];
+    ww = iw >= 0 ? p.w[iw] : 0.0;
+    hh = (TYPE == VAR_BANDWIDTH) ? p.hj[ib] : 0.0;
   };
   auto block = [&](const double (&f)[KC][4], double bb, double wj, double hj) {
 #pragma unroll
@@ -121,12 +127,19 @@ __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
       }
     }
   };
-  if (j0 < j1) fetch(j0, bf0, bb0, wj0, hj0);
+  if (j0 < j1) {
+    fetch_idx(j0, ib0, iw0);
+    if (j0 + 16 < j1) fetch_idx(j0 + 16, ib1, iw1);
+    fetch(ib0, iw0, bf0, bb0, wj0, hj0);
+  }
   for (int jb = j0; jb < j1; jb += 32) {
-    if (jb + 16 < j1) fetch(jb + 16, bf1, bb1, wj1, hj1);
+    // here bf0 holds block jb and (ib1, iw1) the map entries of block jb + 16
+    if (jb + 16 < j1) fetch(ib1, iw1, bf1, bb1, wj1, hj1);
+    if (jb + 32 < j1) fetch_idx(jb + 32, ib0, iw0);
     block(bf0, bb0, wj0, hj0);
     if (jb + 16 >= j1) break;
-    if (jb + 32 < j1) fetch(jb + 32, bf0, bb0, wj0, hj0);
+    if (jb + 32 < j1) fetch(ib0, iw0, bf0, bb0, wj0, hj0);
+    if (jb + 48 < j1) fetch_idx(jb + 48, ib1, iw1);
     block(bf1, bb1, wj1, hj1);
   }
 
@@ -248,16 +261,22 @@ __global__ __launch_bounds__(SMWG) void fused_f64_sym_mfma_kernel(SymMArgs A) {
   const int c1 = min(m, c0 + A.cols_per_split);
   const KerParams P = p.P;
 
+  // map entries two blocks ahead, the data they address one block ahead (as
+  // in the general kernel)
   double bf0[KC][4], bf1[KC][4];
   double bb0 = 0.0, wj0 = 0.0, hj0 = 0.0, bb1 = 0.0, wj1 = 0.0, hj1 = 0.0;
-  auto fetch = [&](int jb, double (&f)[KC][4], double &b2, double &ww, double &hh) {
+  int ib0 = 0, iw0 = -1, ib1 = 0, iw1 = -1;
+  auto fetch_idx = [&](int jb, int &ib, int &iw) {
     const int col = jb + p16;
     const bool ok = col < c1;
-    const int bj = p.bmap[ok ? col : c1 - 1];
-    load_frag<KC, EXACT>(p.XB, (long)bj * k, k, g, f);
-    b2 = p.XB2[bj];
-    ww = ok ? p.w[p.wmap[col]] : 0.0;
-    hh = (TYPE == VAR_BANDWIDTH) ? p.hj[bj] : 0.0;
+    ib = p.bmap[ok ? col : c1 - 1];
+    iw = ok ? p.wmap[col] : -1;
+  };
+  auto fetch = [&](int ib, int iw, double (&f)[KC][4], double &b2, double &ww, double &hh) {
+    load_frag<KC, EXACT>(p.XB, (long)ib * k, k, g, f);
+    b2 = p.XB2[ib];
+    ww = iw >= 0 ? p.w[iw] : 0.0;
+    hh = (TYPE == VAR_BANDWIDTH) ? p.hj[ib] : 0.0;
   };
   // one 16-source block: row sums, and (upper) the lane's column sum
   auto block = [&](const double (&f)[KC][4], double bb, double wj, double hj, bool upper) {
@@ -285,12 +304,16 @@ __global__ __launch_bounds__(SMWG) void fused_f64_sym_mfma_kernel(SymMArgs A) {
   {
     const int lo = max(c0, row0), hi_ = min(c1, row0 + B);
     if (lo < hi_) {
-      fetch(lo, bf0, bb0, wj0, hj0);
+      fetch_idx(lo, ib0, iw0);
+      if (lo + 16 < hi_) fetch_idx(lo + 16, ib1, iw1);
+      fetch(ib0, iw0, bf0, bb0, wj0, hj0);
       for (int jb = lo; jb < hi_; jb += 32) {
-        if (jb + 16 < hi_) fetch(jb + 16, bf1, bb1, wj1, hj1);
+        if (jb + 16 < hi_) fetch(ib1, iw1, bf1, bb1, wj1, hj1);
+        if (jb + 32 < hi_) fetch_idx(jb + 32, ib0, iw0);
         (void)block(bf0, bb0, wj0, hj0, false);
         if (jb + 16 >= hi_) break;
-        if (jb + 32 < hi_) fetch(jb + 32, bf0, bb0, wj0, hj0);
+        if (jb + 32 < hi_) fetch(ib0, iw0, bf0, bb0, wj0, hj0);
+        if (jb + 48 < hi_) fetch_idx(jb + 48, ib1, iw1);
         (void)block(bf1, bb1, wj1, hj1, false);
       }
     }
@@ -318,12 +341,16 @@ __global__ __launch_bounds__(SMWG) void fused_f64_sym_mfma_kernel(SymMArgs A) {
     }
   };
   if (ulo < c1) {
-    fetch(ulo, bf0, bb0, wj0, hj0);
+    fetch_idx(ulo, ib0, iw0);
+    if (ulo + 16 < c1) fetch_idx(ulo + 16, ib1, iw1);
+    fetch(ib0, iw0, bf0, bb0, wj0, hj0);
     for (int jb = ulo; jb < c1; jb += 32) {
-      if (jb + 16 < c1) fetch(jb + 16, bf1, bb1, wj1, hj1);
+      if (jb + 16 < c1) fetch(ib1, iw1, bf1, bb1, wj1, hj1);
+      if (jb + 32 < c1) fetch_idx(jb + 32, ib0, iw0);
       upper_block(jb, bf0, bb0, wj0, hj0);
       if (jb + 16 >= c1) break;
-      if (jb + 32 < c1) fetch(jb + 32, bf0, bb0, wj0, hj0);
+      if (jb + 32 < c1) fetch(ib0, iw0, bf0, bb0, wj0, hj0);
+      if (jb + 48 < c1) fetch_idx(jb + 48, ib1, iw1);
       upper_block(jb + 16, bf1, bb1, wj1, hj1);
     }
   }
