@@ -268,18 +268,24 @@ __device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c
         racc[rb][v] = 0.0;
       }
     }
-    // two source-fragment buffers in turn: the next block's loads are in flight
-    // while this block runs (the last block's "next" is an in-bounds re-read)
+    // two source-fragment buffers in turn: a block's map entries load two
+    // blocks ahead and the data they address one block ahead, so no block
+    // waits on two dependent round trips (blocks past the slice re-read its
+    // last block, in bounds, and are not evaluated)
     double bf0[KC][4], bf1[KC][4];
     double bb0, wj0, hj0, bb1, wj1, hj1;
-    auto fetch = [&](int jb, double (&f)[KC][4], double &b2, double &ww, double &hh) {
+    int ib0, iw0, ib1, iw1;
+    auto fetch_idx = [&](int jb, int &ib, int &iw) {
       const int col = jb + p16;
       const bool ok = col < c1;
-      const int bj = L.bmap[T.b0 + (ok ? col : c1 - 1)];
-      frag(L.XB, (long)bj * k, f);
-      b2 = L.XB2[bj];
-      ww = ok ? L.w[L.wmap[T.b0 + col]] : 0.0;
-      hh = (TYPE == VAR_BANDWIDTH) ? T.hj[bj] : 0.0;
+      ib = L.bmap[T.b0 + (ok ? col : c1 - 1)];
+      iw = ok ? L.wmap[T.b0 + col] : -1;
+    };
+    auto fetch = [&](int ib, int iw, double (&f)[KC][4], double &b2, double &ww, double &hh) {
+      frag(L.XB, (long)ib * k, f);
+      b2 = L.XB2[ib];
+      ww = iw >= 0 ? L.w[iw] : 0.0;
+      hh = (TYPE == VAR_BANDWIDTH) ? T.hj[ib] : 0.0;
     };
     auto block = [&](const double (&f)[KC][4], double bb, double wj, double hj) {
 #pragma unroll
@@ -296,12 +302,17 @@ __device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c
               fma(kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P), wj, racc[rb][v]);
       }
     };
-    fetch(c0, bf0, bb0, wj0, hj0);
+    fetch_idx(c0, ib0, iw0);
+    fetch_idx(min(c0 + 16, jlast), ib1, iw1);
+    fetch(ib0, iw0, bf0, bb0, wj0, hj0);
     for (int jb = c0;; jb += 32) {
-      fetch(min(jb + 16, jlast), bf1, bb1, wj1, hj1);
+      // here bf0 holds block jb and (ib1, iw1) the map entries of block jb + 16
+      fetch(ib1, iw1, bf1, bb1, wj1, hj1);
+      fetch_idx(min(jb + 32, jlast), ib0, iw0);
       block(bf0, bb0, wj0, hj0);
       if (jb + 16 >= c1) break;
-      fetch(min(jb + 32, jlast), bf0, bb0, wj0, hj0);
+      fetch(ib0, iw0, bf0, bb0, wj0, hj0);
+      fetch_idx(min(jb + 48, jlast), ib1, iw1);
       block(bf1, bb1, wj1, hj1);
       if (jb + 32 >= c1) break;
     }
