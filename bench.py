@@ -70,6 +70,10 @@ def build_workload(name):
         d = W.nbody_k3()
         d["kernel"] = dict(W.NBODY_KERNELS[kind])
         d["label"] = f"nbody_k3 {kind} m=n=262144 k=3 permutation maps"
+    elif name == "grad_gauss16":
+        d = W.grad_gauss16()
+        d["label"] = ("grad_gauss16 gaussian h=0.4 m=n=32768 k=16: forward u = K w and the "
+                      "gradient of g.u in dXA, dXB, dw, dh")
     elif name.startswith("list_pool"):
         k = int(name.split("_k")[1]) if "_k" in name else 3
         d = W.list_pool(k=k)
@@ -229,6 +233,56 @@ def bench_list(args, d, L):
     print(json.dumps(out), flush=True)
 
 
+def bench_grad(args, d):
+    """One step = the forward summation and the whole gradient of L = g.u
+    (dXA, dXB, dw, dh: two multi-right-hand-side passes that recompute K), FP64.
+    Flops per pair by the reference's model for the forward (2k + 37) plus
+    4k + 4 for the backward; HIP events on the stream time each step."""
+    import torch
+    from ks_amd import autograd as AG
+    XA = torch.from_numpy(d["XA"]).cuda()
+    XB = torch.from_numpy(d["XB"]).cuda()
+    w = torch.from_numpy(d["w"]).cuda()
+    g = torch.from_numpy(d["g"]).cuda()
+    h, k = d["h"], d["k"]
+    m, n = XA.shape[0], XB.shape[0]
+    stream = torch.cuda.current_stream()
+
+    def step():
+        AG.gaussian_summation_forward(XA, XB, w, h)
+        AG.gaussian_summation_grad(XA, XB, w, g, h)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / args.steps
+    kms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    pairs = float(m) * float(n)
+    fpe = (2 * k + FLOP_C[0]) + (4 * k + 4)
+    achieved = pairs * fpe / (kms / 1e3) / 1e12
+    out = {"metric": METRIC, "value": pairs / (wall / 1e3), "unit": "evals/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded numpy, regenerated each run)",
+           "config": {"workload": d["label"], "m": m, "n": n, "k": k, "kernel": "gaussian"},
+           "roofline": {"bound": "mfma", "pipe": "fp64 (matrix and vector peaks are equal)",
+                        "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                        "flops_per_eval": fpe,
+                        "flop_model": "2k + 37 forward (test/test_dgsks.c:254-283) + 4k + 4 "
+                                      "backward", "kernel_ms": kms,
+                        "kernel": "gsks::fused_f64_mfma_kernel + gsks::multirhs_kernel (2 passes)"}}
+    print(json.dumps(out), flush=True)
+
+
 def bench_single(L, kt, d, k, amap, bmap, X, w, stream, steps, u64):
     """The single-precision front end (sgsks_device) on float copies of the same
     inputs, norms left to the library, in the precision sgsks() would pick:
@@ -304,6 +358,8 @@ def main():
     d = build_workload(args.workload)
     if "kernels" in d:
         return bench_list(args, d, L)
+    if "g" in d:
+        return bench_grad(args, d)
     kd = d["kernel"]
     X = torch.from_numpy(d["X"]).cuda()
     X2 = torch.from_numpy(d["X2"]).cuda()

@@ -33,30 +33,6 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int MWG = 256;  // 4 waves
 
-// Operand fragment of one point: coordinates c*16 + 4g .. +3 for each 16-chunk c.
-// EXACT (k == 16*KC, 16-byte aligned rows): two unguarded 16-byte loads per chunk.
-template <int KC, bool EXACT>
-__device__ __forceinline__ void load_frag(const double *__restrict__ X, long base, int k, int g,
-                                          double (&f)[KC][4]) {
-#pragma unroll
-  for (int c = 0; c < KC; ++c) {
-    if constexpr (EXACT) {
-      const double2 *q = reinterpret_cast<const double2 *>(X + base + c * 16 + 4 * g);
-      const double2 x0 = q[0], x1 = q[1];
-      f[c][0] = x0.x;
-      f[c][1] = x0.y;
-      f[c][2] = x1.x;
-      f[c][3] = x1.y;
-    } else {
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        int idx = c * 16 + 4 * g + t;
-        f[c][t] = (idx < k) ? X[base + idx] : 0.0;
-      }
-    }
-  }
-}
-
 template <int TYPE, int KC, int RB, bool EXACT>
 __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
   const Problem &p = A.p;

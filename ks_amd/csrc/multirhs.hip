@@ -129,6 +129,173 @@ __global__ __launch_bounds__(MR_WG) void multirhs_kernel(MrhsArgs A) {
   if (tid == 0) A.counters[blockIdx.x] = 0u;
 }
 
+// Matrix-core form (9 <= k <= 64, every type but Laplace, nrhs <= 16 NT): the
+// cross term is formed with the sources as the A operand and the targets as the
+// B operand of v_mfma_f64_16x16x4f64, so lane (p16, g) ends up holding
+// K(target p16, source 4v + g) for v = 0..3 -- exactly the A operand of the
+// next product, chunk v of U[targets][rhs] += K[targets][sources] W[sources][rhs].
+// Per 16 x 16 block that is 4 KC cross-term MFMAs, the 4 kernel values per lane
+// on the VALU, and 4 NT MFMAs for the right-hand sides; nothing is transposed.
+// A wave owns RB groups of 16 targets; splits are reduced as in the VALU form.
+constexpr int MRM_RB = 2;
+constexpr int MRM_ROWS = 4 * MRM_RB * 16;  // targets per workgroup
+
+template <int TYPE, int KC, int NT, bool EXACT>
+__global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  constexpr int RB = MRM_RB;
+  const Problem &p = A.p;
+  const int m = p.m, k = p.k, nr = A.nrhs;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p16 = lane & 15, g = lane >> 4;
+  const int rblk = blockIdx.x, split = blockIdx.y;
+  const int row0 = rblk * MRM_ROWS + wave * (RB * 16);
+
+  // targets (B operand): lane (p16, g) -> target row0 + 16 rb + p16
+  double tf[RB][KC][4], aa[RB], hi[RB];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) {
+    const int r = row0 + 16 * rb + p16;
+    const int ai = p.amap[r < m ? r : m - 1];
+    load_frag<KC, EXACT>(p.XA, (long)ai * k, k, g, tf[rb]);
+    aa[rb] = p.XA2[ai];
+    hi[rb] = (TYPE == VAR_BANDWIDTH) ? -0.5 * p.hi[ai] : 0.0;
+  }
+  d4 U[RB][NT];  // lane (p16, g): targets 16 rb + g + 4v' x rhs 16 t + p16
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) U[rb][t] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const int j0 = split * A.cols_per_split;
+  const int j1 = min(p.n, j0 + A.cols_per_split);
+  const KerParams P = p.P;
+  // one 16-source block: the A-operand fragment (lane p16 = source), and for
+  // the sources 4v + g of the lane's results their norms, bandwidths and
+  // weight rows (the B operand of the right-hand-side product)
+  struct SB {
+    double f[KC][4];
+    double bb[4], hj[4];
+    double wv[4][NT];
+  };
+  auto fetch = [&](int jb, SB &b) {
+    const int col = jb + p16;
+    load_frag<KC, EXACT>(p.XB, (long)p.bmap[col < j1 ? col : j1 - 1] * k, k, g, b.f);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int src = jb + 4 * v + g;
+      const bool ok = src < j1;
+      const int bj = p.bmap[ok ? src : j1 - 1];
+      b.bb[v] = p.XB2[bj];
+      b.hj[v] = (TYPE == VAR_BANDWIDTH) ? p.hj[bj] : 0.0;
+      const long wrow = ok ? (long)p.wmap[src] * nr : -1;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int r = 16 * t + p16;
+        b.wv[v][t] = (wrow >= 0 && r < nr) ? p.w[wrow + r] : 0.0;
+      }
+    }
+  };
+  auto block = [&](const SB &b) {
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      d4 c = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int cc = 0; cc < KC; ++cc)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          c = __builtin_amdgcn_mfma_f64_16x16x4f64(b.f[cc][t], tf[rb][cc][t], c, 0, 0, 0);
+      double Kv[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Kv[v] = kval_f64<TYPE>(c[v], aa[rb], b.bb[v], hi[rb], b.hj[v], P);
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          U[rb][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Kv[v], b.wv[v][t], U[rb][t], 0, 0, 0);
+    }
+  };
+  SB b0, b1;
+  if (j0 < j1) fetch(j0, b0);
+  for (int jb = j0; jb < j1; jb += 32) {
+    if (jb + 16 < j1) fetch(jb + 16, b1);
+    block(b0);
+    if (jb + 16 >= j1) break;
+    if (jb + 32 < j1) fetch(jb + 32, b0);
+    block(b1);
+  }
+
+  // ---- results: lane (p16, g) holds target 16 rb + g + 4v, rhs 16 t + p16
+  if (A.nsplit == 1) {
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = row0 + 16 * rb + g + 4 * v, r = 16 * t + p16;
+          if (i < m && r < nr) p.u[(long)p.umap[i] * nr + r] += U[rb][t][v];
+        }
+    return;
+  }
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = row0 + 16 * rb + g + 4 * v, r = 16 * t + p16;
+        if (i < m && r < nr) A.partial[((long)split * m + i) * nr + r] = U[rb][t][v];
+      }
+  __threadfence();
+  __syncthreads();
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+    const unsigned prev = atomicAdd(&A.counters[rblk], 1u);
+    s_last = (prev == (unsigned)(A.nsplit - 1));
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  const int base = rblk * MRM_ROWS;
+  for (int e = threadIdx.x; e < MRM_ROWS * nr; e += MR_WG) {
+    const int i = base + e / nr, r = e % nr;
+    if (i < m) {
+      double s = 0.0;
+      for (int sp = 0; sp < A.nsplit; ++sp) s += A.partial[((long)sp * m + i) * nr + r];
+      p.u[(long)p.umap[i] * nr + r] += s;
+    }
+  }
+  if (threadIdx.x == 0) A.counters[rblk] = 0u;
+}
+
+template <int TYPE, int KC, int NT>
+static hipError_t mrm_go(const MrhsArgs &A, int rblocks, hipStream_t s) {
+  dim3 grid(rblocks, A.nsplit);
+  const bool exact = A.p.k == 16 * KC && (uintptr_t)A.p.XA % 16 == 0 &&
+                     (uintptr_t)A.p.XB % 16 == 0;
+  if (exact)
+    hipLaunchKernelGGL((multirhs_mfma_kernel<TYPE, KC, NT, true>), grid, dim3(MR_WG), 0, s, A);
+  else
+    hipLaunchKernelGGL((multirhs_mfma_kernel<TYPE, KC, NT, false>), grid, dim3(MR_WG), 0, s, A);
+  return hipGetLastError();
+}
+
+template <int TYPE, int NT>
+static hipError_t mrm_kc(const MrhsArgs &A, int rblocks, hipStream_t s) {
+  switch ((A.p.k + 15) / 16) {
+    case 1: return mrm_go<TYPE, 1, NT>(A, rblocks, s);
+    case 2: return mrm_go<TYPE, 2, NT>(A, rblocks, s);
+    case 3: return mrm_go<TYPE, 3, NT>(A, rblocks, s);
+    default: return mrm_go<TYPE, 4, NT>(A, rblocks, s);
+  }
+}
+
+template <int TYPE>
+static hipError_t mrm_nt(const MrhsArgs &A, int rblocks, hipStream_t s) {
+  return A.nrhs <= 16 ? mrm_kc<TYPE, 1>(A, rblocks, s) : mrm_kc<TYPE, 2>(A, rblocks, s);
+}
+
 template <int TYPE, int RMAX>
 static hipError_t mr_k(const MrhsArgs &A, int rblocks, hipStream_t s) {
   dim3 grid(rblocks, A.nsplit);
@@ -149,7 +316,10 @@ static hipError_t mr_r(const MrhsArgs &A, int rblocks, hipStream_t s) {
 
 int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s) {
   if (nrhs < 1 || nrhs > 72) return KS_ERR_INVALID_ARG;
-  FusedPlan plan = plan_split(p.m, p.n, MR_WG, 2048, 128);
+  // the matrix-core form for 9 <= k <= 64 (not Laplace) and up to 32 right-hand sides
+  const bool mfma = mfma_f64_eligible(p.type, p.k) && nrhs <= 32;
+  FusedPlan plan = mfma ? plan_split(p.m, p.n, MRM_ROWS, 2048, 1024)
+                        : plan_split(p.m, p.n, MR_WG, 2048, 128);
   double *partial = nullptr;
   unsigned *counters = nullptr;
   if (plan.nsplit > 1) {
@@ -165,6 +335,18 @@ int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s) {
   A.partial = partial;
   A.counters = counters;
   hipError_t e;
+  if (mfma) {
+    switch (p.type) {
+      case GAUSSIAN: e = mrm_nt<GAUSSIAN>(A, plan.row_blocks, s); break;
+      case POLYNOMIAL: e = mrm_nt<POLYNOMIAL>(A, plan.row_blocks, s); break;
+      case VAR_BANDWIDTH: e = mrm_nt<VAR_BANDWIDTH>(A, plan.row_blocks, s); break;
+      case TANH: e = mrm_nt<TANH>(A, plan.row_blocks, s); break;
+      case QUARTIC: e = mrm_nt<QUARTIC>(A, plan.row_blocks, s); break;
+      case MULTIQUADRATIC: e = mrm_nt<MULTIQUADRATIC>(A, plan.row_blocks, s); break;
+      default: e = mrm_nt<EPANECHNIKOV>(A, plan.row_blocks, s); break;
+    }
+    return e == hipSuccess ? KS_OK : KS_ERR_DEVICE;
+  }
   switch (p.type) {
     case GAUSSIAN: e = mr_r<GAUSSIAN>(A, plan.row_blocks, s); break;
     case POLYNOMIAL: e = mr_r<POLYNOMIAL>(A, plan.row_blocks, s); break;

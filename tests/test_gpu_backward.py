@@ -115,3 +115,55 @@ def test_multirhs_matches_single_rhs(torch_cuda, oracle):
         ref = oracle.dgsks_rows(2, X, X2, amap, X, X2, bmap, Wm[:, r].copy(), wmap, np.zeros(nx),
                                 umap)
         assert oracle.rel_l2(Uh[umap, r], ref) <= 1e-12, r
+
+
+@pytest.mark.parametrize("t,k,R", [(0, 16, 7), (1, 40, 16), (3, 64, 23), (4, 32, 32), (5, 20, 3),
+                                   (6, 16, 17), (7, 64, 9), (0, 16, 40)])
+def test_multirhs_matrix_core_form(torch_cuda, oracle, t, k, R):
+    """The matrix-core multi-RHS form (9 <= k <= 64, not Laplace, R <= 32: the
+    sources are the A operand of the cross term, so the kernel values land in
+    the A-operand layout of the right-hand-side product) against the oracle,
+    column by column, with random maps and ragged sizes; R = 40 runs the
+    vector form."""
+    torch = torch_cuda
+    from ks_amd import _lib
+    L = _lib.lib()
+    prm = {0: dict(scal=-1.5), 1: dict(scal=0.3, cons=0.5, powe=3.0), 3: dict(),
+           4: dict(scal=0.2, cons=0.1), 5: dict(), 6: dict(cons=0.5), 7: dict()}[t]
+    rng = np.random.default_rng(100 + t + k + R)
+    nx, m, n = 2500, 1111, 1777
+    X = rng.random((nx, k)) * (1.2 / np.sqrt(k))
+    X2 = oracle.sqnorm(X)
+    amap = rng.permutation(nx)[:m].astype(np.int32)
+    umap = rng.permutation(nx)[:m].astype(np.int32)
+    bmap = rng.permutation(nx)[:n].astype(np.int32)
+    wmap = rng.permutation(nx)[:n].astype(np.int32)
+    Wm = rng.standard_normal((nx, R))
+    hv = rng.uniform(1.0, 3.0, nx)
+    U0 = rng.standard_normal((nx, R))
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    dX, dX2, da, du_, db, dwm, dW, dh, U = map(dev, (X, X2, amap, umap, bmap, wmap, Wm, hv, U0))
+    kt = _lib.KsT()
+    kt.type = t
+    for key, val in prm.items():
+        setattr(kt, key, val)
+    if t == 3:
+        kt.hi = kt.hj = dh.data_ptr()
+    P = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+    rc = L.dgsks_multirhs_device(ctypes.byref(kt), m, n, k, R, P(U), P(du_), P(dX), P(dX2), P(da),
+                                 P(dX), P(dX2), P(db), P(dW), P(dwm),
+                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    Uh = U.cpu().numpy()
+    h = hv if t == 3 else None
+    worst = 0.0
+    for r in range(R):
+        ref = oracle.dgsks_rows(t, X, X2, amap, X, X2, bmap, Wm[:, r].copy(), wmap, U0[:, r].copy(),
+                                umap, hi=h, hj=h, **prm)
+        worst = max(worst, oracle.rel_l2(Uh[umap, r], ref))
+    print(f"multirhs type {t} k={k} R={R}: worst column rel L2 = {worst:.3e}")
+    assert worst <= 1e-12
+    # targets no umap entry names keep their initial values
+    rest = np.setdiff1d(np.arange(nx), umap)
+    assert np.array_equal(Uh[rest], U0[rest])
