@@ -279,7 +279,7 @@ def bench_grad(args, d):
                         "flops_per_eval": fpe,
                         "flop_model": "2k + 37 forward (test/test_dgsks.c:254-283) + 4k + 4 "
                                       "backward", "kernel_ms": kms,
-                        "kernel": "gsks::fused_f64_mfma_kernel + gsks::multirhs_kernel (2 passes)"}}
+                        "kernel": "gsks::fused_f64_mfma_kernel + gsks::multirhs_mfma_kernel (2 passes)"}}
     print(json.dumps(out), flush=True)
 
 

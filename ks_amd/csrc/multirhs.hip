@@ -12,13 +12,16 @@
 // split slices are reduced deterministically by the last workgroup.
 //
 // The gradient of L = g . u for u = K w, K_ij = exp(-|a_i - b_j|^2 / (2 h^2)),
-// uses two such passes (identity maps):
-//   rows = a, sources = b, RHS [w, w|b|^2, w b_1..w b_k] -> [u_i, T_i, V_i]
-//   rows = b, sources = a, RHS [g, g a_1..g a_k]         -> [dw_j, Q_j]
+// uses two such passes (identity maps), k + 1 right-hand sides each:
+//   rows = a, sources = b, RHS [w b_1..w b_k, w] -> [V_i, u_i]
+//   rows = b, sources = a, RHS [g a_1..g a_k, g] -> [Q_j, dw_j]
 // and then, per point,
 //   dXA_i = -(g_i / h^2) (u_i a_i - V_i)
 //   dXB_j =  (w_j / h^2) (Q_j - dw_j b_j)
-//   dh    = (1 / h^3) sum_i g_i (|a_i|^2 u_i + T_i - 2 a_i . V_i).
+//   dh    = (1 / h^3) (sum_i g_i (|a_i|^2 u_i - 2 a_i . V_i) + sum_j w_j |b_j|^2 dw_j),
+// the last sum being sum_i g_i sum_j K_ij w_j |b_j|^2 regrouped by source, so
+// neither pass needs a w |b|^2 column (at k = 16 both have 17 columns: one
+// 16-wide matrix-core tile and one column on the VALU).
 #include <hip/hip_runtime.h>
 
 #include "gsks_kernels.h"
@@ -136,11 +139,14 @@ __global__ __launch_bounds__(MR_WG) void multirhs_kernel(MrhsArgs A) {
 // next product, chunk v of U[targets][rhs] += K[targets][sources] W[sources][rhs].
 // Per 16 x 16 block that is 4 KC cross-term MFMAs, the 4 kernel values per lane
 // on the VALU, and 4 NT MFMAs for the right-hand sides; nothing is transposed.
+// With XR, right-hand side 16 NT (the last, nrhs = 16 NT + 1) is summed on the
+// VALU instead of a further 16-wide tile: 4 FMAs per lane per block, and the 4
+// lane groups (sources 4v + g) are added by two lane shuffles at the end.
 // A wave owns RB groups of 16 targets; splits are reduced as in the VALU form.
 constexpr int MRM_RB = 2;
 constexpr int MRM_ROWS = 4 * MRM_RB * 16;  // targets per workgroup
 
-template <int TYPE, int KC, int NT, bool EXACT>
+template <int TYPE, int KC, int NT, bool EXACT, bool XR>
 __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
   typedef double d4 __attribute__((ext_vector_type(4)));
   constexpr int RB = MRM_RB;
@@ -162,10 +168,13 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
     hi[rb] = (TYPE == VAR_BANDWIDTH) ? -0.5 * p.hi[ai] : 0.0;
   }
   d4 U[RB][NT];  // lane (p16, g): targets 16 rb + g + 4v' x rhs 16 t + p16
+  double ux[RB];  // XR: right-hand side 16 NT, lane (p16, g)'s sources of target p16
 #pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
+  for (int rb = 0; rb < RB; ++rb) {
+    ux[rb] = 0.0;
 #pragma unroll
     for (int t = 0; t < NT; ++t) U[rb][t] = d4{0.0, 0.0, 0.0, 0.0};
+  }
 
   const int j0 = split * A.cols_per_split;
   const int j1 = min(p.n, j0 + A.cols_per_split);
@@ -177,6 +186,7 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
     double f[KC][4];
     double bb[4], hj[4];
     double wv[4][NT];
+    double wx[4];  // XR: weight 16 NT of the sources 4v + g
   };
   auto fetch = [&](int jb, SB &b) {
     const int col = jb + p16;
@@ -194,6 +204,7 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
         const int r = 16 * t + p16;
         b.wv[v][t] = (wrow >= 0 && r < nr) ? p.w[wrow + r] : 0.0;
       }
+      if constexpr (XR) b.wx[v] = wrow >= 0 ? p.w[wrow + 16 * NT] : 0.0;
     }
   };
   auto block = [&](const SB &b) {
@@ -213,6 +224,10 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
 #pragma unroll
         for (int v = 0; v < 4; ++v)
           U[rb][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Kv[v], b.wv[v][t], U[rb][t], 0, 0, 0);
+      if constexpr (XR) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ux[rb] = fma(Kv[v], b.wx[v], ux[rb]);
+      }
     }
   };
   SB b0, b1;
@@ -225,10 +240,18 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
     block(b1);
   }
 
-  // ---- results: lane (p16, g) holds target 16 rb + g + 4v, rhs 16 t + p16
+  // ---- results: lane (p16, g) holds target 16 rb + g + 4v, rhs 16 t + p16;
+  // the extra right-hand side sums its 4 lane groups into lanes g = 0
+  if constexpr (XR) {
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      ux[rb] += __shfl_xor(ux[rb], 16);
+      ux[rb] += __shfl_xor(ux[rb], 32);
+    }
+  }
   if (A.nsplit == 1) {
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+    for (int rb = 0; rb < RB; ++rb) {
 #pragma unroll
       for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -236,10 +259,15 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
           const int i = row0 + 16 * rb + g + 4 * v, r = 16 * t + p16;
           if (i < m && r < nr) p.u[(long)p.umap[i] * nr + r] += U[rb][t][v];
         }
+      if constexpr (XR) {
+        const int i = row0 + 16 * rb + p16;
+        if (g == 0 && i < m) p.u[(long)p.umap[i] * nr + 16 * NT] += ux[rb];
+      }
+    }
     return;
   }
 #pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
+  for (int rb = 0; rb < RB; ++rb) {
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -247,6 +275,11 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
         const int i = row0 + 16 * rb + g + 4 * v, r = 16 * t + p16;
         if (i < m && r < nr) A.partial[((long)split * m + i) * nr + r] = U[rb][t][v];
       }
+    if constexpr (XR) {
+      const int i = row0 + 16 * rb + p16;
+      if (g == 0 && i < m) A.partial[((long)split * m + i) * nr + 16 * NT] = ux[rb];
+    }
+  }
   __threadfence();
   __syncthreads();
   __shared__ int s_last;
@@ -269,31 +302,34 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
   if (threadIdx.x == 0) A.counters[rblk] = 0u;
 }
 
-template <int TYPE, int KC, int NT>
+template <int TYPE, int KC, int NT, bool XR>
 static hipError_t mrm_go(const MrhsArgs &A, int rblocks, hipStream_t s) {
   dim3 grid(rblocks, A.nsplit);
   const bool exact = A.p.k == 16 * KC && (uintptr_t)A.p.XA % 16 == 0 &&
                      (uintptr_t)A.p.XB % 16 == 0;
   if (exact)
-    hipLaunchKernelGGL((multirhs_mfma_kernel<TYPE, KC, NT, true>), grid, dim3(MR_WG), 0, s, A);
+    hipLaunchKernelGGL((multirhs_mfma_kernel<TYPE, KC, NT, true, XR>), grid, dim3(MR_WG), 0, s, A);
   else
-    hipLaunchKernelGGL((multirhs_mfma_kernel<TYPE, KC, NT, false>), grid, dim3(MR_WG), 0, s, A);
+    hipLaunchKernelGGL((multirhs_mfma_kernel<TYPE, KC, NT, false, XR>), grid, dim3(MR_WG), 0, s, A);
   return hipGetLastError();
 }
 
-template <int TYPE, int NT>
+template <int TYPE, int NT, bool XR>
 static hipError_t mrm_kc(const MrhsArgs &A, int rblocks, hipStream_t s) {
   switch ((A.p.k + 15) / 16) {
-    case 1: return mrm_go<TYPE, 1, NT>(A, rblocks, s);
-    case 2: return mrm_go<TYPE, 2, NT>(A, rblocks, s);
-    case 3: return mrm_go<TYPE, 3, NT>(A, rblocks, s);
-    default: return mrm_go<TYPE, 4, NT>(A, rblocks, s);
+    case 1: return mrm_go<TYPE, 1, NT, XR>(A, rblocks, s);
+    case 2: return mrm_go<TYPE, 2, NT, XR>(A, rblocks, s);
+    case 3: return mrm_go<TYPE, 3, NT, XR>(A, rblocks, s);
+    default: return mrm_go<TYPE, 4, NT, XR>(A, rblocks, s);
   }
 }
 
+// 16-wide tiles on the matrix cores; a 17th right-hand side on the VALU
 template <int TYPE>
 static hipError_t mrm_nt(const MrhsArgs &A, int rblocks, hipStream_t s) {
-  return A.nrhs <= 16 ? mrm_kc<TYPE, 1>(A, rblocks, s) : mrm_kc<TYPE, 2>(A, rblocks, s);
+  if (A.nrhs <= 16) return mrm_kc<TYPE, 1, false>(A, rblocks, s);
+  if (A.nrhs == 17) return mrm_kc<TYPE, 1, true>(A, rblocks, s);
+  return mrm_kc<TYPE, 2, false>(A, rblocks, s);
 }
 
 template <int TYPE, int RMAX>
@@ -362,26 +398,14 @@ int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s) {
 
 // ------------------------------------------------------------------ Gaussian gradient
 
-// RHS of the first pass: [w, w|b|^2, w b]
-__global__ void grad_rhs_b(const double *XB, const double *XB2, const double *w, int n, int k,
-                           double *W1) {
+// RHS of one pass: [s x_1 .. s x_k, s] for the points x (scale s = w or g)
+__global__ void grad_rhs(const double *X, const double *s, int n, int k, double *W) {
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const int R = k + 2;
-  const double wj = w[j];
-  W1[j * R] = wj;
-  W1[j * R + 1] = wj * XB2[j];
-  for (int q = 0; q < k; ++q) W1[j * R + 2 + q] = wj * XB[j * k + q];
-}
-
-// RHS of the second pass: [g, g a]
-__global__ void grad_rhs_a(const double *XA, const double *g, int m, int k, double *W2) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
   const int R = k + 1;
-  const double gi = g[i];
-  W2[i * R] = gi;
-  for (int q = 0; q < k; ++q) W2[i * R + 1 + q] = gi * XA[i * k + q];
+  const double sj = s[j];
+  for (int q = 0; q < k; ++q) W[j * R + q] = sj * X[j * k + q];
+  W[j * R + k] = sj;
 }
 
 // dXA, and per-row dh terms (summed afterwards in a fixed order)
@@ -389,38 +413,47 @@ __global__ void grad_fin_a(const double *XA, const double *XA2, const double *g,
                            int m, int k, double h, double *dXA, double *dh_rows) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  const int R = k + 2;
+  const int R = k + 1;
   const double *u = U1 + i * R;
-  const double ui = u[0], Ti = u[1];
+  const double ui = u[k];
   const double f = -g[i] / (h * h);
   double av = 0.0;
   for (int q = 0; q < k; ++q) {
-    const double a = XA[i * k + q], V = u[2 + q];
+    const double a = XA[i * k + q], V = u[q];
     dXA[i * k + q] = f * (ui * a - V);
     av = fma(a, V, av);
   }
-  dh_rows[i] = g[i] * (XA2[i] * ui + Ti - 2.0 * av);
+  dh_rows[i] = g[i] * (XA2[i] * ui - 2.0 * av);
 }
 
-__global__ void grad_fin_b(const double *XB, const double *w, const double *U2, int n, int k,
-                           double h, double *dXB, double *dw) {
+// dXB, dw, and per-source dh terms w_j |b_j|^2 dw_j
+__global__ void grad_fin_b(const double *XB, const double *XB2, const double *w, const double *U2,
+                           int n, int k, double h, double *dXB, double *dw, double *dh_rows) {
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const int R = k + 1;
   const double *u = U2 + j * R;
-  const double f = w[j] / (h * h);
-  dw[j] = u[0];
-  for (int q = 0; q < k; ++q) dXB[j * k + q] = f * (u[1 + q] - u[0] * XB[j * k + q]);
+  const double f = w[j] / (h * h), dwj = u[k];
+  dw[j] = dwj;
+  for (int q = 0; q < k; ++q) dXB[j * k + q] = f * (u[q] - dwj * XB[j * k + q]);
+  dh_rows[j] = w[j] * XB2[j] * dwj;
 }
 
-// deterministic sum of the per-row dh terms: one workgroup, fixed tree
-__global__ void grad_fin_h(const double *dh_rows, int m, double h, double *dh) {
-  __shared__ double s[256];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < m; i += 256) acc += dh_rows[i];
-  s[threadIdx.x] = acc;
+// deterministic sum of the per-point dh terms: one workgroup of 1024, each lane
+// with 4 interleaved accumulators (independent loads in flight), fixed tree
+constexpr int GF_WG = 1024;
+__global__ __launch_bounds__(GF_WG) void grad_fin_h(const double *dh_rows, int m, double h,
+                                                    double *dh) {
+  __shared__ double s[GF_WG];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  int i = threadIdx.x;
+  for (; i + 3 * GF_WG < m; i += 4 * GF_WG)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] += dh_rows[i + q * GF_WG];
+  for (int q = 0; i < m; i += GF_WG, ++q) acc[q] += dh_rows[i];
+  s[threadIdx.x] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = GF_WG / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
@@ -480,9 +513,9 @@ extern "C" int ks_gauss_grad_device(int m, int n, int k, const double *XA, const
   if (ensure_device() != KS_OK)
     return raise_error(KS_ERR_DEVICE, false, "ks_gauss_grad_device(): no GPU available\n");
   hipStream_t s = (hipStream_t)stream;
-  const int R1 = k + 2, R2 = k + 1;
+  const int R1 = k + 1, R2 = k + 1;
   size_t bytes = sizeof(double) * ((size_t)m + n + (size_t)n * R1 + (size_t)m * R1 +
-                                   (size_t)m * R2 + (size_t)n * R2 + m + (size_t)m * k +
+                                   (size_t)m * R2 + (size_t)n * R2 + m + n + (size_t)m * k +
                                    (size_t)n * k + n + 1) +
                  sizeof(int) * ((size_t)(m > n ? m : n)) + 16 * 256;
   char *buf = nullptr;
@@ -502,7 +535,7 @@ extern "C" int ks_gauss_grad_device(int m, int n, int k, const double *XA, const
   double *U1 = (double *)take(sizeof(double) * (size_t)m * R1);
   double *W2 = (double *)take(sizeof(double) * (size_t)m * R2);
   double *U2 = (double *)take(sizeof(double) * (size_t)n * R2);
-  double *hrows = (double *)take(sizeof(double) * m);
+  double *hrows = (double *)take(sizeof(double) * ((size_t)m + n));  // per target, per source
   double *tXA = (double *)take(sizeof(double) * (size_t)m * k);
   double *tXB = (double *)take(sizeof(double) * (size_t)n * k);
   double *tdw = (double *)take(sizeof(double) * n);
@@ -513,8 +546,8 @@ extern "C" int ks_gauss_grad_device(int m, int n, int k, const double *XA, const
   rc |= ks_sqnorm_device(XA, m, k, XA2, s);
   rc |= ks_sqnorm_device(XB, n, k, XB2, s);
   hipLaunchKernelGGL(iota_i32, dim3((mx + 255) / 256), dim3(256), 0, s, ident, mx);
-  hipLaunchKernelGGL(grad_rhs_b, dim3((n + 255) / 256), dim3(256), 0, s, XB, XB2, w, n, k, W1);
-  hipLaunchKernelGGL(grad_rhs_a, dim3((m + 255) / 256), dim3(256), 0, s, XA, g, m, k, W2);
+  hipLaunchKernelGGL(grad_rhs, dim3((n + 255) / 256), dim3(256), 0, s, XB, w, n, k, W1);
+  hipLaunchKernelGGL(grad_rhs, dim3((m + 255) / 256), dim3(256), 0, s, XA, g, m, k, W2);
   hipLaunchKernelGGL(zero_f64, dim3((unsigned)(((long)m * R1 + 255) / 256)), dim3(256), 0, s, U1,
                      (long)m * R1);
   hipLaunchKernelGGL(zero_f64, dim3((unsigned)(((long)n * R2 + 255) / 256)), dim3(256), 0, s, U2,
@@ -533,9 +566,9 @@ extern "C" int ks_gauss_grad_device(int m, int n, int k, const double *XA, const
   if (rc == KS_OK) {
     hipLaunchKernelGGL(grad_fin_a, dim3((m + 255) / 256), dim3(256), 0, s, XA, XA2, g, U1, m, k, h,
                        dXA ? dXA : tXA, hrows);
-    hipLaunchKernelGGL(grad_fin_b, dim3((n + 255) / 256), dim3(256), 0, s, XB, w, U2, n, k, h,
-                       dXB ? dXB : tXB, dw ? dw : tdw);
-    hipLaunchKernelGGL(grad_fin_h, dim3(1), dim3(256), 0, s, hrows, m, h, dh ? dh : tdh);
+    hipLaunchKernelGGL(grad_fin_b, dim3((n + 255) / 256), dim3(256), 0, s, XB, XB2, w, U2, n, k, h,
+                       dXB ? dXB : tXB, dw ? dw : tdw, hrows + m);
+    hipLaunchKernelGGL(grad_fin_h, dim3(1), dim3(GF_WG), 0, s, hrows, m + n, h, dh ? dh : tdh);
     if (hipGetLastError() != hipSuccess) rc = KS_ERR_DEVICE;
   }
   (void)hipFreeAsync(buf, s);
