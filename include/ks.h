@@ -210,14 +210,30 @@ int dgsks_multirhs_device(const ks_t *kernel, int m, int n, int k, int nrhs,
                           const double *XB, const double *XB2, const int *bmap,
                           const double *W, const int *wmap, void *stream);
 
+/* Mixed-precision form of dgsks_multirhs_device for the Gaussian kernel
+ * (k <= 64, nrhs <= 33; no counterpart in the reference, whose summations are
+ * FP64 only).  Every 16 x 16 pair block whose exponent error, bounded from the
+ * norms of the points translated by their mean, is at most 2^-15 is evaluated
+ * with FP32 cross terms and kernel values on the matrix cores (FP64 norms,
+ * distance assembly and accumulation); every other block exactly as the FP64
+ * form.  Device pointers, enqueued on `stream`. */
+int dgsks_multirhs_mixed_device(const ks_t *kernel, int m, int n, int k, int nrhs,
+                                double *U, const int *umap,
+                                const double *XA, const double *XA2, const int *amap,
+                                const double *XB, const double *XB2, const int *bmap,
+                                const double *W, const int *wmap, void *stream);
+
 /* Gradient of L = sum_i g[i] u[i] for the Gaussian summation u = K w,
  * K_ij = exp(-|a_i - b_j|^2 / (2 h^2)), identity maps, XA [m][k], XB [n][k]:
  * dXA [m][k], dXB [n][k], dw [n] (the transposed summation) and dh [1]
  * (outputs may be NULL).  The kernel matrix is recomputed, never stored.
- * Device pointers, enqueued on `stream`; k <= 70. */
+ * precision: KS_PREC_FP64 (k <= 70) or KS_PREC_MIXED_FP32 (k <= 32: the two
+ * summations through dgsks_multirhs_mixed_device's kernel on centred points).
+ * Device pointers, enqueued on `stream`. */
 int ks_gauss_grad_device(int m, int n, int k, const double *XA, const double *XB,
                          const double *w, const double *g, double h,
-                         double *dXA, double *dXB, double *dw, double *dh, void *stream);
+                         double *dXA, double *dXB, double *dw, double *dh, int precision,
+                         void *stream);
 
 /* Batched summation: ntask independent summations over shared point pools,
  * one persistent launch.  Task t owns the ranges [a_off[t], a_off[t+1]) of

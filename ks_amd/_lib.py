@@ -59,8 +59,12 @@ _SIGS = {
                                   _vp, _vp, ctypes.c_int]),
     "dgsks_multirhs_device": (ctypes.c_int, [ctypes.POINTER(KsT), ctypes.c_int, ctypes.c_int,
                                              ctypes.c_int, ctypes.c_int] + [_vp] * 10 + [_vp]),
+    "dgsks_multirhs_mixed_device": (ctypes.c_int, [ctypes.POINTER(KsT), ctypes.c_int,
+                                                   ctypes.c_int, ctypes.c_int, ctypes.c_int]
+                                    + [_vp] * 10 + [_vp]),
     "ks_gauss_grad_device": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp,
-                                            _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
+                                            _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp,
+                                            ctypes.c_int, _vp]),
     "ks_list_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(KsT),
                                            ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                            ctypes.c_int]),

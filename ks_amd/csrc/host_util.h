@@ -30,6 +30,21 @@ int enqueue_reference(const Problem &p, hipStream_t s);
 int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s);
 // Mixed-precision summation (KS_PREC_MIXED_*).
 int enqueue_mixed(const Problem &p, int precision, hipStream_t s);
+// Mixed-precision multi-right-hand-side Gaussian summation (FP32 matrix cores on
+// centred points where a per-block bound allows, FP64 elsewhere).  `centre`
+// (k doubles, device) translates the points for the FP32 tier; NULL computes
+// the mean of the mapped targets and sources.
+constexpr int kMixedMrhsMaxK = 64;
+constexpr int kMixedMrhsMaxRhs = 33;
+int enqueue_multirhs_mixed(const Problem &p, int nrhs, const double *centre, hipStream_t s);
+// centre[q] = mean over the m rows of XA and the n rows of XB (identity map ident),
+// summed in a fixed order through `work` (centre_work_elems(m + n) doubles).
+size_t centre_work_elems(long rows);
+int enqueue_centre(const double *XA, int m, const double *XB, int n, int k, const int *ident,
+                   double *work, double *centre, hipStream_t s);
+// Xc = X - centre, row by row (X [rows][k]).
+int enqueue_shift(const double *X, int rows, int k, const double *centre, double *Xc,
+                  hipStream_t s);
 
 // Single-precision problem (sgsks / sgsks_device): float arrays, FP64 bandwidths.
 struct SingleProblem {

@@ -504,12 +504,26 @@ extern "C" int dgsks_multirhs_device(const ks_t *kernel, int m, int n, int k, in
 // Gradient of L = g . (K w) for the Gaussian kernel with bandwidth h and identity
 // maps; all arrays device memory, XA [m][k], XB [n][k].  Outputs dXA [m][k],
 // dXB [n][k], dw [n], dh [1] (any output may be NULL).  Temporary buffers come from
-// the stream-ordered pool.
+// the stream-ordered pool.  KS_PREC_MIXED_FP32 (k <= 32) translates both point
+// sets by their mean (the gradient does not change) and runs the two passes
+// through the mixed multi-right-hand-side kernel; the right-hand sides and the
+// finishing kernels then work on the centred FP64 coordinates, so u a - V and
+// Q - dw b are formed from the smallest coordinates the data allows.
 extern "C" int ks_gauss_grad_device(int m, int n, int k, const double *XA, const double *XB,
                                     const double *w, const double *g, double h, double *dXA,
-                                    double *dXB, double *dw, double *dh, void *stream) {
+                                    double *dXB, double *dw, double *dh, int precision,
+                                    void *stream) {
   if (m <= 0 || n <= 0 || k <= 0 || k > 70 || !(h > 0.0) || !XA || !XB || !w || !g)
     return raise_error(KS_ERR_INVALID_ARG, false, "ks_gauss_grad_device(): invalid argument\n");
+  const bool mixed = precision == KS_PREC_MIXED_FP32;
+  if (!mixed && precision != KS_PREC_FP64)
+    return raise_error(KS_ERR_INVALID_ARG, false,
+                       "ks_gauss_grad_device(): precision must be KS_PREC_FP64 or "
+                       "KS_PREC_MIXED_FP32\n");
+  if (mixed && k + 1 > kMixedMrhsMaxRhs)
+    return raise_error(KS_ERR_INVALID_ARG, false,
+                       "ks_gauss_grad_device(): the mixed gradient takes k <= %d\n",
+                       kMixedMrhsMaxRhs - 1);
   if (ensure_device() != KS_OK)
     return raise_error(KS_ERR_DEVICE, false, "ks_gauss_grad_device(): no GPU available\n");
   hipStream_t s = (hipStream_t)stream;
@@ -517,7 +531,10 @@ extern "C" int ks_gauss_grad_device(int m, int n, int k, const double *XA, const
   size_t bytes = sizeof(double) * ((size_t)m + n + (size_t)n * R1 + (size_t)m * R1 +
                                    (size_t)m * R2 + (size_t)n * R2 + m + n + (size_t)m * k +
                                    (size_t)n * k + n + 1) +
-                 sizeof(int) * ((size_t)(m > n ? m : n)) + 16 * 256;
+                 sizeof(int) * ((size_t)(m > n ? m : n)) + 24 * 256;
+  if (mixed)
+    bytes += sizeof(double) * ((size_t)m * k + (size_t)n * k + 2 * 64 +
+                               centre_work_elems((long)m + n));
   char *buf = nullptr;
   if (hipMallocAsync((void **)&buf, bytes, s) != hipSuccess) {
     (void)hipGetLastError();
@@ -543,11 +560,29 @@ extern "C" int ks_gauss_grad_device(int m, int n, int k, const double *XA, const
   int *ident = (int *)take(sizeof(int) * (size_t)(m > n ? m : n));
   const int mx = m > n ? m : n;
   int rc = KS_OK;
-  rc |= ks_sqnorm_device(XA, m, k, XA2, s);
-  rc |= ks_sqnorm_device(XB, n, k, XB2, s);
+  auto chain = [&](int r) {  // keeps the first failure
+    if (rc == KS_OK) rc = r;
+  };
   hipLaunchKernelGGL(iota_i32, dim3((mx + 255) / 256), dim3(256), 0, s, ident, mx);
-  hipLaunchKernelGGL(grad_rhs, dim3((n + 255) / 256), dim3(256), 0, s, XB, w, n, k, W1);
-  hipLaunchKernelGGL(grad_rhs, dim3((m + 255) / 256), dim3(256), 0, s, XA, g, m, k, W2);
+  const double *A = XA, *B = XB;  // the coordinates every later step uses
+  double *zc = nullptr;
+  if (mixed) {
+    double *XAc = (double *)take(sizeof(double) * (size_t)m * k);
+    double *XBc = (double *)take(sizeof(double) * (size_t)n * k);
+    double *cen = (double *)take(sizeof(double) * 64);
+    zc = (double *)take(sizeof(double) * 64);
+    double *cwork = (double *)take(sizeof(double) * centre_work_elems((long)m + n));
+    hipLaunchKernelGGL(zero_f64, dim3(1), dim3(64), 0, s, zc, 64l);
+    chain(enqueue_centre(XA, m, XB, n, k, ident, cwork, cen, s));
+    chain(enqueue_shift(XA, m, k, cen, XAc, s));
+    chain(enqueue_shift(XB, n, k, cen, XBc, s));
+    A = XAc;
+    B = XBc;
+  }
+  chain(ks_sqnorm_device(A, m, k, XA2, s));
+  chain(ks_sqnorm_device(B, n, k, XB2, s));
+  hipLaunchKernelGGL(grad_rhs, dim3((n + 255) / 256), dim3(256), 0, s, B, w, n, k, W1);
+  hipLaunchKernelGGL(grad_rhs, dim3((m + 255) / 256), dim3(256), 0, s, A, g, m, k, W2);
   hipLaunchKernelGGL(zero_f64, dim3((unsigned)(((long)m * R1 + 255) / 256)), dim3(256), 0, s, U1,
                      (long)m * R1);
   hipLaunchKernelGGL(zero_f64, dim3((unsigned)(((long)n * R2 + 255) / 256)), dim3(256), 0, s, U2,
@@ -555,18 +590,21 @@ extern "C" int ks_gauss_grad_device(int m, int n, int k, const double *XA, const
   Problem p1;
   p1.m = m; p1.n = n; p1.k = k; p1.type = GAUSSIAN;
   p1.P = KerParams{-1.0 / (2.0 * h * h), 0.0, 0.0, 0, POW_GENERAL};
-  p1.u = U1; p1.umap = ident; p1.XA = XA; p1.XA2 = XA2; p1.amap = ident;
-  p1.XB = XB; p1.XB2 = XB2; p1.bmap = ident; p1.w = W1; p1.wmap = ident;
+  p1.u = U1; p1.umap = ident; p1.XA = A; p1.XA2 = XA2; p1.amap = ident;
+  p1.XB = B; p1.XB2 = XB2; p1.bmap = ident; p1.w = W1; p1.wmap = ident;
   p1.hi = p1.hj = nullptr;
-  if (rc == KS_OK) rc = enqueue_multirhs(p1, R1, s);
+  auto pass = [&](const Problem &pp, int R) {
+    return mixed ? enqueue_multirhs_mixed(pp, R, zc, s) : enqueue_multirhs(pp, R, s);
+  };
+  if (rc == KS_OK) rc = pass(p1, R1);
   Problem p2 = p1;
   p2.m = n; p2.n = m;
-  p2.u = U2; p2.XA = XB; p2.XA2 = XB2; p2.XB = XA; p2.XB2 = XA2; p2.w = W2;
-  if (rc == KS_OK) rc = enqueue_multirhs(p2, R2, s);
+  p2.u = U2; p2.XA = B; p2.XA2 = XB2; p2.XB = A; p2.XB2 = XA2; p2.w = W2;
+  if (rc == KS_OK) rc = pass(p2, R2);
   if (rc == KS_OK) {
-    hipLaunchKernelGGL(grad_fin_a, dim3((m + 255) / 256), dim3(256), 0, s, XA, XA2, g, U1, m, k, h,
+    hipLaunchKernelGGL(grad_fin_a, dim3((m + 255) / 256), dim3(256), 0, s, A, XA2, g, U1, m, k, h,
                        dXA ? dXA : tXA, hrows);
-    hipLaunchKernelGGL(grad_fin_b, dim3((n + 255) / 256), dim3(256), 0, s, XB, XB2, w, U2, n, k, h,
+    hipLaunchKernelGGL(grad_fin_b, dim3((n + 255) / 256), dim3(256), 0, s, B, XB2, w, U2, n, k, h,
                        dXB ? dXB : tXB, dw ? dw : tdw, hrows + m);
     hipLaunchKernelGGL(grad_fin_h, dim3(1), dim3(GF_WG), 0, s, hrows, m + n, h, dh ? dh : tdh);
     if (hipGetLastError() != hipSuccess) rc = KS_ERR_DEVICE;
