@@ -72,6 +72,32 @@ __device__ __forceinline__ double ipow_f64(double x, int e) {
   return e < 0 ? 1.0 / r : r;
 }
 
+// x[i]^e for N values at once (e wave-uniform): one squaring ladder for all N,
+// so their multiplications interleave instead of running N dependent loops.
+// The same products in the same order as ipow_f64, so the results are equal.
+template <int N>
+__device__ __forceinline__ void ipow_n_f64(double (&x)[N], int e) {
+  int ae = e < 0 ? -e : e;
+  double r[N], b[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    r[i] = 1.0;
+    b[i] = x[i];
+  }
+#pragma unroll 1
+  while (ae) {
+    if (ae & 1) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) r[i] *= b[i];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) b[i] *= b[i];
+    ae >>= 1;
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] = e < 0 ? 1.0 / r[i] : r[i];
+}
+
 __device__ __forceinline__ double powe_f64(double x, const KerParams &P) {
   switch (P.pmode) {
     case POW_SQUARE: return x * x;
@@ -87,6 +113,16 @@ __device__ __forceinline__ double laplace_f64(double d2, const KerParams &P) {
   double s = rsqrt(d2);
   double r = (P.ipow == 1) ? s : ipow_f64(s, P.ipow);
   return P.scal * r;
+}
+
+// laplace_f64 of N squared distances at once (the power ladder shared), in place.
+template <int N>
+__device__ __forceinline__ void laplace_n_f64(double (&d2)[N], const KerParams &P) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) d2[i] = rsqrt((d2[i] < 1e-15) ? 1.79e308 : d2[i]);
+  if (P.ipow != 1) ipow_n_f64<N>(d2, P.ipow);
+#pragma unroll
+  for (int i = 0; i < N; ++i) d2[i] = P.scal * d2[i];
 }
 
 // Pointer into the constant address space: loads through it are known not to be

@@ -9,11 +9,15 @@
 // tile kind are the autotuned tile shape of the task's (kernel type, k).  Per tile:
 //   * k = 3 or 8 (and k outside 9..64): lane = target row, the row in VGPRs,
 //     sources as wave-uniform scalar loads (as fused_f64_kernel);
-//   * 9 <= k <= 64, row tile (always for Laplace, whose rank-k sum must keep
-//     the reference's order): lane = target row, the row in VGPRs padded to
-//     16 / 32 / 64 coordinates, sources staged through LDS;
-//   * 9 <= k <= 64, matrix-core tile (other types): 4 blocks of 16 rows on the
-//     FP64 matrix cores (as fused_f64_mfma_kernel).
+//   * 9 <= k <= 64, row tile: lane = target row, the row in VGPRs padded to
+//     16 / 32 / 64 coordinates, sources staged through LDS (Laplace keeps the
+//     reference's sequential rank-k sum here);
+//   * 9 <= k <= 64, matrix-core tile: 4 blocks of 16 rows on the FP64 matrix
+//     cores (as fused_f64_mfma_kernel).  For Laplace, whose d^2 < 1e-15 drop
+//     makes the result discontinuous, a per-pair bound on the difference from
+//     the reference's sequential sum sends every pair that could be dropped
+//     differently, or whose power d^(2-k) could lose more than 1e-11, through
+//     that exact sum; on spread points only coincident pairs take it.
 // Each tile writes 64 row partials; the last tile of a (task, row block) to
 // finish (device-scope counter) adds its partials in slice order into
 // R[a_off[t] + i].  The last wave to leave resets the tile counter, so the
@@ -44,7 +48,8 @@ struct DevTask {
   int type;
   int m, n;
   int bc;    // source slice per tile (the tuned tile shape)
-  int tile;  // 1: FP64 matrix-core tile (9 <= k <= 64), 0: lane-per-row tile
+  int tile;  // launch kind: 0 lane-per-row tile, 1 FP64 matrix-core tile (9 <= k <= 64),
+             // 2 the same for Laplace (a launch of its own)
   int pad;
   long a0, b0;
   KerParams P;
@@ -250,6 +255,14 @@ __device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c
   };
   const KerParams P = T.P;
   const int jlast = c0 + ((c1 - c0 - 1) >> 4) * 16;
+  // Laplace: |d2 (matrix cores) - d2 (reference's sum)| <= (2 (k + 2) + 4) 2^-53 (|a|^2 + |b|^2)
+  // (both rank-k sums within (k + 1) 2^-53 |a||b| of a.b, plus the assembly);
+  // pairs with d2 below the drop threshold plus that bound, or below the bound
+  // times 1 + (k/2 - 1) 1e11 (a relative error above 1e-11 after the power
+  // d^(2-k)), are recomputed exactly
+  const double lap_b = (2.0 * (k + 2) + 4.0) * 0x1p-53;
+  const double lap_t0 = 1e-15;
+  const double lap_t1 = lap_b * (1.0 + fmax(0.5 * k - 1.0, 1.0) * 1e11) + lap_b;
 #pragma unroll 1
   for (int pass = 0; pass < 4 / RBP; ++pass) {
     const int rbase = row0 + pass * RBP * 16;
@@ -287,7 +300,7 @@ __device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c
       ww = iw >= 0 ? L.w[iw] : 0.0;
       hh = (TYPE == VAR_BANDWIDTH) ? T.hj[ib] : 0.0;
     };
-    auto block = [&](const double (&f)[KC][4], double bb, double wj, double hj) {
+    auto block = [&](const double (&f)[KC][4], double bb, double wj, double hj, int ib) {
 #pragma unroll
       for (int rb = 0; rb < RBP; ++rb) {
         d4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -296,24 +309,48 @@ __device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c
 #pragma unroll
           for (int t = 0; t < 4; ++t)
             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[rb][c][t], f[c][t], acc, 0, 0, 0);
+        if constexpr (TYPE == LAPLACE) {
+          double d2[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          racc[rb][v] =
-              fma(kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P), wj, racc[rb][v]);
+          for (int v = 0; v < 4; ++v) {
+            d2[v] = fma(-2.0, acc[v], aa[rb][v]) + bb;  // the assembly of kval_f64
+            // a pair whose d^2 could fall on the other side of the drop
+            // threshold, or lose more than 1e-11 through the power, takes the
+            // reference's exact sequential sum (and so its d^2, bit for bit)
+            if (d2[v] < lap_t0 + lap_t1 * (aa[rb][v] + bb)) {
+              const int ro = rbase + rb * 16 + g + 4 * v;
+              const int ao = L.amap[T.a0 + (ro < T.m ? ro : T.m - 1)];
+              const double c =
+                  dot_ref_order<LAPLACE>(L.XA + (long)ao * k, to_const(L.XB + (long)ib * k), k);
+              d2[v] = fma(-2.0, c, aa[rb][v]) + bb;
+            }
+          }
+          laplace_n_f64<4>(d2, P);  // the 4 kernel values, one power ladder
+#pragma unroll
+          for (int v = 0; v < 4; ++v) racc[rb][v] = fma(d2[v], wj, racc[rb][v]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            racc[rb][v] =
+                fma(kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P), wj, racc[rb][v]);
+        }
       }
     };
     fetch_idx(c0, ib0, iw0);
     fetch_idx(min(c0 + 16, jlast), ib1, iw1);
     fetch(ib0, iw0, bf0, bb0, wj0, hj0);
+    int sb0 = ib0, sb1;  // the source rows of the blocks held in bf0 / bf1
     for (int jb = c0;; jb += 32) {
       // here bf0 holds block jb and (ib1, iw1) the map entries of block jb + 16
       fetch(ib1, iw1, bf1, bb1, wj1, hj1);
+      sb1 = ib1;
       fetch_idx(min(jb + 32, jlast), ib0, iw0);
-      block(bf0, bb0, wj0, hj0);
+      block(bf0, bb0, wj0, hj0, sb0);
       if (jb + 16 >= c1) break;
       fetch(ib0, iw0, bf0, bb0, wj0, hj0);
+      sb0 = ib0;
       fetch_idx(min(jb + 48, jlast), ib1, iw1);
-      block(bf1, bb1, wj1, hj1);
+      block(bf1, bb1, wj1, hj1, sb1);
       if (jb + 32 >= c1) break;
     }
 #pragma unroll
@@ -335,15 +372,13 @@ __device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c
   return mine;
 }
 
-// MF = 1: matrix-core tiles (padded K = 16 -> one 16-coordinate chunk, 32 ->
-// two, 64 -> three or four; never Laplace); MF = 0: row tiles.
+// MF = 1, 2: matrix-core tiles (padded K = 16 -> one 16-coordinate chunk, 32 ->
+// two, 64 -> three or four); MF = 0: row tiles.
 template <int TYPE, int K, int MF>
 __device__ double run_tile(const ListArgs &L, const DevTask &T, int row0, int c0, int c1,
                            double *red, double *stage) {
-  if constexpr (MF == 1) {
-    if constexpr (TYPE == LAPLACE) {
-      return 0.0;  // not reached: the plan never puts a Laplace task on the matrix cores
-    } else if constexpr (K == 16) {
+  if constexpr (MF >= 1) {
+    if constexpr (K == 16) {
       return tile_mfma<TYPE, 1>(L, T, row0, c0, c1, red);
     } else if constexpr (K == 32) {
       return tile_mfma<TYPE, 2>(L, T, row0, c0, c1, red);
@@ -358,9 +393,10 @@ __device__ double run_tile(const ListArgs &L, const DevTask &T, int row0, int c0
   }
 }
 
-// One persistent launch over one tile kind (MF): the row tiles and the
-// matrix-core tiles of a batch run as two launches, so that each kernel's
-// register allocation is that of its own tile.
+// One persistent launch over one tile kind (MF): the row tiles, the matrix-core
+// tiles and the Laplace matrix-core tiles (whose exact-recompute path needs more
+// registers) of a batch run as three launches, so that each kernel's register
+// allocation is that of its own tiles.
 template <int K, int MF>
 __global__ __launch_bounds__(256) void list_kernel(ListArgs L) {
   const int lane = threadIdx.x & 63;
@@ -382,10 +418,14 @@ __global__ __launch_bounds__(256) void list_kernel(ListArgs L) {
     const int c0 = tl.y * T.bc;
     const int c1 = min(T.n, c0 + T.bc);
     double s;
-    switch (T.type) {
+    if constexpr (MF == 2) {
+      s = run_tile<LAPLACE, K, MF>(L, T, rb.row0, c0, c1, red, stage);
+    } else switch (T.type) {
       case GAUSSIAN: s = run_tile<GAUSSIAN, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
       case POLYNOMIAL: s = run_tile<POLYNOMIAL, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
-      case LAPLACE: s = run_tile<LAPLACE, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+      case LAPLACE:  // row tiles only: Laplace matrix-core tiles are the MF = 2 launch
+        s = MF == 0 ? run_tile<LAPLACE, K, 0>(L, T, rb.row0, c0, c1, red, stage) : 0.0;
+        break;
       case VAR_BANDWIDTH: s = run_tile<VAR_BANDWIDTH, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
       case TANH: s = run_tile<TANH, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
       case QUARTIC: s = run_tile<QUARTIC, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
@@ -445,7 +485,7 @@ static void launch_list(const ListArgs &A, int grid, hipStream_t s) {
   hipLaunchKernelGGL((list_kernel<K, MF>), dim3(grid), dim3(256), 0, s, A);
 }
 
-bool list_mfma_eligible(int type, int k) { return type != KS_LAPLACE && k >= 9 && k <= 64; }
+bool list_mfma_eligible(int type, int k) { return k >= 9 && k <= 64; }
 
 // Device state of one batch layout: everything a run needs except the point
 // data, so a run is kernel launches only.
@@ -454,9 +494,9 @@ struct ListPlan {
   long total_a = 0, ntiles = 0;
   char *buf = nullptr;
   // layout part (tiles, row blocks, tasks, partials, counters, R) of the row-tile
-  // launch [0] and the matrix-core launch [1]
-  ListArgs A[2]{};
-  int grid[2] = {0, 0};
+  // launch [0], the matrix-core launch [1] and the Laplace matrix-core launch [2]
+  ListArgs A[3]{};
+  int grid[3] = {0, 0, 0};
   const int *keys = nullptr, *pos = nullptr;  // umap sorted, with positions
   std::vector<TaskDesc> mixed;                // tasks the mixed kernels run
   const int *umap = nullptr, *amap = nullptr, *bmap = nullptr, *wmap = nullptr;
@@ -488,12 +528,15 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
     }
     const TileChoice tc = force ? *force : list_tile_choice(td.type, k, KS_PREC_FP64);
     dt[t].bc = tc.slice;
-    dt[t].tile = (tc.tile == 1 && list_mfma_eligible(td.type, k)) ? 1 : 0;
+    dt[t].tile = (tc.tile == 1 && list_mfma_eligible(td.type, k))
+                     ? (td.type == KS_LAPLACE ? 2 : 1)
+                     : 0;
     const int nsplit = (td.n + dt[t].bc - 1) / dt[t].bc;
     for (int r0 = 0; r0 < td.m; r0 += 64) rbs.push_back(RowBlock{(int)t, r0, nsplit, 0});
   }
-  // row blocks of row tiles first, then those of matrix-core tiles; within each
-  // kind by work (columns per tile x rows) descending; then lay tiles out
+  // row blocks of row tiles first, then those of matrix-core tiles, then those of
+  // Laplace matrix-core tiles; within each kind by work (columns per tile x
+  // rows) descending; then lay tiles out
   std::vector<int> order(rbs.size());
   std::iota(order.begin(), order.end(), 0);
   auto work = [&](const RowBlock &r) {
@@ -511,7 +554,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   for (size_t i = 0; i < order.size(); ++i) rbs_sorted[i] = rbs[order[i]];
   rbs.swap(rbs_sorted);
   std::vector<int2> tiles;
-  long nkind[2] = {0, 0};
+  long nkind[3] = {0, 0, 0};
   for (size_t i = 0; i < rbs.size(); ++i) {
     rbs[i].tile_base = (int)tiles.size();
     for (int sp = 0; sp < rbs[i].nsplit; ++sp) tiles.push_back(make_int2((int)i, sp));
@@ -539,7 +582,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   size_t off_part = off_tasks + al(dt.size() * sizeof(DevTask));
   size_t off_cnt = off_part + al((size_t)pl->ntiles * 64 * sizeof(double));
   size_t off_head = off_cnt + al(rbs.size() * sizeof(unsigned));
-  size_t off_R = off_head + 256;  // two launches' {head, done} counters
+  size_t off_R = off_head + 256;  // three launches' {head, done} counters
   size_t off_iota = off_R + al((size_t)total_a * sizeof(double));
   size_t off_kout = off_iota + al((size_t)total_a * sizeof(int));
   size_t off_vout = off_kout + al((size_t)total_a * sizeof(int));
@@ -560,9 +603,9 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
       hipMemcpyAsync(buf + off_tasks, dt.data(), dt.size() * sizeof(DevTask), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemsetAsync(buf + off_cnt, 0, off_iota - off_cnt, s) != hipSuccess)
     rc = KS_ERR_DEVICE;
-  for (int kind = 0; kind < 2; ++kind) {
+  for (int kind = 0; kind < 3; ++kind) {
     ListArgs &A = pl->A[kind];
-    A.tile0 = kind == 0 ? 0 : (int)nkind[0];
+    A.tile0 = (int)(kind == 0 ? 0 : kind == 1 ? nkind[0] : nkind[0] + nkind[1]);
     A.ntiles = (int)nkind[kind];
     A.k = k;
     A.tiles = (const int2 *)(buf + off_tiles);
@@ -593,7 +636,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   int dev = 0, ncu = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  for (int kind = 0; kind < 2; ++kind) {
+  for (int kind = 0; kind < 3; ++kind) {
     long grid = std::min<long>((long)ncu * 4, (nkind[kind] + 3) / 4);
     pl->grid[kind] = (int)std::max<long>(grid, 1);
   }
@@ -607,7 +650,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
 
 int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
   if (pl.ntiles > 0) {
-    for (int kind = 0; kind < 2; ++kind) {
+    for (int kind = 0; kind < 3; ++kind) {
       ListArgs A = pl.A[kind];
       if (A.ntiles == 0) continue;
       A.amap = pl.amap;
@@ -626,6 +669,10 @@ int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
         if (kk <= 16) launch_list<16, 1>(A, g, s);
         else if (kk <= 32) launch_list<32, 1>(A, g, s);
         else launch_list<64, 1>(A, g, s);
+      } else if (kind == 2) {
+        if (kk <= 16) launch_list<16, 2>(A, g, s);
+        else if (kk <= 32) launch_list<32, 2>(A, g, s);
+        else launch_list<64, 2>(A, g, s);
       } else if (kk == 3) launch_list<3, 0>(A, g, s);
       else if (kk == 8) launch_list<8, 0>(A, g, s);
       else if (kk >= 9 && kk <= 16) launch_list<16, 0>(A, g, s);

@@ -196,13 +196,91 @@ def test_autotune_choices(L, torch_cuda):
         assert L.ks_list_autotune(t, k, 0, ctypes.byref(sl), ctypes.byref(tl)) == 0
         assert sl.value in (256, 512, 1024, 2048, 4096)
         assert tl.value in (0, 1)
-        if t == 2 or k < 9:
-            assert tl.value == 0  # Laplace and k <= 8 have no matrix-core tile
+        if k < 9:
+            assert tl.value == 0  # k <= 8 has no matrix-core tile
         sl2, tl2 = ctypes.c_int(), ctypes.c_int()
         assert L.ks_list_autotune(t, k, 0, ctypes.byref(sl2), ctypes.byref(tl2)) == 0
         assert (sl2.value, tl2.value) == (sl.value, tl.value)  # cached
         out.append((t, k, sl.value, tl.value))
     print("tuned shapes:", out)
+
+
+def test_laplace_matrix_core_tiles_exact_drops(tmp_path, torch_cuda, oracle):
+    """Laplace tasks at k = 32 on the matrix-core tiles (KS_AUTOTUNE=0 gives the
+    default matrix-core shape, in a process of its own) over points with exact
+    duplicates and near-coincident pairs whose d^2 straddles the reference's
+    1e-15 drop threshold (frame/dgsks_ref.c:343-345): the per-pair bound must
+    send those pairs through the reference's sequential sum, so every drop
+    decision and every kernel value of a kept near pair (up to ~1e225) is the
+    reference's, and the batch matches the oracle to 1e-12."""
+    rng = np.random.default_rng(2024)
+    k, pool = 32, 2400
+    X = rng.random((pool, k)) * (1.5 / np.sqrt(k))
+    # points 1200.. are copies of points 0.. moved by |delta| in 0 .. 6e-8
+    deltas = np.concatenate([[0.0] * 20, 10.0 ** rng.uniform(-12, -7.2, 180)])
+    for i, dl in enumerate(deltas):
+        d = rng.standard_normal(k)
+        X[1200 + i] = X[i] + dl * d / np.linalg.norm(d)
+    X2 = (X * X).sum(axis=1)
+    w = rng.standard_normal(pool)
+    ntask = 6
+    ms = rng.integers(150, 400, ntask)
+    ns = rng.integers(600, 1800, ntask)
+    near = np.arange(200)
+    amap = np.concatenate([np.concatenate([near, rng.choice(np.arange(200, pool), m - 200,
+                                                            replace=False)])
+                           for m in ms]).astype(np.int32)
+    bmap = np.concatenate([np.concatenate([near + 1200, near, rng.choice(np.arange(200, pool),
+                                                                         n - 400, replace=False)])
+                           for n in ns]).astype(np.int32)
+    a_off = np.concatenate([[0], np.cumsum(ms)]).astype(np.int64)
+    b_off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    u0 = rng.standard_normal(pool)
+    data = tmp_path / "batch.npz"
+    np.savez(data, X=X, X2=X2, w=w, amap=amap, bmap=bmap, a_off=a_off, b_off=b_off, u0=u0)
+    out = tmp_path / "u.npy"
+    code = f"""
+import ctypes, sys
+import numpy as np
+sys.path.insert(0, {ROOT!r})
+from ks_amd import _lib
+L = _lib.lib()
+d = np.load({str(data)!r})
+k, nt = {k}, {ntask}
+kern = (_lib.KsT * nt)()
+for t in range(nt):
+    kern[t].type = _lib.KS_LAPLACE
+sl, tl = ctypes.c_int(), ctypes.c_int()
+assert L.ks_list_autotune(_lib.KS_LAPLACE, k, 0, ctypes.byref(sl), ctypes.byref(tl)) == 0
+assert tl.value == 1, "the default shape is the matrix-core tile"
+p = lambda a: ctypes.c_void_p(a.ctypes.data)
+u = d["u0"].copy()
+arr = {{n: np.ascontiguousarray(d[n]) for n in d.files}}
+rc = L.dgsks_list(nt, kern, k, p(u), u.size, p(arr["a_off"]), p(arr["amap"]), p(arr["amap"]),
+                  p(arr["b_off"]), p(arr["bmap"]), p(arr["bmap"]), p(arr["X"]), p(arr["X2"]),
+                  p(arr["X"]), p(arr["X2"]), p(arr["w"]), 0)
+assert rc == 0, rc
+np.save({str(out)!r}, u)
+"""
+    env = dict(os.environ, KS_AUTOTUNE="0")
+    env.pop("KS_TUNE_CACHE", None)
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    got = np.load(out)
+    ref = u0.copy()
+    oracle.dgsks_list([dict(type=2)] * ntask, k, ref, a_off, amap, amap, b_off, bmap, bmap, X, X2,
+                      X, X2, w)
+    # scaled by the largest entry first: kept near pairs reach ~1e227, whose
+    # squares would overflow
+    sc = float(np.max(np.abs(ref)))
+    assert np.all(np.isfinite(got))
+    err = oracle.rel_l2(got / sc, ref / sc)
+    big = np.abs(ref) > 1e100
+    err_big = oracle.rel_l2(got[big] / sc, ref[big] / sc)
+    print(f"Laplace matrix-core tiles, near-coincident pairs: rel L2 = {err:.3e}; "
+          f"{int(big.sum())} targets carry a kept near pair (rel L2 {err_big:.3e})")
+    assert err <= TOL and err_big <= TOL
 
 
 def test_autotune_cache_file(tmp_path, torch_cuda):
