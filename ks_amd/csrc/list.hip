@@ -199,8 +199,33 @@ __device__ double tile_valu_lds(const ListArgs &L, const DevTask &T, int pos, in
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int r = 0;
+    if constexpr (TYPE == LAPLACE && K >= 8) {
+      // four sources at a time: their kernel values share one power ladder
+      // (k = 3 has none: d^-1), and acc still takes them in source order
 #pragma unroll 1
-    for (int r = 0; r < nch; ++r) {
+      for (; r + 4 <= nch; r += 4) {
+        double d2[4], wq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double *b = stage + (r + q) * K;
+          double c;
+          {
+#pragma clang fp contract(off)
+            c = a[0] * b[0];
+#pragma unroll
+            for (int p = 1; p < K; ++p) c = c + a[p] * b[p];
+          }
+          d2[q] = fma(-2.0, c, aa) + side[3 * (r + q)];  // the assembly of kval_f64
+          wq[q] = side[3 * (r + q) + 1];
+        }
+        laplace_n_f64<4>(d2, P);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc = fma(d2[q], wq[q], acc);
+      }
+    }
+#pragma unroll 1
+    for (; r < nch; ++r) {
       const double bb = side[3 * r], wj = side[3 * r + 1], hj = side[3 * r + 2];
       const double *b = stage + r * K;
       double c;
@@ -397,8 +422,13 @@ __device__ double run_tile(const ListArgs &L, const DevTask &T, int row0, int c0
 // tiles and the Laplace matrix-core tiles (whose exact-recompute path needs more
 // registers) of a batch run as three launches, so that each kernel's register
 // allocation is that of its own tiles.
+// Row tiles at k <= 8 wait on their gathered source chunks: 4 waves per SIMD
+// (128 VGPRs, a few spilled at k = 8) hide more of it than 3 without spills
+// (list_pool k = 8: 27.2 -> 24.9 ms); 5 at k = 3 (96 VGPRs, 33 spilled) lost
+// (17.1 -> 17.6 ms).
 template <int K, int MF>
-__global__ __launch_bounds__(256) void list_kernel(ListArgs L) {
+__global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void list_kernel(
+    ListArgs L) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   __shared__ double s_red[4][64];
