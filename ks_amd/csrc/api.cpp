@@ -269,7 +269,7 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
       return e == hipSuccess ? KS_OK : hip_failure(e, "symmetric launch");
     }
   }
-  const bool mfma = mfma_f64_eligible(p.type, p.k);
+  const bool mfma = fused_mfma_eligible(p.type, p.k);
   FusedPlan plan = mfma ? plan_split(p.m, p.n, mfma_rows_per_wg(), kTargetWGs, kMinColsMfma)
                         : plan_fused(p.m, p.n, p.k, kTargetWGs, kMinCols);
   double *partial = nullptr;

@@ -20,8 +20,15 @@
 //     the 16 lanes of a row group are summed once at the end;
 //   * the split-source reduction is the one of fused_f64.hip (partials + the
 //     last workgroup of a row block adds them in split order).
-// Laplace is not routed here: its d^2 < 1e-15 test needs the reference's exact
-// rank-k sum (see fused_f64.hip).
+// Laplace: the reference drops pairs with d^2 < 1e-15 (frame/dgsks_ref.c:343-345),
+// so a near pair's d^2 must be the reference's, bit for bit.  The matrix-core
+// and the reference's sequential sums differ by at most
+// (2 (k + 2) + 4) 2^-53 (|a|^2 + |b|^2); a pair whose d^2 could lie on the other
+// side of the threshold, or whose power d^(2-k) could turn that bound into more
+// than 1e-11 relative, is recomputed with the reference's sequential non-FMA
+// sum (then d^2, the drop and K are the reference's).  On spread points only
+// coincident pairs take that path.  The symmetric and multi-right-hand-side
+// matrix-core kernels keep Laplace on the vector units.
 #include <hip/hip_runtime.h>
 
 #include "gsks_kernels.h"
@@ -86,7 +93,10 @@ __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
     ww = iw >= 0 ? p.w[iw] : 0.0;
     hh = (TYPE == VAR_BANDWIDTH) ? p.hj[ib] : 0.0;
   };
-  auto block = [&](const double (&f)[KC][4], double bb, double wj, double hj) {
+  // Laplace: exact-recompute threshold on d^2, as a multiple of |a|^2 + |b|^2
+  const double lap_b = (2.0 * (k + 2) + 4.0) * 0x1p-53;
+  const double lap_t1 = lap_b * (1.0 + fmax(0.5 * k - 1.0, 1.0) * 1e11) + lap_b;
+  auto block = [&](const double (&f)[KC][4], double bb, double wj, double hj, int ib) {
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
       d4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -96,27 +106,52 @@ __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
         for (int t = 0; t < 4; ++t)
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[rb][c][t], f[c][t], acc, 0, 0, 0);
       }
+      if constexpr (TYPE == LAPLACE) {
+        double d2[4];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        double K = kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P);
-        racc[rb][v] = fma(K, wj, racc[rb][v]);
+        for (int v = 0; v < 4; ++v) {
+          d2[v] = fma(-2.0, acc[v], aa[rb][v]) + bb;  // the assembly of kval_f64
+          if (d2[v] < 1e-15 + lap_t1 * (aa[rb][v] + bb)) {
+            const int ro = row0 + rb * 16 + g + 4 * v;
+            const int ao = p.amap[ro < m ? ro : m - 1];
+            const double c = dot_exact_n(p.XA + (long)ao * k, p.XB + (long)ib * k, k);
+            d2[v] = fma(-2.0, c, aa[rb][v]) + bb;
+          }
+        }
+        laplace_n_f64<4>(d2, P);  // the 4 kernel values, one power ladder
+#pragma unroll
+        for (int v = 0; v < 4; ++v) racc[rb][v] = fma(d2[v], wj, racc[rb][v]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double K = kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P);
+          racc[rb][v] = fma(K, wj, racc[rb][v]);
+        }
       }
     }
   };
+  int sb0 = 0, sb1 = 0;  // the source rows of the blocks held in bf0 / bf1
   if (j0 < j1) {
     fetch_idx(j0, ib0, iw0);
     if (j0 + 16 < j1) fetch_idx(j0 + 16, ib1, iw1);
     fetch(ib0, iw0, bf0, bb0, wj0, hj0);
+    sb0 = ib0;
   }
   for (int jb = j0; jb < j1; jb += 32) {
     // here bf0 holds block jb and (ib1, iw1) the map entries of block jb + 16
-    if (jb + 16 < j1) fetch(ib1, iw1, bf1, bb1, wj1, hj1);
+    if (jb + 16 < j1) {
+      fetch(ib1, iw1, bf1, bb1, wj1, hj1);
+      sb1 = ib1;
+    }
     if (jb + 32 < j1) fetch_idx(jb + 32, ib0, iw0);
-    block(bf0, bb0, wj0, hj0);
+    block(bf0, bb0, wj0, hj0, sb0);
     if (jb + 16 >= j1) break;
-    if (jb + 32 < j1) fetch(ib0, iw0, bf0, bb0, wj0, hj0);
+    if (jb + 32 < j1) {
+      fetch(ib0, iw0, bf0, bb0, wj0, hj0);
+      sb0 = ib0;
+    }
     if (jb + 48 < j1) fetch_idx(jb + 48, ib1, iw1);
-    block(bf1, bb1, wj1, hj1);
+    block(bf1, bb1, wj1, hj1, sb1);
   }
 
   // ---- sum the 16 lanes (source columns) of each row group
@@ -395,6 +430,8 @@ constexpr int kMfmaRB = 2;
 
 bool mfma_f64_eligible(int type, int k) { return type != LAPLACE && k >= 9 && k <= 64; }
 
+bool fused_mfma_eligible(int type, int k) { return k >= 9 && k <= 64; }
+
 int mfma_rows_per_wg() { return 4 * kMfmaRB * 16; }
 
 template <int TYPE, int KC>
@@ -436,6 +473,7 @@ hipError_t launch_fused_f64_mfma(const Problem &p, const FusedPlan &plan, double
     case QUARTIC: return go_type<QUARTIC>(A, plan.row_blocks, s);
     case MULTIQUADRATIC: return go_type<MULTIQUADRATIC>(A, plan.row_blocks, s);
     case EPANECHNIKOV: return go_type<EPANECHNIKOV>(A, plan.row_blocks, s);
+    case LAPLACE: return go_type<LAPLACE>(A, plan.row_blocks, s);
     default: return hipErrorInvalidValue;
   }
 }

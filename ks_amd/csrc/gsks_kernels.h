@@ -135,6 +135,15 @@ __device__ __forceinline__ cptr<T> to_const(const T *p) {
   return (cptr<T>)(p);
 }
 
+// The reference's sequential rank-k sum for a runtime k through plain pointers
+// (rare exact recomputes inside matrix-core kernels).
+__device__ __forceinline__ double dot_exact_n(const double *a, const double *b, int k) {
+#pragma clang fp contract(off)
+  double c = a[0] * b[0];
+  for (int p = 1; p < k; ++p) c = c + a[p] * b[p];
+  return c;
+}
+
 // Rank-k sum taken exactly as the reference's loops do (a rounded product added
 // to a rounded running sum, p = 0..k-1, no FMA).  Used where a threshold on d^2
 // makes the result discontinuous (Laplace: d^2 < 1e-15 drops the pair,

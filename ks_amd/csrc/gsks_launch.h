@@ -53,6 +53,8 @@ hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *par
 
 // FP64 cross term on the matrix cores (fused_f64_mfma.hip)
 bool mfma_f64_eligible(int type, int k);
+// The single-summation matrix-core kernel: every type, Laplace included.
+bool fused_mfma_eligible(int type, int k);
 int mfma_rows_per_wg();
 // ... and its symmetric form (row blocks of sym_mfma_rows() targets)
 bool sym_mfma_eligible(const Problem &p);

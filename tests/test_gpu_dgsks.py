@@ -351,6 +351,47 @@ def test_error_paths_leave_u_untouched(G, torch_cuda):
     assert L.ks_last_error() == _lib.KS_ERR_NO_BANDWIDTH and np.all(u == 1)
 
 
+@pytest.mark.parametrize("k", [12, 32, 64])
+def test_laplace_matrix_cores_near_coincident_pairs(G, oracle, torch_cuda, k):
+    """Laplace at 9 <= k <= 64 runs on the FP64 matrix cores.  Exact duplicates
+    and near-coincident pairs whose d^2 straddles the reference's 1e-15 drop
+    threshold (frame/dgsks_ref.c:343-345) must be recomputed with the
+    reference's sequential sum, so every drop decision and every kernel value
+    of a kept near pair (up to ~1e100 and beyond) is the reference's.  Compared
+    after scaling by the largest entry, whose square would overflow."""
+    torch = torch_cuda
+    rng = np.random.default_rng(3000 + k)
+    nx, m, n = 2400, 900, 1500
+    X = rng.random((nx, k)) * (1.5 / np.sqrt(k))
+    deltas = np.concatenate([[0.0] * 20, 10.0 ** rng.uniform(-12, -7.2, 180)])
+    for i, dl in enumerate(deltas):
+        d = rng.standard_normal(k)
+        X[1200 + i] = X[i] + dl * d / np.linalg.norm(d)
+    X2 = oracle.sqnorm(X)
+    near = np.arange(200)
+    amap = np.concatenate([near, rng.choice(np.arange(200, nx), m - 200, replace=False)])
+    bmap = np.concatenate([near + 1200, near, rng.choice(np.arange(200, nx), n - 400,
+                                                        replace=False)])
+    amap, bmap = amap.astype(np.int32), bmap.astype(np.int32)
+    umap = rng.permutation(nx)[:m].astype(np.int32)
+    wmap = rng.permutation(nx)[:n].astype(np.int32)
+    w = rng.standard_normal(nx)
+    u0 = rng.standard_normal(nx)
+    dX, dX2, damap, dumap, dbmap, dwmap, dw, du = _dev(torch, X, X2, amap, umap, bmap, wmap, w, u0)
+    G.dgsks(k, du, dumap, dX, dX2, damap, dX, dX2, dbmap, dw, dwmap, **_opts(2))
+    torch.cuda.synchronize()
+    got = du.cpu().numpy()[umap]
+    ref = oracle.dgsks_rows(2, X, X2, amap, X, X2, bmap, w, wmap, u0, umap)
+    # at k = 64 a kept pair's d^-62 can overflow: those rows must overflow alike
+    fin = np.isfinite(ref)
+    assert np.array_equal(got[~fin], ref[~fin], equal_nan=True)
+    sc = float(np.max(np.abs(ref[fin])))
+    err = oracle.rel_l2(got[fin] / sc, ref[fin] / sc)
+    big = int(np.sum(np.abs(ref[fin]) > 1e50))
+    print(f"Laplace k={k} near-coincident pairs: rel L2 {err:.3e}, {big} rows with a kept near pair")
+    assert err <= TOL
+
+
 def test_laplace_descriptor_side_effect(G):
     from ks_amd import _lib
     L = _lib.lib()
