@@ -215,7 +215,13 @@ hipError_t launch_fused_f64(const Problem &p, const FusedPlan &plan, double *par
   A.vec2 = 0;
   switch (p.type) {
     case GAUSSIAN: return launch_type<GAUSSIAN>(A, plan, s);
-    case POLYNOMIAL: return launch_type<POLYNOMIAL>(A, plan, s);
+    case POLYNOMIAL:
+      switch (poly_type(p.P)) {  // the power fixed at compile time where it can be
+        case POLY_SQUARE_T: return launch_type<POLY_SQUARE_T>(A, plan, s);
+        case POLY_CUBE_T: return launch_type<POLY_CUBE_T>(A, plan, s);
+        case POLY_FOURTH_T: return launch_type<POLY_FOURTH_T>(A, plan, s);
+        default: return launch_type<POLYNOMIAL>(A, plan, s);
+      }
     case LAPLACE: return launch_type<LAPLACE>(A, plan, s);
     case VAR_BANDWIDTH: return launch_type<VAR_BANDWIDTH>(A, plan, s);
     case TANH: return launch_type<TANH>(A, plan, s);

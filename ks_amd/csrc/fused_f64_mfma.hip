@@ -467,7 +467,13 @@ hipError_t launch_fused_f64_mfma(const Problem &p, const FusedPlan &plan, double
   A.vec2 = 0;
   switch (p.type) {
     case GAUSSIAN: return go_type<GAUSSIAN>(A, plan.row_blocks, s);
-    case POLYNOMIAL: return go_type<POLYNOMIAL>(A, plan.row_blocks, s);
+    case POLYNOMIAL:
+      switch (poly_type(p.P)) {  // the power fixed at compile time where it can be
+        case POLY_SQUARE_T: return go_type<POLY_SQUARE_T>(A, plan.row_blocks, s);
+        case POLY_CUBE_T: return go_type<POLY_CUBE_T>(A, plan.row_blocks, s);
+        case POLY_FOURTH_T: return go_type<POLY_FOURTH_T>(A, plan.row_blocks, s);
+        default: return go_type<POLYNOMIAL>(A, plan.row_blocks, s);
+      }
     case VAR_BANDWIDTH: return go_type<VAR_BANDWIDTH>(A, plan.row_blocks, s);
     case TANH: return go_type<TANH>(A, plan.row_blocks, s);
     case QUARTIC: return go_type<QUARTIC>(A, plan.row_blocks, s);

@@ -270,7 +270,14 @@ hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *par
   hipError_t e;
   switch (p.type) {
     case GAUSSIAN: e = sym_k<GAUSSIAN>(A, pl, s); break;
-    case POLYNOMIAL: e = sym_k<POLYNOMIAL>(A, pl, s); break;
+    case POLYNOMIAL:
+      switch (poly_type(p.P)) {  // the power fixed at compile time where it can be
+        case POLY_SQUARE_T: e = sym_k<POLY_SQUARE_T>(A, pl, s); break;
+        case POLY_CUBE_T: e = sym_k<POLY_CUBE_T>(A, pl, s); break;
+        case POLY_FOURTH_T: e = sym_k<POLY_FOURTH_T>(A, pl, s); break;
+        default: e = sym_k<POLYNOMIAL>(A, pl, s); break;
+      }
+      break;
     case LAPLACE: e = sym_k<LAPLACE>(A, pl, s); break;
     case VAR_BANDWIDTH: e = sym_k<VAR_BANDWIDTH>(A, pl, s); break;
     case TANH: e = sym_k<TANH>(A, pl, s); break;

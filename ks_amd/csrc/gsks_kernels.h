@@ -41,6 +41,19 @@ struct KerParams {
   int pmode;  // PowMode
 };
 
+// Internal kernel types: the polynomial with its power fixed at compile time,
+// so that a kernel's inner loop carries no per-pair branch on the power mode.
+// Same products as powe_f64 (x^2, (x^2) x, (x^2)^2).
+constexpr int POLY_SQUARE_T = 8;
+constexpr int POLY_CUBE_T = 9;
+constexpr int POLY_FOURTH_T = 10;
+__host__ __device__ inline int poly_type(const KerParams &P) {
+  if (P.pmode == POW_SQUARE || (P.pmode == POW_INT && P.ipow == 2)) return POLY_SQUARE_T;
+  if (P.pmode == POW_INT && P.ipow == 3) return POLY_CUBE_T;
+  if (P.pmode == POW_FOURTH || (P.pmode == POW_INT && P.ipow == 4)) return POLY_FOURTH_T;
+  return POLYNOMIAL;
+}
+
 // One summation, all pointers on the device.
 struct Problem {
   int m, n, k;
@@ -197,6 +210,16 @@ __device__ __forceinline__ double kval_f64(double c, double aa, double bb, doubl
                                            double hj, const KerParams &P) {
   if constexpr (TYPE == POLYNOMIAL) {
     return powe_f64(c * P.scal + P.cons, P);
+  } else if constexpr (TYPE == POLY_SQUARE_T) {
+    const double x = c * P.scal + P.cons;
+    return x * x;
+  } else if constexpr (TYPE == POLY_CUBE_T) {
+    const double x = c * P.scal + P.cons;
+    return (x * x) * x;
+  } else if constexpr (TYPE == POLY_FOURTH_T) {
+    const double x = c * P.scal + P.cons;
+    const double t = x * x;
+    return t * t;
   } else if constexpr (TYPE == TANH) {
     return tanh(c * P.scal + P.cons);
   } else {

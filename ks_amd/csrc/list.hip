@@ -74,7 +74,8 @@ struct ListArgs {
   unsigned *rbcount;   // [nrowblocks], zero between launches
   unsigned *head;      // tile counter
   unsigned *done;      // waves that have left
-  double *R;           // [total_a] per-(task, row) totals
+  double *R;           // [total_a + 64] per-(task, row) totals, then 64 sink entries
+  long rsink;          // total_a: where the lanes past a row block's end write
 };
 
 // ------------------------------------------------------------------ per-tile work
@@ -436,10 +437,17 @@ __global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void lis
   constexpr int NST = (K > 0 && MF == 0) ? stage_doubles(K > 0 ? K : 1) + kStageSide : 1;
   __shared__ double s_stage[4][NST];
   double *stage = s_stage[wave];
+  // The loop body has no lane-dependent branch outside the tile itself and no
+  // `continue`: its wave-wide values (the tile index, the split count) come
+  // from lane 0 through readfirstlane, which is only right while every lane
+  // runs every iteration.  With lane-0-only branches and early `continue`s
+  // here, the compiled loop let lanes 1..63 start an iteration without lane 0
+  // after a row block's last tile (tiles rerun, counts read from an inactive
+  // lane: wrong sums that depended on which wave took which tile).
   while (true) {
-    unsigned t = 0;
-    if (lane == 0) t = atomicAdd(L.head, 1u);
-    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+    // one atomic per wave: lane 0 adds 1, the other lanes 0
+    unsigned t = (unsigned)__builtin_amdgcn_readfirstlane(
+        (int)atomicAdd(L.head, lane == 0 ? 1u : 0u));
     if (t >= (unsigned)L.ntiles) break;
     t += (unsigned)L.tile0;
     const int2 tl = L.tiles[t];
@@ -452,7 +460,24 @@ __global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void lis
       s = run_tile<LAPLACE, K, MF>(L, T, rb.row0, c0, c1, red, stage);
     } else switch (T.type) {
       case GAUSSIAN: s = run_tile<GAUSSIAN, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
-      case POLYNOMIAL: s = run_tile<POLYNOMIAL, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+      case POLYNOMIAL:
+        if constexpr (MF == 0) {  // row tiles: the power fixed at compile time where it can be
+          switch (poly_type(T.P)) {
+            case POLY_SQUARE_T:
+              s = run_tile<POLY_SQUARE_T, K, MF>(L, T, rb.row0, c0, c1, red, stage);
+              break;
+            case POLY_CUBE_T:
+              s = run_tile<POLY_CUBE_T, K, MF>(L, T, rb.row0, c0, c1, red, stage);
+              break;
+            case POLY_FOURTH_T:
+              s = run_tile<POLY_FOURTH_T, K, MF>(L, T, rb.row0, c0, c1, red, stage);
+              break;
+            default: s = run_tile<POLYNOMIAL, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+          }
+        } else {
+          s = run_tile<POLYNOMIAL, K, MF>(L, T, rb.row0, c0, c1, red, stage);
+        }
+        break;
       case LAPLACE:  // row tiles only: Laplace matrix-core tiles are the MF = 2 launch
         s = MF == 0 ? run_tile<LAPLACE, K, 0>(L, T, rb.row0, c0, c1, red, stage) : 0.0;
         break;
@@ -462,23 +487,25 @@ __global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void lis
       case MULTIQUADRATIC: s = run_tile<MULTIQUADRATIC, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
       default: s = run_tile<EPANECHNIKOV, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
     }
-    const int pos = rb.row0 + lane;
-    const bool valid = pos < T.m;
-    if (rb.nsplit == 1) {
-      if (valid) L.R[T.a0 + pos] = s;
-      continue;
+    bool last = rb.nsplit == 1;  // wave-uniform
+    if (!last) {
+      L.partial[(long)t * 64 + lane] = s;
+      __threadfence();
+      const unsigned prev = (unsigned)__builtin_amdgcn_readfirstlane(
+          (int)atomicAdd(&L.rbcount[tl.x], lane == 0 ? 1u : 0u));
+      last = prev == (unsigned)(rb.nsplit - 1);
+      if (last) {
+        __threadfence();
+        double tot = 0.0;
+        for (int sp = 0; sp < rb.nsplit; ++sp)
+          tot += L.partial[(long)(rb.tile_base + sp) * 64 + lane];
+        s = tot;
+        L.rbcount[tl.x] = 0u;  // every lane stores the same zero
+      }
     }
-    L.partial[(long)t * 64 + lane] = s;
-    __threadfence();
-    unsigned prev = 0;
-    if (lane == 0) prev = atomicAdd(&L.rbcount[tl.x], 1u);
-    prev = __builtin_amdgcn_readfirstlane(__shfl(prev, 0));
-    if (prev != (unsigned)(rb.nsplit - 1)) continue;
-    __threadfence();
-    double tot = 0.0;
-    for (int sp = 0; sp < rb.nsplit; ++sp) tot += L.partial[(long)(rb.tile_base + sp) * 64 + lane];
-    if (valid) L.R[T.a0 + pos] = tot;
-    if (lane == 0) L.rbcount[tl.x] = 0u;
+    // lanes past the block's last row write the sink entries R[rsink + lane]
+    const int pos = rb.row0 + lane;
+    if (last) L.R[pos < T.m ? T.a0 + pos : L.rsink + lane] = s;
   }
   // the last wave to leave resets the tile counter for the next launch
   if (lane == 0) {
@@ -613,7 +640,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   size_t off_cnt = off_part + al((size_t)pl->ntiles * 64 * sizeof(double));
   size_t off_head = off_cnt + al(rbs.size() * sizeof(unsigned));
   size_t off_R = off_head + 256;  // three launches' {head, done} counters
-  size_t off_iota = off_R + al((size_t)total_a * sizeof(double));
+  size_t off_iota = off_R + al((size_t)(total_a + 64) * sizeof(double));  // + 64 sink entries
   size_t off_kout = off_iota + al((size_t)total_a * sizeof(int));
   size_t off_vout = off_kout + al((size_t)total_a * sizeof(int));
   size_t off_sort = off_vout + al((size_t)total_a * sizeof(int));
@@ -646,6 +673,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
     A.head = (unsigned *)(buf + off_head) + 2 * kind;
     A.done = A.head + 1;
     A.R = (double *)(buf + off_R);
+    A.rsink = total_a;
   }
   // merge order: target ids stably sorted with their positions
   int *iota = (int *)(buf + off_iota);

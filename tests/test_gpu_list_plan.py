@@ -283,6 +283,67 @@ np.save({str(out)!r}, u)
     assert err <= TOL and err_big <= TOL
 
 
+@pytest.mark.parametrize("gauss,lap,poly", [(512, 1024, 2048), (256, 4096, 512),
+                                            (4096, 2048, 256)])
+def test_mixed_slices_per_kernel_type(tmp_path, torch_cuda, oracle, gauss, lap, poly):
+    """Regression: tasks of different kernel types with different tuned slice
+    widths (forced through a KS_TUNE_CACHE file, in a process of its own) put
+    split and unsplit row blocks of every size next to each other in the tile
+    order.  With lane-0-only branches in the persistent loop, these layouts
+    once gave wrong sums that changed from run to run (lanes 1..63 of a wave
+    started an iteration without lane 0 after a row block's last tile).  Every
+    run must match the oracle, and repeated plan runs must agree bit for bit."""
+    cache = tmp_path / "tiles.txt"
+    cache.write_text(f"0 3 0 {gauss} 0\n2 3 0 {lap} 0\n1 3 0 {poly} 0\n")
+    out = tmp_path / "u.npz"
+    code = f"""
+import ctypes, sys
+import numpy as np
+import torch
+sys.path.insert(0, {ROOT!r}); sys.path.insert(0, {HERE!r})
+from ks_amd import _lib
+import test_gpu_list_plan as T
+L = _lib.lib()
+sl, tl = ctypes.c_int(), ctypes.c_int()
+for t, want in ((0, {gauss}), (2, {lap}), (1, {poly})):
+    assert L.ks_list_autotune(t, 3, 0, ctypes.byref(sl), ctypes.byref(tl)) == 0
+    assert (sl.value, tl.value) == (want, 0), (t, sl.value, tl.value)
+B = T._batch(3, 403)
+u0 = np.random.default_rng(1).standard_normal(B["pool"])
+p = lambda a: ctypes.c_void_p(a.ctypes.data)
+outs = {{}}
+for rep in range(2):
+    u = u0.copy()
+    rc = L.dgsks_list(B["ntask"], B["kernels"], 3, p(u), u.size, p(B["a_off"]), p(B["amap"]),
+                      p(B["amap"]), p(B["b_off"]), p(B["bmap"]), p(B["bmap"]), p(B["X"]),
+                      p(B["X2"]), p(B["X"]), p(B["X2"]), p(B["w"]), 0)
+    assert rc == 0, rc
+    outs[f"oneshot{{rep}}"] = u
+plan = T._Plan(L, torch, B)
+for rep in range(3):
+    u = torch.from_numpy(u0.copy()).cuda()
+    assert plan.run(u, torch.cuda.current_stream()) == 0
+    torch.cuda.synchronize()
+    outs[f"plan{{rep}}"] = u.cpu().numpy()
+plan.close()
+np.savez({str(out)!r}, u0=u0, **outs)
+"""
+    env = dict(os.environ, KS_TUNE_CACHE=str(cache))
+    env.pop("KS_AUTOTUNE", None)
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    d = np.load(out)
+    B = _batch(3, 403)
+    ref = _oracle_list(oracle, B, d["u0"])
+    runs = [n for n in d.files if n != "u0"]
+    errs = {n: oracle.rel_l2(d[n], ref) for n in runs}
+    print(f"slices gauss {gauss} laplace {lap} poly {poly}: " +
+          " ".join(f"{n} {e:.2e}" for n, e in errs.items()))
+    assert max(errs.values()) <= TOL
+    assert all(np.array_equal(d["plan0"], d[n]) for n in runs)
+
+
 def test_autotune_cache_file(tmp_path, torch_cuda):
     """KS_TUNE_CACHE: a fresh process tunes and appends its choice; a second
     process reads it back without tuning (KS_AUTOTUNE=0 would otherwise give the
