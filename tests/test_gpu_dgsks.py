@@ -179,6 +179,45 @@ def test_gaussian_exp_ulps(G, torch_cuda, k):
     assert np.abs(got - want)[~normal].max() <= 5e-324
 
 
+@pytest.mark.parametrize("k", [3, 16])
+def test_tanh_ulps(G, torch_cuda, k):
+    """The FP64 kernels' tanh (-e / (2 + e), e = expm1(-2|x|), sign restored)
+    against numpy's over both signs, tiny to saturating arguments and the
+    special values: one source e_1 with weight 1 and targets (x_i, 0, ...) with
+    scal = 1, cons = 0, so c = x_i exactly and u_i = tanh(x_i).  k = 3 runs the
+    vector kernel, k = 16 the matrix-core one.  Bar: 4 ulps (3 in a CPU
+    restatement with libm's expm1), 0 and +-1 exact, NaN kept (u starts at 0,
+    so tanh(-0) arrives as +0)."""
+    torch = torch_cuda
+    rng = np.random.default_rng(21 + k)
+    x = np.concatenate([[0.0, -0.0, 5e-324, 1e-300, 1e-17, 1e-8, 0.25, 0.5493, 0.55, 1.0, 10.0,
+                         19.0, 20.0, 40.0, 1e300, np.inf, -np.inf, np.nan],
+                        10 ** rng.uniform(-300, -8, 2000), 10 ** rng.uniform(-8, 0, 4000),
+                        rng.uniform(0, 1.2, 4000), rng.uniform(0.5, 25, 4000)])
+    x = np.concatenate([x, -x[2:]])
+    m = x.size
+    XA = np.zeros((m, k))
+    XA[:, 0] = x
+    XB = np.zeros((1, k))
+    XB[0, 0] = 1.0
+    dXA, dXA2, dXB, dXB2 = _dev(torch, XA, np.zeros(m), XB, np.ones(1))
+    idx = np.arange(m, dtype=np.int32)
+    dm, dz, dw = _dev(torch, idx, np.zeros(1, dtype=np.int32), np.ones(1))
+    du = torch.zeros(m, dtype=torch.float64, device="cuda")
+    G.dgsks(k, du, dm, dXA, dXA2, dm, dXB, dXB2, dz, dw, dz, type="Tanh", scal=1.0, cons=0.0)
+    torch.cuda.synchronize()
+    got = du.cpu().numpy()
+    want = np.tanh(x)
+    nan = np.isnan(want)
+    assert np.all(np.isnan(got[nan]))
+    exact = (want == 0) | (np.abs(want) == 1)
+    assert np.array_equal(got[exact], want[exact])
+    rest = ~nan & ~exact
+    err = np.abs(got[rest] - want[rest]) / np.spacing(np.abs(want[rest]))
+    print(f"k={k}: tanh max ulp error {err.max():.2f}, mean {err.mean():.3f}")
+    assert err.max() <= 4.0
+
+
 @pytest.mark.parametrize("k", [3, 16, 64])
 def test_clamp_semantics_with_inconsistent_norms(G, oracle, torch_cuda, k):
     """Norms lowered by 1e-7 (relative) and every target also a source: d^2 of
