@@ -84,6 +84,7 @@ struct MixPack {
   double4 *Bd;      // FP64 source operand [np/16][2 nc][64 lanes]
   float2 *ra;       // per target: {-|a|^2 / 2, -hi log2(e) / 2 (VarBW)}
   float4 *cb;       // per source: {xs |b|^2 (Gaussian) or |b|^2, hj, w, |b| (Gaussian)}
+  float2 *cs;       // per source, the skip test's two factors: {xs |b|^2, |b|} (Gaussian)
   double2 *rd;      // per target: {|a|^2, -hi / 2}
   double4 *cd;      // per source: {|b|^2, w, hj, 0}
   const int *prow;  // sorted target position -> target index of the call
@@ -227,11 +228,13 @@ __global__ void mixed_pack_kernel(const Problem p, int nc, float xs, long mp, lo
       const float nb = (float)sqrt(bb);
       if (TYPE == GAUSSIAN) pk.cb[j] = make_float4((float)(bb * (double)xs), 0.0f, (float)w, nb);
       else pk.cb[j] = make_float4((float)bb, (float)hj, (float)w, nb);
+      pk.cs[j] = make_float2((float)(bb * (double)xs), nb);
       pk.cd[j] = make_double4(bb, w, hj, 0.0);
     } else {  // padding source: exponent -inf, weight 0
       pack_point<PREC>(nullptr, p.k, nc, j, pk.Bp, pk.Bd);
       pk.cb[j] = (TYPE == GAUSSIAN) ? make_float4(-INFINITY, 0.0f, 0.0f, 0.0f)
                                     : make_float4(INFINITY, 1.0f, 0.0f, 0.0f);
+      pk.cs[j] = make_float2(-INFINITY, 0.0f);
       pk.cd[j] = make_double4(0.0, 0.0, 1.0, 0.0);
     }
   }
@@ -428,11 +431,13 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
     c4 = pk.cb[jb + p16];
   };
   uint4 blo[NC];     // lo halves of the source block lo_block (BF16x3)
+  float4 bcb;        // and its sources' records {xs |b|^2, 0, w, |b|}
   int lo_block = -1;
   auto fetch_lo = [&](int jb) {
     const uint4 *blk = pk.Bp + (long)(jb >> 4) * NC * 128 + lane;
 #pragma unroll
     for (int c = 0; c < NC; ++c) blo[c] = blk[(c * 2 + 1) * 64];
+    bcb = pk.cb[jb + p16];
     lo_block = jb;
   };
 
@@ -500,15 +505,18 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
     // are independent and issue back to back, their skip tests are four
     // instructions each, and one ballot lets the usual all-skipped step
     // through.  The hi halves of the next two blocks load meanwhile.
+    // a step loads the hi halves and the skip test's two factors per source
+    // (8 bytes, not the 16-byte record with the weight, which only blocks that
+    // pass need: it comes with their lo halves)
     struct HB {
       uint4 h[NC];
-      float4 cb;
+      float2 cs;
     };
     auto fetch_h = [&](int jb, HB &b) {
       const uint4 *blk = pk.Bp + (long)(jb >> 4) * NC * 128 + lane;
 #pragma unroll
       for (int c = 0; c < NC; ++c) b.h[c] = blk[(c * 2) * 64];
-      b.cb = pk.cb[jb + p16];
+      b.cs = pk.cs[jb + p16];
     };
     auto est = [&](int rb, const HB &b) {
       f4 e = cinit[rb];
@@ -524,7 +532,7 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
     const bool skip_on = xs < 0.0f;
     const float m2s = m2xs * kShrink;
     auto tval = [&](const HB &b) {
-      return fmaf(b.cb.x, kShrink, fmaf(fmaf(mk2, b.cb.w, mk1), b.cb.w, mk0));
+      return fmaf(b.cs.x, kShrink, fmaf(fmaf(mk2, b.cs.y, mk1), b.cs.y, mk0));
     };
     auto hot = [&](const f4 &e, float t) {
       const float mx = fmaxf(fmaxf(e[0], e[1]), fmaxf(e[2], e[3]));
@@ -551,7 +559,7 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
           c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
           c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
         }
-        evaluate(rb, jb, c, b.cb, f64, used64);
+        evaluate(rb, jb, c, bcb, f64, used64);
       }
       // blocks that pass come in runs once the points are bucketed: after one,
       // start the next block's lo loads
@@ -773,6 +781,7 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   const size_t o_Bd = off; off += al(np * nc * 256);
   const size_t o_ra = off; off += al(mp * sizeof(float2));
   const size_t o_cb = off; off += al(np * sizeof(float4));
+  const size_t o_cs = off; off += al(np * sizeof(float2));
   const size_t o_rd = off; off += al(mp * sizeof(double2));
   const size_t o_cd = off; off += al(np * sizeof(double4));
   const size_t o_prow = off; off += al((size_t)p.m * sizeof(int));
@@ -796,6 +805,7 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   pk.Bd = (double4 *)(base + o_Bd);
   pk.ra = (float2 *)(base + o_ra);
   pk.cb = (float4 *)(base + o_cb);
+  pk.cs = (float2 *)(base + o_cs);
   pk.rd = (double2 *)(base + o_rd);
   pk.cd = (double4 *)(base + o_cd);
   int *prow = (int *)(base + o_prow);

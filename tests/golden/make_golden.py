@@ -81,7 +81,7 @@ def main():
                 out[key + "_h"] = h
             case += 1
 
-    # the driver convention of test/test_dgsks.c (glibc rand() replaced by a seeded
+    # the input convention of the test program test/test_dgsks.c (glibc rand() replaced by a seeded
     # integer stream with the same value ranges)
     rs = np.random.default_rng(1)
     for t in range(8):
