@@ -38,7 +38,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix peak
 # WRITE_SIZE x 1), keyed by (workload, precision); the summaries and commands
 # are in profiles/mixture_k64_fp64_and_mixed.md.
 TRAFFIC = {("mixture_k64", 0): 2.344e9,    # fused_f64_mfma_kernel
-           ("mixture_k64", 2): 3.43e9}    # mixed_kernel + pack + bucketing pre-pass
+           ("mixture_k64", 2): 3.49e9}    # mixed_kernel + pack + bucketing pre-pass
 # flops per kernel evaluation in the reference's model, test/test_dgsks.c:254-283
 FLOP_C = {0: 37, 1: 6, 2: 60, 3: 35, 4: 89, 5: 8, 6: 6, 7: 7}
 TYPE_NAMES = ["gaussian", "polynomial", "laplace", "var_bandwidth", "tanh", "quartic",
