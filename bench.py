@@ -209,6 +209,11 @@ def bench_list(args, d, L):
 
     evals = float(np.sum(np.diff(a_off) * np.diff(b_off)))
     types = [kd["type"] for kd in d["kernels"]]
+    # tasks a mixed-precision batch sends to the mixed kernels: Gaussian,
+    # k >= 16 and m, n >= 16384 (the routing rule of the batched launch)
+    n_mixed = 0 if prec == 0 else int(np.sum([
+        t in (0, 3) and k >= 16 and ma >= 16384 and nb >= 16384
+        for t, ma, nb in zip(types, np.diff(a_off), np.diff(b_off))]))
     per = float(np.sum([(2 * k + FLOP_C[t]) * float(ma) * float(nb) for t, ma, nb in
                         zip(types, np.diff(a_off), np.diff(b_off))]))
     achieved = per / (kernel_ms / 1e3) / 1e12
@@ -216,9 +221,9 @@ def bench_list(args, d, L):
         "metric": METRIC, "value": evals / (ms_per_step / 1e3), "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if prec == 0 else "mixed", "data": "synthetic (seeded numpy)",
+        "dtype": "f64" if n_mixed == 0 else "mixed", "data": "synthetic (seeded numpy)",
         "config": {"workload": d["label"], "ntask": ntask, "evals": evals, "k": k,
-                   "precision": args.precision},
+                   "precision": args.precision, "tasks_on_mixed_kernels": n_mixed},
         "roofline": {"bound": "mfma" if 9 <= k <= 64 else "valu", "pipe": "fp64",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,

@@ -579,7 +579,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
     dt[t] = DevTask{td.type, td.m, td.n, 1024, 0, 0, td.a0, td.b0, td.P, td.hi, td.hj};
     if (td.m == 0 || td.n == 0) continue;
     if (precision != KS_PREC_FP64 && (td.type == KS_GAUSSIAN || td.type == KS_GAUSSIAN_VAR_BANDWIDTH) &&
-        k >= kMixedListMinK && (long)td.m * td.n >= kMixedListMin) {
+        k >= kMixedListMinK && td.m >= kMixedListMinSide && td.n >= kMixedListMinSide) {
       pl->mixed.push_back(td);
       continue;
     }

@@ -37,10 +37,14 @@ TileChoice list_tile_choice(int type, int k, int precision);
 TileChoice list_default_tile(int type, int k);
 
 // In a mixed-precision batch the Gaussian tasks with k >= kMixedListMinK and
-// at least kMixedListMin pairs run through the mixed kernels (one pre-pass and
-// launch sequence per task, paid back only on large tasks where the BF16 cross
-// term matters); every other task runs in the persistent FP64 launch.
-constexpr long kMixedListMin = 1l << 24;
+// m, n >= kMixedListMinSide run through the mixed kernels (one pre-pass and
+// launch sequence per task); every other task runs in the persistent FP64
+// launch.  The sides are those at which the mixed form buckets its points and
+// beats the FP64 kernels (the single-precision front end uses the same rule).
+// Routing tasks of 2^24 pairs and more, as once, made list_pool k = 32 slower
+// than FP64 (41 ms against 27.7 ms): below these sides one persistent launch
+// beats a launch sequence per task.
+constexpr int kMixedListMinSide = 16384;
 constexpr int kMixedListMinK = 16;
 
 // A batch layout on the device (list.hip).  Building it does the host work

@@ -31,10 +31,11 @@ def L():
     return _lib.lib()
 
 
-def _batch(k, seed, pool=9000, ntask=36, big_gauss=0):
+def _batch(k, seed, pool=9000, ntask=36, big_gauss=0, big=4200):
     """Tasks over one pool: Gaussian / Laplace / polynomial kernels, shared
     targets, an empty task, sizes across the tile slices; `big_gauss` extra
-    Gaussian tasks of 4200 x 4200 (past the mixed-routing threshold)."""
+    Gaussian tasks of big x big (16384 x 16384 reaches the mixed kernels in a
+    mixed-precision batch)."""
     from ks_amd import _lib
     rng = np.random.default_rng(seed)
     X = rng.random((pool, k)) * (1.5 / np.sqrt(k))
@@ -42,8 +43,8 @@ def _batch(k, seed, pool=9000, ntask=36, big_gauss=0):
     for p in range(k):
         X2 += X[:, p] * X[:, p]
     w = rng.standard_normal(pool)
-    ms = list(rng.integers(1, 300, ntask)) + [4200] * big_gauss
-    ns = list(rng.integers(1, 2600, ntask)) + [4200] * big_gauss
+    ms = list(rng.integers(1, 300, ntask)) + [big] * big_gauss
+    ns = list(rng.integers(1, 2600, ntask)) + [big] * big_gauss
     ms[2] = 0
     kinds = [dict(type=0, scal=-1.5), dict(type=2), dict(type=1, scal=1.0 / k, cons=1.0, powe=2.0)]
     kernels_py = [kinds[t % 3] for t in range(ntask)] + [dict(type=0, scal=-40.0)] * big_gauss
@@ -368,9 +369,10 @@ def test_autotune_cache_file(tmp_path, torch_cuda):
 
 def test_mixed_batch_routing(L, torch_cuda, oracle):
     """A mixed-precision batch: Laplace and polynomial tasks and small Gaussian
-    tasks run in the persistent FP64 launch, the large Gaussian tasks in the
-    mixed kernels; the whole batch stays within 1e-6 of the oracle."""
-    B = _batch(16, 600, ntask=30, big_gauss=2)
+    tasks run in the persistent FP64 launch, the large Gaussian tasks
+    (16384 x 16384, k = 16) in the mixed kernels; the whole batch stays within
+    1e-6 of the oracle."""
+    B = _batch(16, 600, pool=20000, ntask=30, big_gauss=2, big=16384)
     u = np.zeros(B["pool"])
     ref = _oracle_list(oracle, B, u)
     p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
