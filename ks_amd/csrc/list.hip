@@ -225,7 +225,10 @@ __device__ double tile_valu_lds(const ListArgs &L, const DevTask &T, int pos, in
         for (int q = 0; q < 4; ++q) acc = fma(d2[q], wq[q], acc);
       }
     }
-#pragma unroll 1
+    // two sources per iteration: their kernel values are independent chains
+    // that interleave (acc still takes them in source order); more spills at
+    // 4 waves per SIMD, but list_pool k = 3 / 8 12.8 -> 11.7 / 19.4 -> 18.7 ms
+#pragma unroll 2
     for (; r < nch; ++r) {
       const double bb = side[3 * r], wj = side[3 * r + 1], hj = side[3 * r + 2];
       const double *b = stage + r * K;
