@@ -188,17 +188,32 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
     double wv[4][NT];
     double wx[4];  // XR: weight 16 NT of the sources 4v + g
   };
-  auto fetch = [&](int jb, SB &b) {
+  // a block's map entries load two blocks ahead of the data they address, so
+  // no block waits on two dependent round trips
+  struct SI {
+    int fb;     // the lane's fragment source (row of XB)
+    int bj[4];  // sources 4v + g
+    int wi[4];  // their weight rows (-1: past the slice)
+  };
+  auto fetch_idx = [&](int jb, SI &x) {
     const int col = jb + p16;
-    load_frag<KC, EXACT>(p.XB, (long)p.bmap[col < j1 ? col : j1 - 1] * k, k, g, b.f);
+    x.fb = p.bmap[col < j1 ? col : j1 - 1];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int src = jb + 4 * v + g;
       const bool ok = src < j1;
-      const int bj = p.bmap[ok ? src : j1 - 1];
+      x.bj[v] = p.bmap[ok ? src : j1 - 1];
+      x.wi[v] = ok ? p.wmap[src] : -1;
+    }
+  };
+  auto fetch = [&](const SI &x, SB &b) {
+    load_frag<KC, EXACT>(p.XB, (long)x.fb * k, k, g, b.f);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int bj = x.bj[v];
       b.bb[v] = p.XB2[bj];
       b.hj[v] = (TYPE == VAR_BANDWIDTH) ? p.hj[bj] : 0.0;
-      const long wrow = ok ? (long)p.wmap[src] * nr : -1;
+      const long wrow = x.wi[v] >= 0 ? (long)x.wi[v] * nr : -1;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const int r = 16 * t + p16;
@@ -230,13 +245,26 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
       }
     }
   };
+  // b0 / b1: data of blocks jb / jb + 16; i0 / i1: map entries of the block
+  // whose data loads next into b0 / b1
   SB b0, b1;
-  if (j0 < j1) fetch(j0, b0);
+  SI i0, i1;
+  if (j0 < j1) {
+    fetch_idx(j0, i0);
+    fetch(i0, b0);
+    if (j0 + 16 < j1) fetch_idx(j0 + 16, i1);
+  }
   for (int jb = j0; jb < j1; jb += 32) {
-    if (jb + 16 < j1) fetch(jb + 16, b1);
+    if (jb + 16 < j1) {
+      fetch(i1, b1);
+      if (jb + 32 < j1) fetch_idx(jb + 32, i0);
+    }
     block(b0);
     if (jb + 16 >= j1) break;
-    if (jb + 32 < j1) fetch(jb + 32, b0);
+    if (jb + 32 < j1) {
+      fetch(i0, b0);
+      if (jb + 48 < j1) fetch_idx(jb + 48, i1);
+    }
     block(b1);
   }
 
