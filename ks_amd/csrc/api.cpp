@@ -258,13 +258,23 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
                                 : plan_sym_rows(p.m, kTargetWGs, sym_mfma_rows());
     const size_t ncol = (size_t)pl.row_blocks * p.m;
     if (ncol <= kSymMaxColPartials) {
+      // the vector form runs over compact tables gathered beside its partials
+      // (N-body k = 3: the source loads stop being a gather through amap)
+      const size_t tab = sym_valu ? sym_table_doubles(p) + 32 : 0;
       double *partial = nullptr;
       unsigned *counters = nullptr;
-      int rc = get_scratch(s, (size_t)pl.nsplit * p.m + ncol, (size_t)pl.row_blocks, &partial,
-                           &counters);
+      int rc = get_scratch(s, (size_t)pl.nsplit * p.m + ncol + tab, (size_t)pl.row_blocks,
+                           &partial, &counters);
       if (rc != KS_OK) return rc;
       double *colp = partial + (size_t)pl.nsplit * p.m;
-      hipError_t e = sym_valu ? launch_fused_f64_sym(p, pl, partial, counters, colp, s)
+      Problem q = p;
+      if (sym_valu) {
+        double *t = colp + ncol;
+        t = reinterpret_cast<double *>((reinterpret_cast<uintptr_t>(t) + 255) & ~(uintptr_t)255);
+        hipError_t ge = launch_sym_gather(p, t, &q, s);
+        if (ge != hipSuccess) return hip_failure(ge, "symmetric gather launch");
+      }
+      hipError_t e = sym_valu ? launch_fused_f64_sym(q, pl, partial, counters, colp, s)
                               : launch_fused_f64_sym_mfma(p, pl, partial, counters, colp, s);
       return e == hipSuccess ? KS_OK : hip_failure(e, "symmetric launch");
     }

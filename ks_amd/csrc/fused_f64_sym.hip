@@ -258,6 +258,58 @@ static hipError_t sym_k(const SymArgs &A, const SymPlan &pl, hipStream_t s) {
   }
 }
 
+// One thread per point, in map order.  The symmetric kernel reads every source
+// row and its factors once per row block with wave-uniform scalar loads; through
+// a permutation map each is a different line that the scalar cache mostly
+// misses, from the compact tables they are consecutive.
+__global__ void sym_gather_kernel(const Problem p, double *__restrict__ Xc,
+                                  double *__restrict__ X2c, double *__restrict__ wc,
+                                  double *__restrict__ hc, int *__restrict__ ids) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.m) return;
+  const int a = p.amap[i], k = p.k;
+  const double *x = p.XA + (long)a * k;
+  for (int q = 0; q < k; ++q) Xc[(long)i * k + q] = x[q];
+  X2c[i] = p.XA2[a];
+  wc[i] = p.w[p.wmap[i]];
+  if (hc) hc[i] = p.hi[a];
+  ids[i] = i;
+}
+
+static size_t up32(size_t e) { return (e + 31) / 32 * 32; }
+
+size_t sym_table_doubles(const Problem &p) {
+  const size_t m = (size_t)p.m;
+  return up32(m * p.k) + 2 * up32(m) + (p.type == VAR_BANDWIDTH ? up32(m) : 0) +
+         up32((m + 1) / 2);
+}
+
+hipError_t launch_sym_gather(const Problem &p, double *tab, Problem *q, hipStream_t s) {
+  const size_t m = (size_t)p.m;
+  size_t off = 0;
+  double *Xc = tab + off;
+  off += up32(m * p.k);
+  double *X2c = tab + off;
+  off += up32(m);
+  double *wc = tab + off;
+  off += up32(m);
+  double *hc = nullptr;
+  if (p.type == VAR_BANDWIDTH) {
+    hc = tab + off;
+    off += up32(m);
+  }
+  int *ids = reinterpret_cast<int *>(tab + off);
+  hipLaunchKernelGGL(sym_gather_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, p, Xc,
+                     X2c, wc, hc, ids);
+  *q = p;
+  q->XA = q->XB = Xc;
+  q->XA2 = q->XB2 = X2c;
+  q->amap = q->bmap = q->wmap = ids;
+  q->w = wc;
+  if (hc) q->hi = q->hj = hc;
+  return hipGetLastError();
+}
+
 hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *partial,
                                 unsigned *counters, double *colp, hipStream_t s) {
   SymArgs A;

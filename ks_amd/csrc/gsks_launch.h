@@ -50,6 +50,11 @@ hipError_t launch_sym_colsum(const Problem &p, int B, const double *colp, hipStr
 // partial: [nsplit][m]; counters: [row_blocks] (zero); colp: [row_blocks][m]
 hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *partial,
                                 unsigned *counters, double *colp, hipStream_t s);
+// Compact tables of a symmetric problem (the mapped rows, norms, weights and
+// bandwidths in map order, and an identity map) in `tab` (sym_table_doubles(p)
+// doubles, 256-byte aligned); *q is p over those tables, writing through p's umap.
+size_t sym_table_doubles(const Problem &p);
+hipError_t launch_sym_gather(const Problem &p, double *tab, Problem *q, hipStream_t s);
 
 // FP64 cross term on the matrix cores (fused_f64_mfma.hip)
 bool mfma_f64_eligible(int type, int k);
