@@ -246,6 +246,8 @@ constexpr int kMinCols = 128;     // keep each workgroup's source slice >= 128 p
 // norms) and its split hand-off.  Measured at m = n = 8192, k = 16: 0.205 ms
 // with 256-source slices (2048 workgroups), 0.155 ms with 1024.
 constexpr int kMinColsMfma = 1024;
+// the vector kernel gathers its sources into compact tables from this many targets
+constexpr int kGatherMinRows = 1024;
 // symmetric sums keep one column partial per (row block, source): at most 4 GiB
 constexpr size_t kSymMaxColPartials = (size_t)1 << 29;
 
@@ -282,15 +284,26 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
   const bool mfma = fused_mfma_eligible(p.type, p.k);
   FusedPlan plan = mfma ? plan_split(p.m, p.n, mfma_rows_per_wg(), kTargetWGs, kMinColsMfma)
                         : plan_fused(p.m, p.n, p.k, kTargetWGs, kMinCols);
+  // the vector kernel reads its sources from compact tables when there are
+  // enough targets to pay for the copy
+  const bool gather = !mfma && p.m >= kGatherMinRows;
+  const size_t npart = plan.nsplit > 1 ? (size_t)plan.nsplit * p.m : 0;
+  const size_t tab = gather ? gather_table_doubles(p.n, p.k, p.type) + 32 : 0;
   double *partial = nullptr;
   unsigned *counters = nullptr;
-  if (plan.nsplit > 1) {
-    int rc = get_scratch(s, (size_t)plan.nsplit * p.m, (size_t)plan.row_blocks, &partial,
-                         &counters);
+  if (npart + tab > 0) {
+    int rc = get_scratch(s, npart + tab, (size_t)plan.row_blocks, &partial, &counters);
     if (rc != KS_OK) return rc;
   }
+  Problem q = p;
+  if (gather) {
+    double *t = partial + npart;
+    t = reinterpret_cast<double *>((reinterpret_cast<uintptr_t>(t) + 255) & ~(uintptr_t)255);
+    hipError_t ge = launch_source_gather(p, t, &q, s);
+    if (ge != hipSuccess) return hip_failure(ge, "source gather launch");
+  }
   hipError_t e = mfma ? launch_fused_f64_mfma(p, plan, partial, counters, s)
-                      : launch_fused_f64(p, plan, partial, counters, s);
+                      : launch_fused_f64(q, plan, partial, counters, s);
   return e == hipSuccess ? KS_OK : hip_failure(e, mfma ? "fused_f64_mfma launch" : "fused_f64 launch");
 }
 

@@ -258,56 +258,94 @@ static hipError_t sym_k(const SymArgs &A, const SymPlan &pl, hipStream_t s) {
   }
 }
 
-// One thread per point, in map order.  The symmetric kernel reads every source
-// row and its factors once per row block with wave-uniform scalar loads; through
-// a permutation map each is a different line that the scalar cache mostly
+// One thread per point, in map order: rows X[map[i]], norms, weights
+// w[wmap[i]], bandwidths and an identity map.  The vector kernels read every
+// source row and its factors with wave-uniform scalar loads; through a
+// permutation map each is a different line that the scalar cache mostly
 // misses, from the compact tables they are consecutive.
-__global__ void sym_gather_kernel(const Problem p, double *__restrict__ Xc,
-                                  double *__restrict__ X2c, double *__restrict__ wc,
-                                  double *__restrict__ hc, int *__restrict__ ids) {
+__global__ void gather_points_kernel(int count, int k, const double *__restrict__ X,
+                                     const double *__restrict__ X2, const int *__restrict__ map,
+                                     const double *__restrict__ w, const int *__restrict__ wmap,
+                                     const double *__restrict__ h, double *__restrict__ Xc,
+                                     double *__restrict__ X2c, double *__restrict__ wc,
+                                     double *__restrict__ hc, int *__restrict__ ids) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.m) return;
-  const int a = p.amap[i], k = p.k;
-  const double *x = p.XA + (long)a * k;
+  if (i >= count) return;
+  const int a = map[i];
+  const double *x = X + (long)a * k;
   for (int q = 0; q < k; ++q) Xc[(long)i * k + q] = x[q];
-  X2c[i] = p.XA2[a];
-  wc[i] = p.w[p.wmap[i]];
-  if (hc) hc[i] = p.hi[a];
+  X2c[i] = X2[a];
+  wc[i] = w[wmap[i]];
+  if (hc) hc[i] = h[a];
   ids[i] = i;
 }
 
 static size_t up32(size_t e) { return (e + 31) / 32 * 32; }
 
-size_t sym_table_doubles(const Problem &p) {
-  const size_t m = (size_t)p.m;
-  return up32(m * p.k) + 2 * up32(m) + (p.type == VAR_BANDWIDTH ? up32(m) : 0) +
-         up32((m + 1) / 2);
+size_t gather_table_doubles(long count, int k, int type) {
+  const size_t c = (size_t)count;
+  return up32(c * k) + 2 * up32(c) + (type == VAR_BANDWIDTH ? up32(c) : 0) + up32((c + 1) / 2);
 }
 
-hipError_t launch_sym_gather(const Problem &p, double *tab, Problem *q, hipStream_t s) {
-  const size_t m = (size_t)p.m;
+// The tables of `count` points (rows X[map], X2[map], w[wmap], h[map] and the
+// identity map) laid out in `tab`, filled by one launch.
+static hipError_t gather_points(int count, int k, int type, const double *X, const double *X2,
+                                const int *map, const double *w, const int *wmap,
+                                const double *h, double *tab, const double **Xc,
+                                const double **X2c, const double **wc, const double **hc,
+                                const int **ids, hipStream_t s) {
+  const size_t c = (size_t)count;
   size_t off = 0;
-  double *Xc = tab + off;
-  off += up32(m * p.k);
-  double *X2c = tab + off;
-  off += up32(m);
-  double *wc = tab + off;
-  off += up32(m);
-  double *hc = nullptr;
-  if (p.type == VAR_BANDWIDTH) {
-    hc = tab + off;
-    off += up32(m);
+  double *x = tab + off;
+  off += up32(c * k);
+  double *x2 = tab + off;
+  off += up32(c);
+  double *ww = tab + off;
+  off += up32(c);
+  double *hh = nullptr;
+  if (type == VAR_BANDWIDTH) {
+    hh = tab + off;
+    off += up32(c);
   }
-  int *ids = reinterpret_cast<int *>(tab + off);
-  hipLaunchKernelGGL(sym_gather_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, p, Xc,
-                     X2c, wc, hc, ids);
+  int *id = reinterpret_cast<int *>(tab + off);
+  hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)((c + 255) / 256)), dim3(256), 0, s,
+                     count, k, X, X2, map, w, wmap, h, x, x2, ww, hh, id);
+  *Xc = x;
+  *X2c = x2;
+  *wc = ww;
+  *hc = hh;
+  *ids = id;
+  return hipGetLastError();
+}
+
+size_t sym_table_doubles(const Problem &p) { return gather_table_doubles(p.m, p.k, p.type); }
+
+hipError_t launch_sym_gather(const Problem &p, double *tab, Problem *q, hipStream_t s) {
+  const double *Xc, *X2c, *wc, *hc;
+  const int *ids;
+  hipError_t e = gather_points(p.m, p.k, p.type, p.XA, p.XA2, p.amap, p.w, p.wmap, p.hi, tab, &Xc,
+                               &X2c, &wc, &hc, &ids, s);
   *q = p;
   q->XA = q->XB = Xc;
   q->XA2 = q->XB2 = X2c;
   q->amap = q->bmap = q->wmap = ids;
   q->w = wc;
   if (hc) q->hi = q->hj = hc;
-  return hipGetLastError();
+  return e;
+}
+
+hipError_t launch_source_gather(const Problem &p, double *tab, Problem *q, hipStream_t s) {
+  const double *Xc, *X2c, *wc, *hc;
+  const int *ids;
+  hipError_t e = gather_points(p.n, p.k, p.type, p.XB, p.XB2, p.bmap, p.w, p.wmap, p.hj, tab, &Xc,
+                               &X2c, &wc, &hc, &ids, s);
+  *q = p;
+  q->XB = Xc;
+  q->XB2 = X2c;
+  q->bmap = q->wmap = ids;
+  q->w = wc;
+  if (hc) q->hj = hc;
+  return e;
 }
 
 hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *partial,

@@ -55,6 +55,10 @@ hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *par
 // doubles, 256-byte aligned); *q is p over those tables, writing through p's umap.
 size_t sym_table_doubles(const Problem &p);
 hipError_t launch_sym_gather(const Problem &p, double *tab, Problem *q, hipStream_t s);
+// The same for the sources of a general problem (n points through bmap / wmap);
+// *q reads its sources from the tables, its targets as p does.
+size_t gather_table_doubles(long count, int k, int type);
+hipError_t launch_source_gather(const Problem &p, double *tab, Problem *q, hipStream_t s);
 
 // FP64 cross term on the matrix cores (fused_f64_mfma.hip)
 bool mfma_f64_eligible(int type, int k);
