@@ -521,6 +521,23 @@ __global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void lis
   }
 }
 
+// The compact source pool of a plan run: position i (task offset b0 + j) gets
+// row XB[bmap[i]], its norm and the weight w[wmap[i]].  The tiles then read a
+// chunk of a task's sources as consecutive rows instead of one line per source.
+__global__ void list_gather_kernel(long n, int k, const int *__restrict__ bmap,
+                                   const int *__restrict__ wmap, const double *__restrict__ XB,
+                                   const double *__restrict__ XB2, const double *__restrict__ w,
+                                   double *__restrict__ cX, double *__restrict__ cX2,
+                                   double *__restrict__ cw) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int bj = bmap[i];
+  const double *x = XB + (long)bj * k;
+  for (int q = 0; q < k; ++q) cX[i * k + q] = x[q];
+  cX2[i] = XB2[bj];
+  cw[i] = w[wmap[i]];
+}
+
 __global__ void iota_kernel(int *v, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] = (int)i;
@@ -559,6 +576,16 @@ struct ListPlan {
   int grid[3] = {0, 0, 0};
   const int *keys = nullptr, *pos = nullptr;  // umap sorted, with positions
   std::vector<TaskDesc> mixed;                // tasks the mixed kernels run
+  // k < 8 without variable-bandwidth tasks: every run first copies the
+  // sources into a compact pool (rows, norms, weights in bmap order) that the
+  // tiles read through the identity map cid.  A row shorter than a 64-byte line
+  // then stops costing a line of its own per chunk (list_pool k = 3: 11.7 ->
+  // 10.7 ms); at k = 8 a row is a whole line already, and the copy only adds
+  // (18.7 -> 19.6 ms).
+  bool compact = false;
+  long total_b = 0;
+  double *cX = nullptr, *cX2 = nullptr, *cw = nullptr;
+  int *cid = nullptr;
   const int *umap = nullptr, *amap = nullptr, *bmap = nullptr, *wmap = nullptr;
 };
 
@@ -577,6 +604,12 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   // ---- which tasks the persistent FP64 launch runs, with their tile shapes
   std::vector<DevTask> dt(tasks.size());
   std::vector<RowBlock> rbs;
+  bool varbw = false;
+  for (const TaskDesc &td : tasks) {
+    pl->total_b = std::max(pl->total_b, td.b0 + td.n);
+    varbw = varbw || td.type == KS_GAUSSIAN_VAR_BANDWIDTH;
+  }
+  pl->compact = k < 8 && !varbw && pl->total_b > 0;
   for (size_t t = 0; t < tasks.size(); ++t) {
     const TaskDesc &td = tasks[t];
     dt[t] = DevTask{td.type, td.m, td.n, 1024, 0, 0, td.a0, td.b0, td.P, td.hi, td.hj};
@@ -647,7 +680,12 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   size_t off_kout = off_iota + al((size_t)total_a * sizeof(int));
   size_t off_vout = off_kout + al((size_t)total_a * sizeof(int));
   size_t off_sort = off_vout + al((size_t)total_a * sizeof(int));
-  size_t bytes = off_sort + al(sort_bytes);
+  const size_t cb = pl->compact ? (size_t)pl->total_b : 0;
+  size_t off_cx = off_sort + al(sort_bytes);
+  size_t off_cx2 = off_cx + al(cb * k * sizeof(double));
+  size_t off_cw = off_cx2 + al(cb * sizeof(double));
+  size_t off_cid = off_cw + al(cb * sizeof(double));
+  size_t bytes = off_cid + al(cb * sizeof(int));
   char *buf = nullptr;
   if (hipMalloc((void **)&buf, bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -682,6 +720,15 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   int *iota = (int *)(buf + off_iota);
   int *kout = (int *)(buf + off_kout);
   int *vout = (int *)(buf + off_vout);
+  if (pl->compact) {
+    pl->cX = (double *)(buf + off_cx);
+    pl->cX2 = (double *)(buf + off_cx2);
+    pl->cw = (double *)(buf + off_cw);
+    pl->cid = (int *)(buf + off_cid);
+    if (rc == KS_OK)
+      hipLaunchKernelGGL(iota_kernel, dim3((unsigned)((pl->total_b + 255) / 256)), dim3(256), 0, s,
+                         pl->cid, pl->total_b);
+  }
   if (rc == KS_OK) {
     const long nb = (total_a + 255) / 256;
     hipLaunchKernelGGL(iota_kernel, dim3((unsigned)nb), dim3(256), 0, s, iota, total_a);
@@ -711,17 +758,28 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
 
 int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
   if (pl.ntiles > 0) {
+    const int *bm = pl.bmap, *wm = pl.wmap;
+    const double *xb = L.XB, *xb2 = L.XB2, *ww = L.w;
+    if (pl.compact) {
+      hipLaunchKernelGGL(list_gather_kernel, dim3((unsigned)((pl.total_b + 255) / 256)), dim3(256),
+                         0, s, pl.total_b, pl.k, pl.bmap, pl.wmap, L.XB, L.XB2, L.w, pl.cX,
+                         pl.cX2, pl.cw);
+      bm = wm = pl.cid;
+      xb = pl.cX;
+      xb2 = pl.cX2;
+      ww = pl.cw;
+    }
     for (int kind = 0; kind < 3; ++kind) {
       ListArgs A = pl.A[kind];
       if (A.ntiles == 0) continue;
       A.amap = pl.amap;
-      A.bmap = pl.bmap;
-      A.wmap = pl.wmap;
+      A.bmap = bm;
+      A.wmap = wm;
       A.XA = L.XA;
       A.XA2 = L.XA2;
-      A.XB = L.XB;
-      A.XB2 = L.XB2;
-      A.w = L.w;
+      A.XB = xb;
+      A.XB2 = xb2;
+      A.w = ww;
       // k = 3 and 8: rows and sources in registers / scalar loads; 9 <= k <= 64:
       // padded to 16, 32 or 64 (row tiles stage the sources in LDS); other k:
       // generic row tiles.  Matrix-core tiles only exist for 9 <= k <= 64.
