@@ -524,19 +524,28 @@ __global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void lis
 // The compact source pool of a plan run: position i (task offset b0 + j) gets
 // row XB[bmap[i]], its norm and the weight w[wmap[i]].  The tiles then read a
 // chunk of a task's sources as consecutive rows instead of one line per source.
-__global__ void list_gather_kernel(long n, int k, const int *__restrict__ bmap,
+// K is the row length (k < 8), so a thread's loads are all in flight before its
+// stores.
+template <int K>
+__global__ void list_gather_kernel(long n, const int *__restrict__ bmap,
                                    const int *__restrict__ wmap, const double *__restrict__ XB,
                                    const double *__restrict__ XB2, const double *__restrict__ w,
                                    double *__restrict__ cX, double *__restrict__ cX2,
                                    double *__restrict__ cw) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int bj = bmap[i];
-  const double *x = XB + (long)bj * k;
-  for (int q = 0; q < k; ++q) cX[i * k + q] = x[q];
-  cX2[i] = XB2[bj];
-  cw[i] = w[wmap[i]];
+  const int bj = bmap[i], wi = wmap[i];
+  const double *x = XB + (long)bj * K;
+  double v[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) v[q] = x[q];
+  const double b2 = XB2[bj], ww = w[wi];
+#pragma unroll
+  for (int q = 0; q < K; ++q) cX[i * K + q] = v[q];
+  cX2[i] = b2;
+  cw[i] = ww;
 }
+
 
 __global__ void iota_kernel(int *v, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -756,14 +765,26 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   return KS_OK;
 }
 
+template <int K>
+static void launch_list_gather(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
+  hipLaunchKernelGGL((list_gather_kernel<K>), dim3((unsigned)((pl.total_b + 255) / 256)), dim3(256),
+                     0, s, pl.total_b, pl.bmap, pl.wmap, L.XB, L.XB2, L.w, pl.cX, pl.cX2, pl.cw);
+}
+
 int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
   if (pl.ntiles > 0) {
     const int *bm = pl.bmap, *wm = pl.wmap;
     const double *xb = L.XB, *xb2 = L.XB2, *ww = L.w;
     if (pl.compact) {
-      hipLaunchKernelGGL(list_gather_kernel, dim3((unsigned)((pl.total_b + 255) / 256)), dim3(256),
-                         0, s, pl.total_b, pl.k, pl.bmap, pl.wmap, L.XB, L.XB2, L.w, pl.cX,
-                         pl.cX2, pl.cw);
+      switch (pl.k) {
+        case 1: launch_list_gather<1>(pl, L, s); break;
+        case 2: launch_list_gather<2>(pl, L, s); break;
+        case 3: launch_list_gather<3>(pl, L, s); break;
+        case 4: launch_list_gather<4>(pl, L, s); break;
+        case 5: launch_list_gather<5>(pl, L, s); break;
+        case 6: launch_list_gather<6>(pl, L, s); break;
+        default: launch_list_gather<7>(pl, L, s); break;
+      }
       bm = wm = pl.cid;
       xb = pl.cX;
       xb2 = pl.cX2;
