@@ -100,13 +100,22 @@ __global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv) {
   piv[t] = q < p.k ? (float)p.XA[(long)p.amap[r] * p.k + q] : 0.0f;
 }
 
-// Bucket (nearest pivot, FP32 squared distance) of every target, then every source.
+// Bucket (nearest pivot) of every target, then every source: the pivot with the
+// least |p|^2 - 2 x.p in FP32 (|x - p|^2 less the point's own |x|^2, one FMA per
+// coordinate).  The buckets only decide which points share a block.
 template <int KP>
 __global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p, int P,
                                                          const float *__restrict__ piv,
                                                          unsigned *key_a, unsigned *key_b) {
   __shared__ float sp[kPivots * KP];
+  __shared__ float spn[kPivots];
   for (int e = threadIdx.x; e < P * KP; e += 256) sp[e] = piv[e];
+  __syncthreads();
+  for (int i = threadIdx.x; i < P; i += 256) {
+    float n2 = 0.0f;
+    for (int q = 0; q < KP; ++q) n2 = fmaf(sp[i * KP + q], sp[i * KP + q], n2);
+    spn[i] = n2;
+  }
   __syncthreads();
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
   if (t >= (long)p.m + p.n) return;
@@ -118,12 +127,10 @@ __global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p, int P,
   unsigned bi = 0;
   for (int i = 0; i < P; ++i) {
     const float *pv = sp + i * KP;
-    float d = 0.0f;
+    float dot = 0.0f;
 #pragma unroll
-    for (int q = 0; q < KP; ++q) {
-      const float e = xv[q] - pv[q];
-      d = fmaf(e, e, d);
-    }
+    for (int q = 0; q < KP; ++q) dot = fmaf(xv[q], pv[q], dot);
+    const float d = fmaf(-2.0f, dot, spn[i]);
     if (d < best) {
       best = d;
       bi = (unsigned)i;
