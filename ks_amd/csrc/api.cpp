@@ -256,7 +256,7 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
   // on the vector units (k <= 8) or the matrix cores (9 <= k <= 64, not Laplace)
   const bool sym_valu = sym_eligible(p), sym_mfma = !sym_valu && sym_mfma_eligible(p);
   if (sym_valu || sym_mfma) {
-    const SymPlan pl = sym_valu ? plan_sym(p.m, kTargetWGs)
+    const SymPlan pl = sym_valu ? plan_sym(p.m, p.k, kTargetWGs)
                                 : plan_sym_rows(p.m, kTargetWGs, sym_mfma_rows());
     const size_t ncol = (size_t)pl.row_blocks * p.m;
     if (ncol <= kSymMaxColPartials) {

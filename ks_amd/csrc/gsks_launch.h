@@ -43,7 +43,9 @@ struct SymPlan {
   int nsplit;
 };
 bool sym_eligible(const Problem &p);
-SymPlan plan_sym(int m, int target_wgs);
+// rows per lane of the symmetric vector kernel at dimension k
+int sym_rows_per_lane(int k);
+SymPlan plan_sym(int m, int k, int target_wgs);
 SymPlan plan_sym_rows(int m, int target_wgs, int rows_per_wg);
 // u[umap[j]] += sum_{I < J} colp[I][j] over row blocks of B targets (J = j's block)
 hipError_t launch_sym_colsum(const Problem &p, int B, const double *colp, hipStream_t s);
