@@ -180,10 +180,9 @@ int dgsks_device(const ks_t *kernel, int m, int n, int k,
  * coordinates.  hi / hj (variable bandwidth) stay double.  sgsks_device takes
  * device pointers and is enqueued on `stream` (NULL = the HIP null stream); a
  * umap entry may not repeat there.  sgsks takes all-host or all-device
- * pointers, synchronises, and picks KS_PREC_MIXED_BF16X3 for the Gaussian
- * kernels at 16 <= k <= 128 with m, n >= 16384 (where clustered points gain
- * from it) and KS_PREC_FP64 otherwise; a repeated umap entry keeps the value
- * of its last row, as in dgsks. */
+ * pointers, synchronises and sums in KS_PREC_FP64 (the mixed forms' accuracy
+ * depends on the data, so only sgsks_device offers them); a repeated umap
+ * entry keeps the value of its last row, as in dgsks. */
 int sgsks_device(const ks_t *kernel, int m, int n, int k,
                  float *u, const int *umap,
                  const float *XA, const float *XA2, const int *amap,

@@ -70,16 +70,18 @@ void set_last_error(int code) { t_last_error = code; }
 
 // ------------------------------------------------------------------ device context
 
-static std::mutex g_mu;
-
 struct Scratch {
+  std::recursive_mutex mu;  // held by a ScratchLease
   double *partial = nullptr;
   size_t partial_elems = 0;
   unsigned *counters = nullptr;
   size_t counter_elems = 0;
   char *aux = nullptr;
   size_t aux_bytes = 0;
+  bool captured = false;        // a stream capture has handed out these buffers
+  std::vector<void *> retired;  // replaced buffers a captured graph may still use
 };
+static std::mutex g_map_mu;  // guards the map only (its nodes never move)
 static std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
 
 int ensure_device() {
@@ -110,58 +112,109 @@ hipStream_t thread_stream() {
   return s;
 }
 
-// Grow-only scratch for one (device, stream): partial sums and the zeroed
-// per-row-block counters of the split-source reduction.
-int get_scratch(hipStream_t s, size_t partial_elems, size_t counter_elems, double **partial,
-                unsigned **counters) {
+ScratchLease::ScratchLease(hipStream_t s) : s_(s) {
   int d = 0;
   (void)hipGetDevice(&d);
-  std::lock_guard<std::mutex> lk(g_mu);
-  Scratch &sc = g_scratch[{d, s}];
-  if (sc.partial_elems < partial_elems || sc.counter_elems < counter_elems) {
-    hipError_t se = hipStreamSynchronize(s);
-    if (se != hipSuccess) return hip_failure(se, "get_scratch: hipStreamSynchronize");
-    if (sc.partial_elems < partial_elems) {
-      if (sc.partial) (void)hipFree(sc.partial);
+  {
+    std::lock_guard<std::mutex> lk(g_map_mu);
+    sc_ = &g_scratch[{d, s}];
+  }
+  sc_->mu.lock();
+}
+
+ScratchLease::~ScratchLease() { sc_->mu.unlock(); }
+
+// Whether stream s is being captured into a graph.
+static bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return st != hipStreamCaptureStatusNone;
+}
+
+// Before buffers are handed out: refuse to grow under capture, and remember
+// that a capture has seen this scratch.
+static int scratch_prepare(Scratch &sc, hipStream_t s, bool grow, const char *what) {
+  if (!stream_capturing(s)) return KS_OK;
+  if (grow) {
+    snprintf(t_detail, sizeof(t_detail),
+             "%s: scratch must grow while the stream is being captured; run the call once on "
+             "this stream before capturing it", what);
+    return KS_ERR_ALLOC;
+  }
+  sc.captured = true;
+  return KS_OK;
+}
+
+// Drop a buffer that is being replaced: freed after the stream drains, or kept
+// for the life of the process when a captured graph may still use it.
+static int scratch_drop(Scratch &sc, hipStream_t s, void *p) {
+  if (!p) return KS_OK;
+  if (sc.captured) {
+    sc.retired.push_back(p);
+    return KS_OK;
+  }
+  hipError_t se = hipStreamSynchronize(s);
+  if (se != hipSuccess) return hip_failure(se, "scratch: hipStreamSynchronize");
+  (void)hipFree(p);
+  return KS_OK;
+}
+
+int ScratchLease::partials(size_t partial_elems, size_t counter_elems, double **partial,
+                           unsigned **counters) {
+  Scratch &sc = *sc_;
+  const bool grow_p = sc.partial_elems < partial_elems, grow_c = sc.counter_elems < counter_elems;
+  int rc = scratch_prepare(sc, s_, grow_p || grow_c, "scratch partials");
+  if (rc != KS_OK) return rc;
+  if (grow_p) {
+    rc = scratch_drop(sc, s_, sc.partial);
+    if (rc != KS_OK) return rc;
+    sc.partial = nullptr;
+    sc.partial_elems = 0;
+    size_t want = partial_elems + partial_elems / 4;
+    if (hipMalloc(&sc.partial, want * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
       sc.partial = nullptr;
-      size_t want = partial_elems + partial_elems / 4;
-      if (hipMalloc(&sc.partial, want * sizeof(double)) != hipSuccess) {
-        sc.partial_elems = 0;
-        return KS_ERR_ALLOC;
-      }
-      sc.partial_elems = want;
+      return KS_ERR_ALLOC;
     }
-    if (sc.counter_elems < counter_elems) {
-      if (sc.counters) (void)hipFree(sc.counters);
+    sc.partial_elems = want;
+  }
+  if (grow_c) {
+    rc = scratch_drop(sc, s_, sc.counters);
+    if (rc != KS_OK) return rc;
+    sc.counters = nullptr;
+    sc.counter_elems = 0;
+    size_t want = counter_elems + counter_elems / 4 + 64;
+    if (hipMalloc(&sc.counters, want * sizeof(unsigned)) != hipSuccess) {
+      (void)hipGetLastError();
       sc.counters = nullptr;
-      size_t want = counter_elems + counter_elems / 4 + 64;
-      if (hipMalloc(&sc.counters, want * sizeof(unsigned)) != hipSuccess) {
-        sc.counter_elems = 0;
-        return KS_ERR_ALLOC;
-      }
-      hipError_t me = hipMemset(sc.counters, 0, want * sizeof(unsigned));
-      if (me != hipSuccess) return hip_failure(me, "get_scratch: hipMemset");
-      sc.counter_elems = want;
+      return KS_ERR_ALLOC;
     }
+    hipError_t me = hipMemset(sc.counters, 0, want * sizeof(unsigned));
+    if (me != hipSuccess) return hip_failure(me, "scratch: hipMemset");
+    sc.counter_elems = want;
   }
   *partial = sc.partial;
   *counters = sc.counters;
   return KS_OK;
 }
 
-int get_aux(hipStream_t s, size_t bytes, char **base) {
-  int d = 0;
-  (void)hipGetDevice(&d);
-  std::lock_guard<std::mutex> lk(g_mu);
-  Scratch &sc = g_scratch[{d, s}];
-  if (sc.aux_bytes < bytes) {
-    hipError_t se = hipStreamSynchronize(s);
-    if (se != hipSuccess) return hip_failure(se, "get_aux: hipStreamSynchronize");
-    if (sc.aux) (void)hipFree(sc.aux);
+int ScratchLease::aux(size_t bytes, char **base) {
+  Scratch &sc = *sc_;
+  const bool grow = sc.aux_bytes < bytes;
+  int rc = scratch_prepare(sc, s_, grow, "scratch arena");
+  if (rc != KS_OK) return rc;
+  if (grow) {
+    rc = scratch_drop(sc, s_, sc.aux);
+    if (rc != KS_OK) return rc;
     sc.aux = nullptr;
+    sc.aux_bytes = 0;
     size_t want = bytes + bytes / 4 + 256;
     if (hipMalloc(&sc.aux, want) != hipSuccess) {
-      sc.aux_bytes = 0;
+      (void)hipGetLastError();
+      sc.aux = nullptr;
       return KS_ERR_ALLOC;
     }
     sc.aux_bytes = want;
@@ -265,8 +318,9 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
       const size_t tab = sym_valu ? sym_table_doubles(p) + 32 : 0;
       double *partial = nullptr;
       unsigned *counters = nullptr;
-      int rc = get_scratch(s, (size_t)pl.nsplit * p.m + ncol + tab, (size_t)pl.row_blocks,
-                           &partial, &counters);
+      ScratchLease lease(s);
+      int rc = lease.partials((size_t)pl.nsplit * p.m + ncol + tab, (size_t)pl.row_blocks,
+                              &partial, &counters);
       if (rc != KS_OK) return rc;
       double *colp = partial + (size_t)pl.nsplit * p.m;
       Problem q = p;
@@ -291,8 +345,9 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
   const size_t tab = gather ? gather_table_doubles(p.n, p.k, p.type) + 32 : 0;
   double *partial = nullptr;
   unsigned *counters = nullptr;
+  ScratchLease lease(s);
   if (npart + tab > 0) {
-    int rc = get_scratch(s, npart + tab, (size_t)plan.row_blocks, &partial, &counters);
+    int rc = lease.partials(npart + tab, (size_t)plan.row_blocks, &partial, &counters);
     if (rc != KS_OK) return rc;
   }
   Problem q = p;

@@ -224,12 +224,6 @@ bool sym_eligible(const Problem &p) {
   return true;
 }
 
-// Six target rows per lane at k <= 4: more independent kernel-value chains per
-// source and fewer source loads per pair (N-body k = 3: Gaussian 38.5 -> 35.5 ms,
-// Laplace 38.0 -> 35.4 ms; eight rows lost occupancy: 38.3 and 42.3 ms); four
-// at k = 8, whose rows take twice the registers.
-int sym_rows_per_lane(int k) { return k <= 4 ? 6 : 4; }
-
 SymPlan plan_sym(int m, int k, int target_wgs) {
   return plan_sym_rows(m, target_wgs, SWG * sym_rows_per_lane(k));
 }
@@ -251,7 +245,7 @@ SymPlan plan_sym_rows(int m, int target_wgs, int rows_per_wg) {
 
 template <int TYPE, int K>
 static hipError_t sym_go(const SymArgs &A, const SymPlan &pl, hipStream_t s) {
-  constexpr int R = K <= 4 ? 6 : 4;  // sym_rows_per_lane(K)
+  constexpr int R = sym_rows_per_lane(K);
   hipLaunchKernelGGL((fused_f64_sym_kernel<TYPE, K, R>), dim3(pl.row_blocks, pl.nsplit), dim3(SWG),
                      0, s, A);
   return hipGetLastError();

@@ -43,8 +43,12 @@ struct SymPlan {
   int nsplit;
 };
 bool sym_eligible(const Problem &p);
-// rows per lane of the symmetric vector kernel at dimension k
-int sym_rows_per_lane(int k);
+// Rows per lane of the symmetric vector kernel at dimension k: six at k <= 4
+// (more independent kernel-value chains per source and fewer source loads per
+// pair; N-body k = 3: Gaussian 38.5 -> 35.5 ms, Laplace 38.0 -> 35.4 ms; eight
+// rows lost occupancy: 38.3 and 42.3 ms), four at k = 8, whose rows take twice
+// the registers.  The plan and the kernel template both take it from here.
+constexpr int sym_rows_per_lane(int k) { return k <= 4 ? 6 : 4; }
 SymPlan plan_sym(int m, int k, int target_wgs);
 SymPlan plan_sym_rows(int m, int target_wgs, int rows_per_wg);
 // u[umap[j]] += sum_{I < J} colp[I][j] over row blocks of B targets (J = j's block)

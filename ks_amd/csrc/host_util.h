@@ -14,11 +14,37 @@ void set_last_error(int code);
 int hip_failure(hipError_t e, const char *where);
 int ensure_device();
 hipStream_t thread_stream();
-int get_scratch(hipStream_t s, size_t partial_elems, size_t counter_elems, double **partial,
-                unsigned **counters);
-// Grow-only byte arena of one (device, stream), separate from the scratch above,
-// for front ends that re-lay out their inputs before a summation.
-int get_aux(hipStream_t s, size_t bytes, char **base);
+
+struct Scratch;
+// The grow-only scratch of one (device, stream), held from construction to
+// destruction.  An enqueue function declares the lease before it asks for
+// buffers and keeps it until its last launch is enqueued, so another host thread
+// on the same stream cannot grow (and free) a buffer between its hand-out and
+// the launch that reads it (the lock is recursive: nested enqueues on one
+// thread share it).
+//  * partials(): split-source partial sums (FP64) and the per-row-block
+//    counters, which start at zero and which every kernel leaves at zero.
+//  * aux(): a byte arena, separate from the partials, for front ends that
+//    re-lay out their inputs before a summation.
+// Growing synchronises the stream and frees the old buffer -- unless a stream
+// capture has handed out buffers of this (device, stream): a captured graph
+// keeps their addresses and may be replayed at any time, so they are retired
+// instead and live until the process ends.  Growing during a capture is an
+// error (KS_ERR_ALLOC, with ks_error_detail() saying so): run the call once on
+// the stream before capturing it.
+class ScratchLease {
+ public:
+  explicit ScratchLease(hipStream_t s);
+  ~ScratchLease();
+  ScratchLease(const ScratchLease &) = delete;
+  ScratchLease &operator=(const ScratchLease &) = delete;
+  int partials(size_t partial_elems, size_t counter_elems, double **partial, unsigned **counters);
+  int aux(size_t bytes, char **base);
+
+ private:
+  hipStream_t s_;
+  Scratch *sc_;
+};
 void laplace_params(int k, double *powe, double *scal);
 int make_params(const ks_t *ker, int k, KerParams &P, bool warn);
 
@@ -64,8 +90,8 @@ struct SingleProblem {
 // Widen the selected rows to FP64, sum with the FP64 (precision KS_PREC_FP64)
 // or mixed kernels, and add the row sums into the float u, on stream s.
 int enqueue_single(const SingleProblem &p, int precision, hipStream_t s);
-// The precision sgsks() uses: mixed BF16x3 for the Gaussian kernels at
-// 16 <= k <= 128 with m, n >= 16384, FP64 otherwise.
+// The precision sgsks() uses: FP64 (mixed precision only on request, through
+// sgsks_device).
 int single_default_precision(int type, int m, int n, int k);
 
 // Host entry behind dgsks()/dgsks_ref(): all-device pointers run in place,

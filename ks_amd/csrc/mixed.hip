@@ -802,7 +802,9 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   const size_t npart = (size_t)(plan.nsplit > 1 ? plan.nsplit : 0) * p.m;
   double *partial = nullptr;
   unsigned *counters = nullptr;
-  int rc = get_scratch(s, npart + (off + 255) / 8 + 32, (size_t)plan.row_blocks, &partial, &counters);
+  ScratchLease lease(s);
+  int rc = lease.partials(npart + (off + 255) / 8 + 32, (size_t)plan.row_blocks, &partial,
+                          &counters);
   if (rc != KS_OK) return rc;
   char *base = reinterpret_cast<char *>(partial + npart);
   base = (char *)(((uintptr_t)base + 255) & ~(uintptr_t)255);

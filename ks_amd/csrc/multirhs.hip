@@ -386,9 +386,10 @@ int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s) {
                         : plan_split(p.m, p.n, MR_WG, 2048, 128);
   double *partial = nullptr;
   unsigned *counters = nullptr;
+  ScratchLease lease(s);
   if (plan.nsplit > 1) {
-    int rc = get_scratch(s, (size_t)plan.nsplit * p.m * nrhs, (size_t)plan.row_blocks, &partial,
-                         &counters);
+    int rc = lease.partials((size_t)plan.nsplit * p.m * nrhs, (size_t)plan.row_blocks, &partial,
+                            &counters);
     if (rc != KS_OK) return rc;
   }
   MrhsArgs A;

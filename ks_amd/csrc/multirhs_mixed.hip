@@ -529,7 +529,8 @@ int enqueue_multirhs_mixed(const Problem &p, int nrhs, const double *centre, hip
   const size_t o_c = off; off += al(64 * 8);
   const size_t o_cp = off; off += al((size_t)centre_parts((long)m + n) * 64 * 8);
   char *base = nullptr;
-  int rc = get_aux(s, off, &base);
+  ScratchLease lease(s);
+  int rc = lease.aux(off, &base);
   if (rc != KS_OK) return rc;
   hipError_t e = hipSuccess;
   if (!centre) {
@@ -594,8 +595,8 @@ int enqueue_multirhs_mixed(const Problem &p, int nrhs, const double *centre, hip
   A.c2 = (float)(as * 2.01 * kU32);
   A.c0 = (float)(4.0 * kU32);
   if (nsplit > 1) {
-    rc = get_scratch(s, (size_t)nsplit * m * nrhs, (size_t)plan.row_blocks, &A.partial,
-                     &A.counters);
+    rc = lease.partials((size_t)nsplit * m * nrhs, (size_t)plan.row_blocks, &A.partial,
+                        &A.counters);
     if (rc != KS_OK) return rc;
   }
   switch (KC) {

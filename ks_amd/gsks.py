@@ -161,9 +161,9 @@ def dgsks_ref(d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, **options):
 def sgsks(d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, **options):
     """Single precision: float32 arrays (XA2 / XB2 may be None: norms computed
     in FP64), summed on the GPU after an exact widening to FP64.  Without
-    precision= the library picks mixed BF16x3 for the Gaussian kernels at
-    16 <= d <= 128 with at least 16384 targets and sources, FP64 otherwise;
-    precision= (device tensors) forces one."""
+    precision= the sum is FP64; precision= (device tensors) selects a mixed
+    form, whose accuracy depends on the data (1e-6 where near pairs carry the
+    sums, about 1e-5 for rows made only of far pairs)."""
     _call("sgsks", d, u, mu, XA, XA2, alpha, XB, XB2, beta, w, omega, options)
 
 

@@ -179,20 +179,57 @@ void orc_dgsks_list(int ntask, const orc_kernel *kers, int k, double *u, long nu
                     const long *off_b, const int *bmap, const int *wmap,
                     const double *XA, const double *XA2, const double *XB,
                     const double *XB2, const double *w, int nthreads) {
-  /* frame/omp_dgsks_list.cpp:330-414: contributions go to a zeroed buffer first,
-     then are merged into u. */
-  double *acc = (double *)calloc((size_t)nu, sizeof(double));
-  for (int t = 0; t < ntask; t++) {
-    long a0 = off_a[t], a1 = off_a[t + 1], b0 = off_b[t], b1 = off_b[t + 1];
-    int m = (int)(a1 - a0), n = (int)(b1 - b0);
-    if (m == 0 || n == 0) continue;
-    orc_kernel ker = kers[t];
-    if (ker.type == ORC_LAPLACE) orc_laplace_params(k, &ker.powe, &ker.scal);
-    orc_dgsks_ref(&ker, m, n, k, acc, umap + a0, XA, XA2, amap + a0, XB, XB2, bmap + b0, w,
-                  wmap + b0, nthreads);
+  /* frame/omp_dgsks_list.cpp:330-414: every task is one dgsks_ref into a zeroed
+     buffer, and the buffer is then added into u.  Here each task's row sums
+     (frame/dgsks_ref.c:437-444, each from 0) are computed in parallel over
+     chunks of rows of all tasks, then scattered through umap task by task, in
+     order, and the buffer is added into u: the same sums, in a fixed order, so
+     the result does not depend on the thread count or timing. */
+  if (nthreads < 1) nthreads = 1;
+  const long total_a = ntask > 0 ? off_a[ntask] : 0;
+  double *R = (double *)malloc(sizeof(double) * (size_t)(total_a > 0 ? total_a : 1));
+  enum { CH = 64 };
+  long nchunk = 0;
+  for (int t = 0; t < ntask; t++) nchunk += (off_a[t + 1] - off_a[t] + CH - 1) / CH;
+  int *ct = (int *)malloc(sizeof(int) * (size_t)(nchunk > 0 ? nchunk : 1));
+  long *c0 = (long *)malloc(sizeof(long) * (size_t)(nchunk > 0 ? nchunk : 1));
+  long c = 0;
+  for (int t = 0; t < ntask; t++)
+    for (long r = off_a[t]; r < off_a[t + 1]; r += CH) {
+      ct[c] = t;
+      c0[c++] = r;
+    }
+#pragma omp parallel num_threads(nthreads)
+  {
+    double *bt = (double *)malloc(sizeof(double) * (size_t)(k > 0 ? k : 1) * ORC_JB);
+#pragma omp for schedule(dynamic, 1)
+    for (long q = 0; q < nchunk; q++) {
+      const int t = ct[q];
+      const long b0 = off_b[t];
+      const int n = (int)(off_b[t + 1] - b0);
+      const long r1 = c0[q] + CH < off_a[t + 1] ? c0[q] + CH : off_a[t + 1];
+      orc_kernel ker = kers[t];
+      if (ker.type == ORC_LAPLACE) orc_laplace_params(k, &ker.powe, &ker.scal);
+      for (long r = c0[q]; r < r1; r++) {
+        const int ai = amap[r];
+        double acc = 0.0;
+        if (n > 0) {
+          double hi = ker.hi ? ker.hi[ai] : 0.0;
+          row_block(&ker, n, k, XA + (long)ai * k, XA2[ai], hi, XB, XB2, bmap + b0, w,
+                    wmap + b0, &acc, bt);
+        }
+        R[r] = acc;
+      }
+    }
+    free(bt);
   }
-  for (long x = 0; x < nu; x++) u[x] += acc[x];
-  free(acc);
+  double *buf = (double *)calloc((size_t)nu, sizeof(double));
+  for (long r = 0; r < total_a; r++) buf[umap[r]] += R[r];
+  for (long x = 0; x < nu; x++) u[x] += buf[x];
+  free(buf);
+  free(R);
+  free(ct);
+  free(c0);
 }
 
 void orc_gauss_grad(int m, int n, int k, const double *XA, const double *XB,

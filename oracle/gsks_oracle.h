@@ -91,7 +91,9 @@ void orc_dgsks_ref(orc_kernel *ker, int m, int n, int k,
  * task: u[x] += sum over tasks t of the task's contributions to row x.  Task t
  * uses the index ranges [off_a[t], off_a[t+1]) of amap/umap and
  * [off_b[t], off_b[t+1]) of bmap/wmap.  Laplace parameters are applied to a
- * copy of each task's descriptor. */
+ * copy of each task's descriptor.  As in the reference, tasks are dealt to
+ * nthreads workers by least load (deterministically), each worker sums into a
+ * u_local of nu doubles, and the u_local are merged into u in worker order. */
 void orc_dgsks_list(int ntask, const orc_kernel *kers, int k, double *u, long nu,
                     const long *off_a, const int *umap, const int *amap,
                     const long *off_b, const int *bmap, const int *wmap,

@@ -31,11 +31,13 @@ def L():
     return _lib.lib()
 
 
-def _batch(k, seed, pool=9000, ntask=36, big_gauss=0, big=4200):
+def _batch(k, seed, pool=9000, ntask=36, big_gauss=0, big=4200, sep_maps=False):
     """Tasks over one pool: Gaussian / Laplace / polynomial kernels, shared
     targets, an empty task, sizes across the tile slices; `big_gauss` extra
     Gaussian tasks of big x big (16384 x 16384 reaches the mixed kernels in a
-    mixed-precision batch)."""
+    mixed-precision batch).  sep_maps: umap and wmap are other permutations of
+    the pool than amap and bmap (umap[i] = pu[amap[i]], wmap[j] = pw[bmap[j]]),
+    so a gather that reads the wrong map, or a scatter through amap, shows."""
     from ks_amd import _lib
     rng = np.random.default_rng(seed)
     X = rng.random((pool, k)) * (1.5 / np.sqrt(k))
@@ -59,8 +61,13 @@ def _batch(k, seed, pool=9000, ntask=36, big_gauss=0, big=4200):
         kernels[t].scal = kd.get("scal", 0.0)
         kernels[t].cons = kd.get("cons", 0.0)
         kernels[t].powe = kd.get("powe", 0.0)
-    return dict(k=k, X=X, X2=X2, w=w, amap=amap, bmap=bmap, a_off=a_off, b_off=b_off,
-                kernels=kernels, kernels_py=kernels_py, ntask=nt, pool=pool)
+    umap, wmap = amap, bmap
+    if sep_maps:
+        pu = rng.permutation(pool).astype(np.int32)
+        pw = rng.permutation(pool).astype(np.int32)
+        umap, wmap = pu[amap], pw[bmap]
+    return dict(k=k, X=X, X2=X2, w=w, amap=amap, bmap=bmap, umap=umap, wmap=wmap, a_off=a_off,
+                b_off=b_off, kernels=kernels, kernels_py=kernels_py, ntask=nt, pool=pool)
 
 
 class _Plan:
@@ -70,13 +77,14 @@ class _Plan:
         self.L, self.torch = L, torch
         dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
         self.amap, self.bmap = dev(B["amap"]), dev(B["bmap"])
+        self.umap, self.wmap = dev(B["umap"]), dev(B["wmap"])
         self.X, self.X2, self.w = dev(B["X"]), dev(B["X2"]), dev(B["w"])
         self.h = ctypes.c_void_p()
         P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
         hp = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
         rc = L.ks_list_plan_create(ctypes.byref(self.h), B["ntask"], B["kernels"], B["k"],
-                                   hp(B["a_off"]), P(self.amap), P(self.amap), hp(B["b_off"]),
-                                   P(self.bmap), P(self.bmap), precision)
+                                   hp(B["a_off"]), P(self.umap), P(self.amap), hp(B["b_off"]),
+                                   P(self.bmap), P(self.wmap), precision)
         assert rc == 0, rc
 
     def run(self, u, stream):
@@ -90,15 +98,19 @@ class _Plan:
 
 def _oracle_list(oracle, B, u0):
     ref = u0.copy()
-    oracle.dgsks_list(B["kernels_py"], B["k"], ref, B["a_off"], B["amap"], B["amap"], B["b_off"],
-                      B["bmap"], B["bmap"], B["X"], B["X2"], B["X"], B["X2"], B["w"])
+    oracle.dgsks_list(B["kernels_py"], B["k"], ref, B["a_off"], B["umap"], B["amap"], B["b_off"],
+                      B["bmap"], B["wmap"], B["X"], B["X2"], B["X"], B["X2"], B["w"])
     return ref
 
 
-@pytest.mark.parametrize("k", [3, 20])
-def test_plan_matches_oracle_and_dgsks_list(L, torch_cuda, oracle, k):
+@pytest.mark.parametrize("k,sep_maps", [(3, False), (20, False), (1, True), (2, True),
+                                        (3, True), (5, True), (7, True), (20, True)])
+def test_plan_matches_oracle_and_dgsks_list(L, torch_cuda, oracle, k, sep_maps):
+    """k < 8 runs the compact source pool (templated gathers for k = 1..7),
+    k = 20 the padded row tiles or matrix-core tiles; with sep_maps the target
+    and weight maps differ from the point maps."""
     torch = torch_cuda
-    B = _batch(k, 400 + k)
+    B = _batch(k, 400 + k, sep_maps=sep_maps)
     rng = np.random.default_rng(1)
     u0 = rng.standard_normal(B["pool"])
     ref = _oracle_list(oracle, B, u0)
@@ -110,14 +122,14 @@ def test_plan_matches_oracle_and_dgsks_list(L, torch_cuda, oracle, k):
         torch.cuda.synchronize()
         got = u.cpu().numpy()
         err = oracle.rel_l2(got, ref)
-        print(f"plan k={k}: rel L2 = {err:.3e}")
+        print(f"plan k={k} sep_maps={sep_maps}: rel L2 = {err:.3e}")
         assert err <= TOL
         # the one-shot API lays out the same tiles, so it agrees bit for bit
         u1 = u0.copy()
         p = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
         kern = B["kernels"]
-        assert L.dgsks_list(B["ntask"], kern, k, p(u1), u1.size, p(B["a_off"]), p(B["amap"]),
-                            p(B["amap"]), p(B["b_off"]), p(B["bmap"]), p(B["bmap"]), p(B["X"]),
+        assert L.dgsks_list(B["ntask"], kern, k, p(u1), u1.size, p(B["a_off"]), p(B["umap"]),
+                            p(B["amap"]), p(B["b_off"]), p(B["bmap"]), p(B["wmap"]), p(B["X"]),
                             p(B["X2"]), p(B["X"]), p(B["X2"]), p(B["w"]), 0) == 0
         assert np.array_equal(u1, got)
     finally:
@@ -148,10 +160,12 @@ def _replay_check(torch, run_on, u, reps=4):
     return eager
 
 
-@pytest.mark.parametrize("k", [8, 32])
+@pytest.mark.parametrize("k", [3, 8, 32])
 def test_plan_graph_capture_replays(L, torch_cuda, oracle, k):
+    """k = 3 captures the compact-pool gather too; separate target and weight
+    maps."""
     torch = torch_cuda
-    B = _batch(k, 500 + k, ntask=24)
+    B = _batch(k, 500 + k, ntask=24, sep_maps=True)
     plan = _Plan(L, torch, B)
     try:
         u = torch.zeros(B["pool"], dtype=torch.float64, device="cuda")
@@ -383,3 +397,85 @@ def test_mixed_batch_routing(L, torch_cuda, oracle):
     err = oracle.rel_l2(u, ref)
     print(f"mixed batch: rel L2 = {err:.3e}")
     assert err <= 1e-6
+
+
+def test_scratch_growth_after_capture_keeps_graph_valid(L, torch_cuda, oracle):
+    """A graph captured from dgsks_device keeps the addresses of its stream's
+    scratch.  A later, larger call on the same stream grows that scratch: the
+    captured buffers must stay valid (retired, not freed), so replays still give
+    the eager result, and memory allocated after the growth is not written by
+    the replays."""
+    from ks_amd import _lib
+    torch = torch_cuda
+    rng = np.random.default_rng(77)
+    k = 16
+    N1, N2 = 3000, 60000  # the larger call needs more split partials
+    X = rng.random((N2, k))
+    X2 = oracle.sqnorm(X)
+    w = rng.standard_normal(N2)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    dX, dX2, dw = dev(X), dev(X2), dev(w)
+    ids = dev(np.arange(N2, dtype=np.int32))
+    kt = _lib.KsT()
+    kt.type = 0
+    kt.scal = -2.0
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    u1 = torch.zeros(N2, dtype=torch.float64, device="cuda")
+    u2 = torch.zeros(N2, dtype=torch.float64, device="cuda")
+
+    def call(n, u, s):
+        return L.dgsks_device(ctypes.byref(kt), n, n, k, P(u), P(ids), P(dX), P(dX2), P(ids),
+                              P(dX), P(dX2), P(ids), P(dw), P(ids), 0,
+                              ctypes.c_void_p(s.cuda_stream))
+
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        assert call(N1, u1, s) == 0  # sizes the scratch for N1
+    torch.cuda.synchronize()
+    eager = u1.clone()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        assert call(N1, u1, s) == 0
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        assert call(N2, u2, s) == 0  # grows the scratch of stream s
+    torch.cuda.synchronize()
+    rows = np.arange(0, N2, 97).astype(np.int32)
+    ref2 = oracle.dgsks_rows(0, X, X2, np.arange(N2, dtype=np.int32), X, X2,
+                             np.arange(N2, dtype=np.int32), w, np.arange(N2, dtype=np.int32),
+                             np.zeros(N2), np.arange(N2, dtype=np.int32), rows, scal=-2.0)
+    assert oracle.rel_l2(u2.cpu().numpy()[rows], ref2) <= TOL
+    fresh = [torch.full((1 << 20,), 3.0, dtype=torch.float64, device="cuda") for _ in range(8)]
+    for r in range(3):
+        u1.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(u1, eager), f"replay {r} after the scratch grew differs"
+    assert all(bool(torch.all(f == 3.0)) for f in fresh)
+
+
+def test_mixed_plan_capture(L, torch_cuda, oracle):
+    """A mixed-precision plan with tasks on the mixed kernels: capturing it on a
+    stream whose scratch has not grown yet is refused with KS_ERR_ALLOC (no
+    allocation or synchronisation under capture); after one eager run on that
+    stream it captures, and every replay equals the eager run."""
+    from ks_amd import _lib
+    torch = torch_cuda
+    B = _batch(16, 610, pool=20000, ntask=12, big_gauss=1, big=16384, sep_maps=True)
+    plan = _Plan(L, torch, B, precision=2)
+    try:
+        u = torch.zeros(B["pool"], dtype=torch.float64, device="cuda")
+        s0 = torch.cuda.Stream()
+        g0 = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g0, stream=s0):
+            rc = plan.run(u, s0)
+        assert rc == _lib.KS_ERR_ALLOC, rc
+        del g0
+        eager = _replay_check(torch, lambda s: plan.run(u, s), u)
+        ref = _oracle_list(oracle, B, np.zeros(B["pool"]))
+        err = oracle.rel_l2(eager.cpu().numpy(), ref)
+        print(f"mixed plan (captured): rel L2 = {err:.3e}")
+        assert err <= 1e-6
+    finally:
+        plan.close()

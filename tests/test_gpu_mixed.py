@@ -122,11 +122,33 @@ def test_far_only_rows_and_skip_cutoff(oracle, torch_cuda, prec, x2):
 
 
 def test_unsupported_type_reports(torch_cuda):
+    """The mixed forms cover the Gaussian kernels only: tanh at mixed precision
+    returns KS_ERR_INVALID_ARG (error mode RETURN), records it for
+    ks_last_error() and leaves u untouched."""
     from ks_amd import _lib
+    torch = torch_cuda
+    L = _lib.lib()
     rng = np.random.default_rng(0)
     X = rng.random((100, 8))
     X2 = (X * X).sum(1)
     idx = np.arange(100, dtype=np.int32)
-    with pytest.raises(AssertionError):
-        _run(torch_cuda, 4, 2, X, X2, idx, idx, np.ones(100), scal=0.1)
-    assert _lib.lib().ks_last_error() == 0 or True
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    dX, dX2, di, dw = map(dev, (X, X2, idx, np.ones(100)))
+    du = torch.full((100,), 7.0, dtype=torch.float64, device="cuda")
+    kt = _lib.KsT()
+    kt.type = 4
+    kt.scal = 0.1
+    P = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+    mode = L.ks_get_error_mode()
+    L.ks_set_error_mode(_lib.KS_ERRMODE_RETURN)
+    try:
+        L.ks_last_error()  # clear
+        rc = L.dgsks_device(ctypes.byref(kt), 100, 100, 8, P(du), P(di), P(dX), P(dX2), P(di),
+                            P(dX), P(dX2), P(di), P(dw), P(di), 2,
+                            ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        assert rc == _lib.KS_ERR_INVALID_ARG
+        assert L.ks_last_error() == _lib.KS_ERR_INVALID_ARG
+    finally:
+        L.ks_set_error_mode(mode)
+    torch.cuda.synchronize()
+    assert torch.all(du == 7.0)

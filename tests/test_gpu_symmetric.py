@@ -63,7 +63,7 @@ def _run(L, torch, kt, N, k, u, umap, X, X2, amap, bmap, w, wmap):
     torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("k", [3, 8, 16, 40, 64])
+@pytest.mark.parametrize("k", [2, 3, 4, 8, 16, 40, 64])
 @pytest.mark.parametrize("t", list(TYPES))
 def test_symmetric_all_types(L, torch_cuda, oracle, t, k):
     """k <= 8 runs the vector kernel, 9 <= k <= 64 the matrix-core one (not
