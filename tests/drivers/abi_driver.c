@@ -50,13 +50,18 @@ static void shuffle(int *x, int n) {
   }
 }
 
+/* relative L2 of x against ref over the entries idx (test/test_dgsks.c:43-79),
+   both scaled by max |ref| first: Laplace at large k reaches ~1e230, whose
+   squares would overflow */
 static double rel_l2(const double *x, const double *ref, int n, const int *idx) {
-  double num = 0.0, den = 0.0;
+  double sc = 0.0, num = 0.0, den = 0.0;
+  for (int i = 0; i < n; ++i) sc = fmax(sc, fabs(ref[idx ? idx[i] : i]));
+  if (sc == 0.0 || !isfinite(sc)) sc = 1.0;
   for (int i = 0; i < n; ++i) {
     int r = idx ? idx[i] : i;
-    double d = x[r] - ref[r];
+    double d = x[r] / sc - ref[r] / sc;
     num += d * d;
-    den += ref[r] * ref[r];
+    den += (ref[r] / sc) * (ref[r] / sc);
   }
   return den > 0.0 ? sqrt(num / den) : sqrt(num);
 }

@@ -18,7 +18,7 @@ BIN = os.path.join(HERE, "drivers", "bin")
 def _run(args, timeout=300):
     exe = os.path.join(BIN, args[0])
     assert os.path.exists(exe), f"{exe} is not built: run build() (tests/drivers/Makefile)"
-    print("running", " ".join(args), flush=True)
+    print("running", " ".join(str(a) for a in args), flush=True)
     r = subprocess.run([exe] + [str(a) for a in args[1:]], capture_output=True, text=True,
                        timeout=timeout)
     print(r.stdout[-6000:])

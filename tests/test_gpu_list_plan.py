@@ -48,7 +48,10 @@ def _batch(k, seed, pool=9000, ntask=36, big_gauss=0, big=4200, sep_maps=False):
     ms = list(rng.integers(1, 300, ntask)) + [big] * big_gauss
     ns = list(rng.integers(1, 2600, ntask)) + [big] * big_gauss
     ms[2] = 0
-    kinds = [dict(type=0, scal=-1.5), dict(type=2), dict(type=1, scal=1.0 / k, cons=1.0, powe=2.0)]
+    # Laplace needs k >= 3 (frame/dgsks.c:445-447 divides by k - 2): a Gaussian of
+    # another width takes its place below that
+    lap = dict(type=2) if k >= 3 else dict(type=0, scal=-0.7)
+    kinds = [dict(type=0, scal=-1.5), lap, dict(type=1, scal=1.0 / k, cons=1.0, powe=2.0)]
     kernels_py = [kinds[t % 3] for t in range(ntask)] + [dict(type=0, scal=-40.0)] * big_gauss
     amap = np.concatenate([rng.choice(pool, int(s), replace=False).astype(np.int32) for s in ms])
     bmap = np.concatenate([rng.choice(pool, int(s), replace=False).astype(np.int32) for s in ns])

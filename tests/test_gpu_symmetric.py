@@ -68,6 +68,8 @@ def _run(L, torch, kt, N, k, u, umap, X, X2, amap, bmap, w, wmap):
 def test_symmetric_all_types(L, torch_cuda, oracle, t, k):
     """k <= 8 runs the vector kernel, 9 <= k <= 64 the matrix-core one (not
     Laplace, which takes the general path there)."""
+    if t == 2 and k < 3:
+        pytest.skip("Laplace is defined for k >= 3 (frame/dgsks.c:445-447 divides by k - 2)")
     torch = torch_cuda
     N = 5000  # not a multiple of the 1024-row blocks
     X, X2, w, amap, umap, wmap, h = _problem(N, k, 700 + 10 * t + k)
