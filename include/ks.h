@@ -169,6 +169,24 @@ int dgsks_device(const ks_t *kernel, int m, int n, int k,
                  const double *XB, const double *XB2, const int *bmap,
                  const double *w, const int *wmap, int precision, void *stream);
 
+/* dgsks_device without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The arguments and
+ * results are dgsks_device's with precision KS_PREC_FP64 (any other precision
+ * returns KS_ERR_INVALID_ARG); the same FP64 kernels run.  Their scratch
+ * (split partial sums, counters, compact source tables) lives in a static
+ * 64 MiB device array that every GPU the library runs on holds until the
+ * process ends; a summation that needs more is refused with
+ * KS_ERR_INVALID_ARG before anything is enqueued.  The call allocates nothing
+ * and does not synchronise.  All calls on one GPU share that array, which keeps
+ * the last call's contents: calls that overlap on one GPU (other streams or
+ * threads) race and must be ordered by the caller. */
+int dgsks_device_noalloc(const ks_t *kernel, int m, int n, int k,
+                         double *u, const int *umap,
+                         const double *XA, const double *XA2, const int *amap,
+                         const double *XB, const double *XB2, const int *bmap,
+                         const double *w, const int *wmap, int precision, void *stream);
+
 /* Single precision.  The reference declares a single-precision micro-kernel
  * for every kernel type and implements none of them
  * (micro_kernel/x86_64/haswell/gaussian_int_d8x6.c:38-54: "not yet

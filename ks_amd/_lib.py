@@ -48,6 +48,8 @@ _SIGS = {
     "ks_get_error_mode": (ctypes.c_int, []),
     "dgsks_device": (ctypes.c_int, [ctypes.POINTER(KsT), ctypes.c_int, ctypes.c_int, ctypes.c_int] +
                      [_vp] * 10 + [ctypes.c_int, _vp]),
+    "dgsks_device_noalloc": (ctypes.c_int, [ctypes.POINTER(KsT), ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int] + [_vp] * 10 + [ctypes.c_int, _vp]),
     "sgsks_device": (ctypes.c_int, [ctypes.POINTER(KsT), ctypes.c_int, ctypes.c_int, ctypes.c_int] +
                      [_vp] * 10 + [ctypes.c_int, _vp]),
     "sgsks": (None, [ctypes.POINTER(KsT), ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_vp] * 10),

@@ -16,7 +16,7 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["fused_f64.hip", "fused_f64_sym.hip", "fused_f64_mfma.hip", "mixed.hip", "multirhs.hip", "multirhs_mixed.hip", "ref_f64.hip", "norms.hip", "list.hip", "sgsks.hip",
+SOURCES = ["fused_f64.hip", "fused_f64_sym.hip", "fused_f64_mfma.hip", "mixed.hip", "multirhs.hip", "multirhs_mixed.hip", "ref_f64.hip", "norms.hip", "list.hip", "sgsks.hip", "noalloc.hip",
            "api.cpp", "staging.cpp", "list_api.cpp", "list_tune.cpp", "omp_dgsks_list.cpp"]
 
 # Packed FP32 instructions are disabled for all device code: some of their

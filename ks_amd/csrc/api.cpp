@@ -13,6 +13,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <optional>
 #include <utility>
 #include <vector>
 
@@ -304,7 +305,39 @@ constexpr int kGatherMinRows = 1024;
 // symmetric sums keep one column partial per (row block, source): at most 4 GiB
 constexpr size_t kSymMaxColPartials = (size_t)1 << 29;
 
-int enqueue_fused(const Problem &p, hipStream_t s) {
+size_t fixed_scratch_need(size_t partial_elems, size_t counter_elems) {
+  return ((counter_elems * sizeof(unsigned) + 255) & ~(size_t)255) + partial_elems * sizeof(double);
+}
+
+// Hands out enqueue_fused's partial sums and counters: from the stream's
+// scratch (held by a lease until the caller's launches are enqueued), or carved
+// from a caller-owned fixed buffer, whose counters are cleared first.
+class FusedScratch {
+ public:
+  FusedScratch(hipStream_t s, const FixedScratch *fixed) : s_(s), fixed_(fixed) {
+    if (!fixed_) lease_.emplace(s);
+  }
+  bool fits(size_t partial_elems, size_t counter_elems) const {
+    return !fixed_ || fixed_scratch_need(partial_elems, counter_elems) <= fixed_->bytes;
+  }
+  int get(size_t partial_elems, size_t counter_elems, double **partial, unsigned **counters) {
+    if (lease_) return lease_->partials(partial_elems, counter_elems, partial, counters);
+    if (!fits(partial_elems, counter_elems)) return KS_ERR_INVALID_ARG;
+    *counters = reinterpret_cast<unsigned *>(fixed_->base);
+    *partial = reinterpret_cast<double *>(
+        fixed_->base + fixed_scratch_need(0, counter_elems));
+    hipError_t e = launch_zero_u32(*counters, (long)counter_elems, s_);
+    return e == hipSuccess ? KS_OK : hip_failure(e, "counter clear launch");
+  }
+
+ private:
+  hipStream_t s_;
+  const FixedScratch *fixed_;
+  std::optional<ScratchLease> lease_;
+};
+
+int enqueue_fused(const Problem &p, hipStream_t s, const FixedScratch *fixed) {
+  FusedScratch scratch(s, fixed);
   // sources are the targets: each pair of different row blocks evaluated once,
   // on the vector units (k <= 8) or the matrix cores (9 <= k <= 64, not Laplace)
   const bool sym_valu = sym_eligible(p), sym_mfma = !sym_valu && sym_mfma_eligible(p);
@@ -312,15 +345,15 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
     const SymPlan pl = sym_valu ? plan_sym(p.m, p.k, kTargetWGs)
                                 : plan_sym_rows(p.m, kTargetWGs, sym_mfma_rows());
     const size_t ncol = (size_t)pl.row_blocks * p.m;
-    if (ncol <= kSymMaxColPartials) {
-      // the vector form runs over compact tables gathered beside its partials
-      // (N-body k = 3: the source loads stop being a gather through amap)
-      const size_t tab = sym_valu ? sym_table_doubles(p) + 32 : 0;
+    // the vector form runs over compact tables gathered beside its partials
+    // (N-body k = 3: the source loads stop being a gather through amap)
+    const size_t tab = sym_valu ? sym_table_doubles(p) + 32 : 0;
+    const size_t need = (size_t)pl.nsplit * p.m + ncol + tab;
+    // a fixed buffer too small for the column partials takes the general path
+    if (ncol <= kSymMaxColPartials && scratch.fits(need, (size_t)pl.row_blocks)) {
       double *partial = nullptr;
       unsigned *counters = nullptr;
-      ScratchLease lease(s);
-      int rc = lease.partials((size_t)pl.nsplit * p.m + ncol + tab, (size_t)pl.row_blocks,
-                              &partial, &counters);
+      int rc = scratch.get(need, (size_t)pl.row_blocks, &partial, &counters);
       if (rc != KS_OK) return rc;
       double *colp = partial + (size_t)pl.nsplit * p.m;
       Problem q = p;
@@ -345,9 +378,8 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
   const size_t tab = gather ? gather_table_doubles(p.n, p.k, p.type) + 32 : 0;
   double *partial = nullptr;
   unsigned *counters = nullptr;
-  ScratchLease lease(s);
   if (npart + tab > 0) {
-    int rc = lease.partials(npart + tab, (size_t)plan.row_blocks, &partial, &counters);
+    int rc = scratch.get(npart + tab, (size_t)plan.row_blocks, &partial, &counters);
     if (rc != KS_OK) return rc;
   }
   Problem q = p;
@@ -360,6 +392,54 @@ int enqueue_fused(const Problem &p, hipStream_t s) {
   hipError_t e = mfma ? launch_fused_f64_mfma(p, plan, partial, counters, s)
                       : launch_fused_f64(q, plan, partial, counters, s);
   return e == hipSuccess ? KS_OK : hip_failure(e, mfma ? "fused_f64_mfma launch" : "fused_f64 launch");
+}
+
+int device_problem(const char *name, const ks_t *kernel, int m, int n, int k, double *u,
+                   const int *umap, const double *XA, const double *XA2, const int *amap,
+                   const double *XB, const double *XB2, const int *bmap, const double *w,
+                   const int *wmap, Problem &p, bool &empty) {
+  empty = false;
+  if (!kernel || m < 0 || n < 0 || k < 0)
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): invalid argument\n", name);
+  if (m == 0 || n == 0 || k == 0) {
+    empty = true;
+    return KS_OK;
+  }
+  if (!u || !umap || !XA || !XA2 || !amap || !XB || !XB2 || !bmap || !w || !wmap)
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): NULL array\n", name);
+  int rc = ensure_device();
+  if (rc != KS_OK) return raise_error(rc, false, "%s(): no GPU available\n", name);
+  rc = make_params(kernel, k, p.P, true);
+  if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return rc;
+  p.m = m;
+  p.n = n;
+  p.k = k;
+  p.type = (int)kernel->type;
+  p.u = u;
+  p.umap = umap;
+  p.XA = XA;
+  p.XA2 = XA2;
+  p.amap = amap;
+  p.XB = XB;
+  p.XB2 = XB2;
+  p.bmap = bmap;
+  p.w = w;
+  p.wmap = wmap;
+  p.hi = kernel->hi;
+  p.hj = kernel->hj;
+  // A host pointer handed to a kernel would fault the GPU: check every array once
+  // (results are cached per thread for repeated calls with the same buffers).
+  const void *arrays[] = {u, umap, XA, XA2, amap, XB, XB2, bmap, w, wmap,
+                          p.type == KS_GAUSSIAN_VAR_BANDWIDTH ? p.hi : nullptr,
+                          p.type == KS_GAUSSIAN_VAR_BANDWIDTH ? p.hj : nullptr};
+  for (const void *a : arrays)
+    if (a && !device_ptr_cached(a))
+      return raise_error(KS_ERR_MIXED_POINTERS, false, "%s(): every array must be device memory\n",
+                         name);
+  if (p.type == KS_GAUSSIAN_VAR_BANDWIDTH && (!p.hi || !p.hj))
+    return raise_error(KS_ERR_NO_BANDWIDTH, false, "%s(): %s\n", name,
+                       err_string(KS_ERR_NO_BANDWIDTH));
+  return KS_OK;
 }
 
 }  // namespace gsks
@@ -400,44 +480,11 @@ int dgsks_device(const ks_t *kernel, int m, int n, int k, double *u, const int *
                  const double *XA, const double *XA2, const int *amap, const double *XB,
                  const double *XB2, const int *bmap, const double *w, const int *wmap,
                  int precision, void *stream) {
-  if (!kernel || m < 0 || n < 0 || k < 0)
-    return raise_error(KS_ERR_INVALID_ARG, false, "dgsks_device(): invalid argument\n");
-  if (m == 0 || n == 0 || k == 0) return KS_OK;
-  if (!u || !umap || !XA || !XA2 || !amap || !XB || !XB2 || !bmap || !w || !wmap)
-    return raise_error(KS_ERR_INVALID_ARG, false, "dgsks_device(): NULL array\n");
-  int rc = ensure_device();
-  if (rc != KS_OK) return raise_error(rc, false, "dgsks_device(): no GPU available\n");
   Problem p;
-  rc = make_params(kernel, k, p.P, true);
-  if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return rc;
-  p.m = m;
-  p.n = n;
-  p.k = k;
-  p.type = (int)kernel->type;
-  p.u = u;
-  p.umap = umap;
-  p.XA = XA;
-  p.XA2 = XA2;
-  p.amap = amap;
-  p.XB = XB;
-  p.XB2 = XB2;
-  p.bmap = bmap;
-  p.w = w;
-  p.wmap = wmap;
-  p.hi = kernel->hi;
-  p.hj = kernel->hj;
-  // A host pointer handed to a kernel would fault the GPU: check every array once
-  // (results are cached per thread for repeated calls with the same buffers).
-  const void *arrays[] = {u, umap, XA, XA2, amap, XB, XB2, bmap, w, wmap,
-                          p.type == KS_GAUSSIAN_VAR_BANDWIDTH ? p.hi : nullptr,
-                          p.type == KS_GAUSSIAN_VAR_BANDWIDTH ? p.hj : nullptr};
-  for (const void *a : arrays)
-    if (a && !device_ptr_cached(a))
-      return raise_error(KS_ERR_MIXED_POINTERS, false,
-                         "dgsks_device(): every array must be device memory\n");
-  if (p.type == KS_GAUSSIAN_VAR_BANDWIDTH && (!p.hi || !p.hj))
-    return raise_error(KS_ERR_NO_BANDWIDTH, false, "dgsks_device(): %s\n",
-                       err_string(KS_ERR_NO_BANDWIDTH));
+  bool empty = false;
+  int rc = device_problem("dgsks_device", kernel, m, n, k, u, umap, XA, XA2, amap, XB, XB2, bmap,
+                          w, wmap, p, empty);
+  if (rc != KS_OK || empty) return rc;
   hipStream_t s = (hipStream_t)stream;  // NULL: the HIP null stream (torch's default stream)
   int r;
   if (precision == KS_PREC_FP64) {

@@ -66,6 +66,9 @@ hipError_t launch_sym_gather(const Problem &p, double *tab, Problem *q, hipStrea
 size_t gather_table_doubles(long count, int k, int type);
 hipError_t launch_source_gather(const Problem &p, double *tab, Problem *q, hipStream_t s);
 
+// x[0..n) = 0 on stream s with a kernel (no memset node in a captured graph).
+hipError_t launch_zero_u32(unsigned *x, long n, hipStream_t s);
+
 // FP64 cross term on the matrix cores (fused_f64_mfma.hip)
 bool mfma_f64_eligible(int type, int k);
 // The single-summation matrix-core kernel: every type, Laplace included.

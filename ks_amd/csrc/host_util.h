@@ -48,8 +48,29 @@ class ScratchLease {
 void laplace_params(int k, double *powe, double *scal);
 int make_params(const ks_t *ker, int k, KerParams &P, bool warn);
 
-// Fused FP64 summation of one problem (all device pointers) on stream s.
-int enqueue_fused(const Problem &p, hipStream_t s);
+// A device buffer owned by the caller of enqueue_fused, used instead of the
+// stream's grow-only scratch: the per-row-block counters at its start (cleared
+// by a kernel on the stream before every use), then the partial sums and
+// tables.  A summation whose scratch does not fit is refused with
+// KS_ERR_INVALID_ARG before anything is enqueued.
+struct FixedScratch {
+  char *base;
+  size_t bytes;
+};
+// Bytes of a FixedScratch that partial_elems doubles and counter_elems counters take.
+size_t fixed_scratch_need(size_t partial_elems, size_t counter_elems);
+
+// The checks of the stream-ordered entry points (sizes, NULL arrays, a GPU,
+// kernel parameters, device pointers, bandwidths), raising the error they find;
+// fills p, or sets empty when m, n or k is zero (nothing to do).
+int device_problem(const char *name, const ks_t *kernel, int m, int n, int k, double *u,
+                   const int *umap, const double *XA, const double *XA2, const int *amap,
+                   const double *XB, const double *XB2, const int *bmap, const double *w,
+                   const int *wmap, Problem &p, bool &empty);
+
+// Fused FP64 summation of one problem (all device pointers) on stream s, with
+// its scratch from the stream's scratch, or from `fixed` when it is given.
+int enqueue_fused(const Problem &p, hipStream_t s, const FixedScratch *fixed = nullptr);
 // The reference algorithm (unfused: cross-term panel, elementwise kernel, GEMV).
 int enqueue_reference(const Problem &p, hipStream_t s);
 // Multi-right-hand-side summation: p.u / p.w are [npoints][nrhs] matrices.

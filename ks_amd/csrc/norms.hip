@@ -2,6 +2,7 @@
 // right like the reference's norm precompute (frame/omp_dgsks_list.cpp:75-81).
 #include <hip/hip_runtime.h>
 
+#include "gsks_launch.h"
 #include "host_util.h"
 
 #pragma clang fp contract(off)
@@ -15,6 +16,17 @@ __global__ void sqnorm_kernel(const double *X, long nx, int k, double *X2) {
   double t = 0.0;
   for (int p = 0; p < k; ++p) t += x[p] * x[p];
   X2[i] = t;
+}
+
+__global__ void zero_u32_kernel(unsigned *x, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0u;
+}
+
+hipError_t launch_zero_u32(unsigned *x, long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n);
+  return hipGetLastError();
 }
 
 }  // namespace gsks
