@@ -69,6 +69,15 @@ hipError_t launch_source_gather(const Problem &p, double *tab, Problem *q, hipSt
 // x[0..n) = 0 on stream s with a kernel (no memset node in a captured graph).
 hipError_t launch_zero_u32(unsigned *x, long n, hipStream_t s);
 
+// u[umap[r]] += sum over sp < nsplit of partial[sp][i] (added in split order),
+// r = prow ? prow[i] : i, for every i < m: the split-source reduction of the
+// fused kernels, in a launch of its own.  The launch boundary orders the
+// partials' stores before these loads, so the kernels that write them need no
+// device-scope fences (which write back and invalidate the XCD's L2 on gfx950)
+// and no counters.
+hipError_t launch_split_reduce(const double *partial, int nsplit, int m, double *u,
+                               const int *umap, const int *prow, hipStream_t s);
+
 // FP64 cross term on the matrix cores (fused_f64_mfma.hip)
 bool mfma_f64_eligible(int type, int k);
 // The single-summation matrix-core kernel: every type, Laplace included.

@@ -78,8 +78,26 @@ constexpr bool kSkipTier = PREC == KS_PREC_MIXED_BF16X3 && TYPE == GAUSSIAN;
 
 // ------------------------------------------------------------------ pre-pass
 
+// Pruning bounds of the skip tier (mix_bounds_kernel): Vt[T][i] and Vs[i][S]
+// per pivot i and 16-point target block T / source block S, the bucket range
+// (first | last << 16) of every block, and the pivot distances
+// pp[A][B] = |p_A - p_B|^2 rounded up.
+struct MixBounds {
+  const float *Vt, *Vs, *pp;
+  const unsigned *rt, *rs;
+  int ntb, nsb;  // target and source blocks in the tables
+  float thr_d2;  // a block pair whose d^2 lower bound exceeds this is negligible
+};
+
 struct MixPack {
+  // Which source blocks each wave's row group needs (bucketed BF16x3 Gaussian
+  // with a decaying kernel; null: every block): word (g nsplit + split) ngw + gi
+  // has bit b set when block 64 gi + b of slice `split` may hold a pair with
+  // K >= 2^-60 for row group g (32 sorted targets), mix_need_kernel.
+  const unsigned long long *need;
+  int ngw;          // mask words per (row group, slice)
   uint4 *Ap;        // BF16 / FP32 target operand [mp/16][nc][2 parts][64 lanes]
+  double *Ad;       // FP64 targets in sorted order [mp][32 nc], zero past k and m
   uint4 *Bp;        // BF16 / FP32 source operand [np/16][nc][2 parts][64 lanes]
   double4 *Bd;      // FP64 source operand [np/16][2 nc][64 lanes]
   float2 *ra;       // per target: {-|a|^2 / 2, -hi log2(e) / 2 (VarBW)}
@@ -91,13 +109,19 @@ struct MixPack {
   const int *pcol;  // sorted source position -> source index of the call
 };
 
-// FP32 copies of the pivots: pivot i is target (i * m) / P, zero past k.
-__global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv) {
+// FP32 copies of the pivots: pivot i is target (i * m) / P, zero past k.  With
+// pivB, also in the B-operand order of v_mfma_f32_16x16x4f32 for
+// mix_bounds_kernel: tile t (pivots 16 t ..), step st (coordinates 4 st ..),
+// lane 16 g + c holding coordinate 4 st + g of pivot 16 t + c, so that one
+// fragment load is 256 contiguous bytes.
+__global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv, float *pivB) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P * kp) return;
   const int i = t / kp, q = t - i * kp;
   const int r = (int)(((long)i * p.m) / P);
-  piv[t] = q < p.k ? (float)p.XA[(long)p.amap[r] * p.k + q] : 0.0f;
+  const float v = q < p.k ? (float)p.XA[(long)p.amap[r] * p.k + q] : 0.0f;
+  piv[t] = v;
+  if (pivB) pivB[((i >> 4) * (kp >> 2) + (q >> 2)) * 64 + (q & 3) * 16 + (i & 15)] = v;
 }
 
 // Bucket (nearest pivot) of every target, then every source: the pivot with the
@@ -138,6 +162,241 @@ __global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p, int P,
   }
   if (t < p.m) key_a[t] = bi;
   else key_b[t - p.m] = bi;
+}
+
+// ---- pruning bounds (skip tier)
+//
+// For any two points p_A, p_B, a target a and a source b,
+//   |a - b|^2 >= (e_B(a) - e_A(a)) + (e_A(b) - e_B(b)) - |p_A - p_B|^2,
+// with e_i(x) = |x - p_i|^2: write a - b = D + (a - p_A) - (b - p_B) with
+// D = p_A - p_B and drop the square |(a - p_A) - (b - p_B)|^2 >= 0 from the
+// expansion.  With the pivots p_i and each point's bucket as A (targets) or B
+// (sources), the per-block minima
+//   V[i][blk] = min over the block's points x of e_i(x) - e_own(x)
+// bound every pair of a target block T (bucket A) and a source block S (bucket
+// B) from below by Vt[B][T] + Vs[A][S] - pp[A][B] (blocks that span several
+// buckets take the least value over their ranges).  Across clusters the two
+// differences are nearly |D|^2 each, so whole runs of far source blocks are
+// proven negligible before any of their operands is loaded.
+
+// pp[i][j] = |p_i - p_j|^2 in FP64 from the FP32 pivots, rounded up to FP32,
+// and pn[i] = |p_i|^2 (nearest FP32).  Workgroup i, thread j.
+__global__ void mix_pivot_gram_kernel(int P, int kp, const float *piv, float *pp, float *pn) {
+  const int i = blockIdx.x, j = threadIdx.x;
+  if (j >= P) return;
+  double t = 0.0;
+  for (int q = 0; q < kp; ++q) {
+    const double d = (double)piv[i * kp + q] - (double)piv[j * kp + q];  // exact
+    t += d * d;
+  }
+  t *= 1.0 + 0x1p-40;  // covers the FP64 roundings of the sum
+  float f = (float)t;
+  if ((double)f < t) f = __uint_as_float(__float_as_uint(f) + 1u);  // next float up (f >= 0)
+  pp[i * P + j] = f;
+  if (j == i) {
+    double n2 = 0.0;
+    for (int q = 0; q < kp; ++q) n2 += (double)piv[i * kp + q] * (double)piv[i * kp + q];
+    pn[i] = (float)n2;
+  }
+}
+
+// One wave per 16-point block of the sorted targets (blocks [0, ntb)) or
+// sources ([ntb, ntb + nsb)): the 128 dot products x . p_i of its points on the
+// FP32 matrix cores (v_mfma_f32_16x16x4f32, eight 16-pivot tiles from the packed
+// pivB), then
+//   V[blk][i] = min_x (pn_i - pn_own - 2 (x.p_i - x.p_own)) - mu(x),
+// the minimum of e_i(x) - e_own(x) less a margin mu(x) that covers every FP32
+// rounding (the coordinates, the dot products, the norms and the combination):
+//   mu(x) = (4 kp + 32) 2^-24 (|x| + max |p_i|)^2.
+// Targets go to Vt[T][i], sources to Vs[i][S].  Points past m / n do not count
+// (+inf); rng[blk] = first | last << 16 bucket of the block's points.
+template <int KP>
+__global__ __launch_bounds__(256) void mix_bounds_kernel(const Problem p, int ntb, int nsb,
+                                                         const float *__restrict__ pivB,
+                                                         const float *__restrict__ pn,
+                                                         const unsigned *__restrict__ keys_a,
+                                                         const unsigned *__restrict__ keys_b,
+                                                         const int *__restrict__ prow,
+                                                         const int *__restrict__ pcol, float *Vt,
+                                                         float *Vs, unsigned *rt, unsigned *rs) {
+  constexpr int NS = KP / 4;  // MFMA steps of 4 coordinates
+  constexpr int NT = kPivots / 16;
+  const int lane = threadIdx.x & 63;
+  const int blk = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (blk >= ntb + nsb) return;  // wave-uniform
+  const bool tgt = blk < ntb;
+  const int b0 = tgt ? blk : blk - ntb;
+  const int cnt = tgt ? p.m : p.n;
+  const unsigned *keys = tgt ? keys_a : keys_b;
+  const int p16 = lane & 15, g = lane >> 4;
+  // the A operand: row p16 supplies coordinates 4 st + g
+  const int pos = b0 * 16 + p16;
+  const bool valid = pos < cnt;
+  const double *x = nullptr;
+  if (valid) x = tgt ? p.XA + (long)p.amap[prow[pos]] * p.k : p.XB + (long)p.bmap[pcol[pos]] * p.k;
+  float a[NS];
+  float n2 = 0.0f;
+#pragma unroll
+  for (int st = 0; st < NS; ++st) {
+    const int q = 4 * st + g;
+    a[st] = (valid && q < p.k) ? (float)x[q] : 0.0f;
+    n2 = fmaf(a[st], a[st], n2);
+  }
+  n2 += __shfl_xor(n2, 16);
+  n2 += __shfl_xor(n2, 32);  // |x|^2 of row p16 on every lane of that row
+  float pnt[NT], pmx = 0.0f;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    pnt[t] = pn[16 * t + p16];
+    pmx = fmaxf(pmx, pnt[t]);
+  }
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) pmx = fmaxf(pmx, __shfl_xor(pmx, o));
+  const float pm = sqrtf(pmx) * 1.0001f;  // >= max |p_i| (pn rounded to nearest)
+  f4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    const float *pv = pivB + (long)t * NS * 64 + lane;
+#pragma unroll
+    for (int st = 0; st < NS; ++st)
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], pv[st * 64], acc[t], 0, 0, 0);
+  }
+  // acc[t][v]: row 4 g + v, pivot 16 t + p16
+  constexpr float kMu = (4.0f * KP + 32.0f) * 0x1p-24f * 1.01f;
+  float best[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) best[t] = INFINITY;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = 4 * g + v;
+    const int rpos = b0 * 16 + r;
+    const bool rv = rpos < cnt;
+    const int own = rv ? (int)keys[rpos] : 0;
+    // x . p_own: picked out of the tile that holds it, then summed over the row's 16 lanes
+    float dsel = 0.0f;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) dsel = (16 * t + p16 == own) ? acc[t][v] : dsel;
+    dsel += __shfl_xor(dsel, 1);
+    dsel += __shfl_xor(dsel, 2);
+    dsel += __shfl_xor(dsel, 4);
+    dsel += __shfl_xor(dsel, 8);
+    const float nx = sqrtf(__shfl(n2, r)) * 1.0001f;
+    const float mu = kMu * (nx + pm) * (nx + pm);
+    const float pno = pn[own];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const float val = (pnt[t] - pno) - 2.0f * (acc[t][v] - dsel) - mu;
+      best[t] = rv ? fminf(best[t], val) : best[t];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    best[t] = fminf(best[t], __shfl_xor(best[t], 16));
+    best[t] = fminf(best[t], __shfl_xor(best[t], 32));
+  }
+  if (g == 0) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (tgt) Vt[(long)b0 * kPivots + 16 * t + p16] = best[t];
+      else Vs[(long)(16 * t + p16) * nsb + b0] = best[t];
+    }
+  }
+  if (lane == 0) {
+    const int last = min(cnt - b0 * 16, 16) - 1;
+    const unsigned f = last >= 0 ? keys[b0 * 16] : 0u, l = last >= 0 ? keys[b0 * 16 + last] : 0u;
+    (tgt ? rt : rs)[b0] = f | (l << 16);
+  }
+}
+
+// One wave per (row group g of 32 sorted targets, source slice): for each
+// group gi of 64 source blocks of the slice, bit b of
+// need[(g nsplit + split) ngw + gi] is set unless the bound proves every pair of
+// source block 64 gi + b negligible for both 16-row blocks of the group:
+//   L = min_B Vt[T][B] + min_A Vs[A][S] - max_{A,B} pp[A][B]  > thr_d2
+// over the blocks' bucket ranges, less the FP32 roundings of that sum.  Blocks
+// spanning more than four buckets, and row blocks spanning more than four, are
+// kept.  The row blocks' Vt rows and the pp rows of their buckets stay in
+// registers (lane l holds entries l and 64 + l) and are read with lane
+// shuffles, so a group's only memory loads are the blocks' bucket ranges and
+// their Vs entries, independent of each other.
+__global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ngrp, int nsplit,
+                                                       int cps, int ngw, int n,
+                                                       unsigned long long *need) {
+  const int lane = threadIdx.x & 63;
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= (long)ngrp * nsplit) return;  // wave-uniform
+  const int split = (int)(w % nsplit), g = (int)(w / nsplit);
+  const int j0 = split * cps, j1 = min(n, j0 + cps);
+  const int nbs = j0 < j1 ? (j1 - j0 + 15) >> 4 : 0;
+  unsigned long long *out = need + w * ngw;
+  int a0[2], a1[2];
+  float vt_lo[2], vt_hi[2];
+  bool simple = true;  // both row blocks lie in one bucket each
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) {
+    const int T = 2 * g + rb;
+    const unsigned rtv = b.rt[T];
+    a0[rb] = (int)(rtv & 0xFFFFu);
+    a1[rb] = (int)(rtv >> 16);
+    simple = simple && a0[rb] == a1[rb];
+    vt_lo[rb] = b.Vt[(long)T * kPivots + lane];
+    vt_hi[rb] = b.Vt[(long)T * kPivots + 64 + lane];
+  }
+  // pp rows of the two row blocks' buckets (the common case)
+  const float pp0_lo = b.pp[a0[0] * kPivots + lane], pp0_hi = b.pp[a0[0] * kPivots + 64 + lane];
+  const float pp1_lo = b.pp[a0[1] * kPivots + lane], pp1_hi = b.pp[a0[1] * kPivots + 64 + lane];
+  auto pick = [&](float lo, float hi, int B) {  // entry B of a row held as (lo, hi)
+    const float x = __shfl(lo, B & 63), y = __shfl(hi, B & 63);
+    return B < 64 ? x : y;
+  };
+  for (int gi = 0; gi < ngw; ++gi) {
+    const int sb = gi * 64 + lane;
+    const bool in = sb < nbs;
+    const int S = (j0 >> 4) + (in ? sb : 0);
+    const unsigned rsv = b.rs[S];
+    const int b0 = (int)(rsv & 0xFFFFu), b1 = (int)(rsv >> 16);
+    const float vs0 = b.Vs[(long)a0[0] * b.nsb + S], vs1 = b.Vs[(long)a0[1] * b.nsb + S];
+    bool nd = false;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      bool keep = true;
+      float tmin = INFINITY, smin = INFINITY, ppmax = 0.0f;
+      if (simple) {
+        // one bucket per row block: shuffles read the rows (every lane takes part)
+        const float pl = rb == 0 ? pp0_lo : pp1_lo, ph = rb == 0 ? pp0_hi : pp1_hi;
+        float t0 = pick(vt_lo[rb], vt_hi[rb], b0), t1 = pick(vt_lo[rb], vt_hi[rb], b1);
+        float q0 = pick(pl, ph, b0), q1 = pick(pl, ph, b1);
+        tmin = fminf(t0, t1);
+        ppmax = fmaxf(q0, q1);
+        smin = rb == 0 ? vs0 : vs1;
+        if (b1 - b0 > 1) {  // buckets strictly inside the block's range: memory
+          for (int B = b0 + 1; B < b1; ++B) {
+            tmin = fminf(tmin, b.Vt[(long)(2 * g + rb) * kPivots + B]);
+            ppmax = fmaxf(ppmax, b.pp[a0[rb] * kPivots + B]);
+          }
+        }
+      } else {
+        for (int B = b0; B <= b1; ++B) tmin = fminf(tmin, b.Vt[(long)(2 * g + rb) * kPivots + B]);
+        for (int A = a0[rb]; A <= a1[rb]; ++A) {
+          smin = fminf(smin, b.Vs[(long)A * b.nsb + S]);
+          for (int B = b0; B <= b1; ++B) ppmax = fmaxf(ppmax, b.pp[A * kPivots + B]);
+        }
+      }
+      if (b1 - b0 >= 4 || a1[rb] - a0[rb] >= 4) {
+        keep = true;
+      } else if (tmin == INFINITY || smin == INFINITY) {
+        keep = false;  // a block of padding points only: no pairs
+      } else {
+        const float L = (tmin + smin) - ppmax;
+        const float err = 0x1p-22f * (fabsf(tmin) + fabsf(smin) + ppmax);
+        keep = !(L - err > b.thr_d2);  // NaN keeps the block
+      }
+      nd = nd || keep;
+    }
+    const unsigned long long msk = __ballot(in && nd);
+    if (lane == 0) out[gi] = msk;
+  }
 }
 
 __global__ void mix_identity_kernel(int *v, long n) {
@@ -200,6 +459,25 @@ __device__ __forceinline__ void pack_point(const double *__restrict__ x, int k, 
         dblk[c * 64 + g * 16 + p16] = make_double4(v[0], v[1], v[2], v[3]);
       }
   }
+}
+
+// The FP64 targets in sorted order, rows of kp = 32 nc doubles (zero past k and
+// for padding rows): one thread per pair of coordinates, so that loads of a
+// row and stores of the copy are contiguous across lanes.
+__global__ void mix_rows_kernel(const Problem p, int kp, long mp, const int *__restrict__ prow,
+                                double2 *Ad) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int h = kp / 2;
+  if (e >= mp * h) return;
+  const long t = e / h;
+  const int q = 2 * (int)(e - t * h);
+  double2 v = make_double2(0.0, 0.0);
+  if (t < p.m) {
+    const double *x = p.XA + (long)p.amap[prow[t]] * p.k;
+    if (q < p.k) v.x = x[q];
+    if (q + 1 < p.k) v.y = x[q + 1];
+  }
+  Ad[e] = v;
 }
 
 // One thread per point of the padded target range [0, mp) then source range [0, np),
@@ -343,15 +621,26 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
   // 32 NC + 2 doubles so the 16 rows of one FP64 fragment load fall on
   // different banks; zero past k and for padding rows
   constexpr int KS = 32 * NC + 2;
-  __shared__ double s_rows[4][WROWS * KS];
+  __shared__ __align__(16) double s_rows[4][WROWS * KS];
   __shared__ double s_tot[4][WROWS];
   // FP64 row factors {|a|^2, -hi/2}, read by near blocks only
   __shared__ double2 s_rd[4][WROWS];
   double *arows = s_rows[wave];
-  for (int e = lane; e < WROWS * 32 * NC; e += 64) {
-    const int rl = e / (32 * NC), q = e - rl * (32 * NC);
-    const int r = row0 + rl;
-    arows[rl * KS + q] = (q < p.k && r < m) ? p.XA[(long)p.amap[pk.prow[r]] * p.k + q] : 0.0;
+  {
+    // the wave's rows are contiguous in the sorted FP64 copy: every load is
+    // independent and issued before the first store (a gather through prow
+    // and amap made each iteration wait for three dependent round trips)
+    constexpr int PER = WROWS * 32 * NC / 128;  // 16-byte loads per lane
+    const double2 *src = reinterpret_cast<const double2 *>(pk.Ad + (long)row0 * 32 * NC);
+    double2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) v[i] = src[i * 64 + lane];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = 2 * (i * 64 + lane);
+      const int rl = e / (32 * NC), q = e - rl * (32 * NC);
+      *reinterpret_cast<double2 *>(&arows[rl * KS + q]) = v[i];
+    }
   }
 
   // BF16 / FP32 accumulators start at -|a|^2/2 (the packed row factor), so the
@@ -508,22 +797,60 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
   const int jlast = j0 + ((j1 - j0 - 1) >> 4) * 16;  // start of the slice's last block
 
   if constexpr (kSkipTier<PREC, TYPE>) {
+    // ---- which source blocks of the slice can matter (pruning bounds)
+    // Bit b of mask word gi marks source block 64 gi + b of the slice as one
+    // that may hold a pair with K >= 2^-60 for the wave's rows
+    // (mix_need_kernel); every other block is proven negligible and never
+    // loaded.  Lane gi holds word gi; without masks every block is kept.
+    const int nbs = j0 < j1 ? (j1 - j0 + 15) >> 4 : 0;  // source blocks of the slice
+    const int ngroups = (nbs + 63) >> 6;
+    unsigned long long myword = 0ull;
+    if (lane < ngroups) {
+      if (pk.need) {
+        const long g = row0 >> 5;  // the wave's row group
+        myword = pk.need[(g * A.nsplit + split) * pk.ngw + lane];
+      } else {
+        const int left = nbs - 64 * lane;
+        myword = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
+      }
+    }
+    // the needed blocks in order, as a wave-uniform generator
+    auto read_mask = [&](int gi) {
+      const unsigned lo = __builtin_amdgcn_readlane((unsigned)myword, gi);
+      const unsigned hi = __builtin_amdgcn_readlane((unsigned)(myword >> 32), gi);
+      return ((unsigned long long)hi << 32) | lo;
+    };
+    int gcur = 0;
+    unsigned long long mcur = ngroups > 0 ? read_mask(0) : 0ull;
+    auto next_block = [&]() -> int {
+      while (mcur == 0ull) {
+        if (++gcur >= ngroups) return -1;
+        mcur = read_mask(gcur);
+      }
+      const int b = __builtin_ctzll(mcur);
+      mcur &= mcur - 1ull;
+      return j0 + ((gcur << 6) + b) * 16;
+    };
+
     // Two source blocks x RB row blocks per step: their 2 RB first products
     // are independent and issue back to back, their skip tests are four
     // instructions each, and one ballot lets the usual all-skipped step
-    // through.  The hi halves of the next two blocks load meanwhile.
+    // through.  The hi halves of the next two needed blocks load meanwhile.
     // a step loads the hi halves and the skip test's two factors per source
     // (8 bytes, not the 16-byte record with the weight, which only blocks that
     // pass need: it comes with their lo halves)
     struct HB {
       uint4 h[NC];
       float2 cs;
+      int jb;  // the source block held, -1 past the last needed one
     };
-    auto fetch_h = [&](int jb, HB &b) {
-      const uint4 *blk = pk.Bp + (long)(jb >> 4) * NC * 128 + lane;
+    auto fetch_h = [&](HB &b) {
+      b.jb = next_block();
+      if (b.jb < 0) return;  // wave-uniform
+      const uint4 *blk = pk.Bp + (long)(b.jb >> 4) * NC * 128 + lane;
 #pragma unroll
       for (int c = 0; c < NC; ++c) b.h[c] = blk[(c * 2) * 64];
-      b.cs = pk.cs[jb + p16];
+      b.cs = pk.cs[b.jb + p16];
     };
     auto est = [&](int rb, const HB &b) {
       f4 e = cinit[rb];
@@ -569,11 +896,13 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
         evaluate(rb, jb, c, bcb, f64, used64);
       }
       // blocks that pass come in runs once the points are bucketed: after one,
-      // start the next block's lo loads
-      if (usedlo && jnext != jb) fetch_lo(jnext);
+      // start the next needed block's lo loads
+      if (usedlo && jnext >= 0 && jnext != jb) fetch_lo(jnext);
     };
-    auto step = [&](int jb, const HB &b0, const HB &b1) {
-      const bool two = jb + 16 < j1;  // the second block lies in the slice
+    // one step over the pair (b0, b1); false once the needed blocks are done
+    auto step = [&](const HB &b0, const HB &b1, int jafter) {
+      if (b0.jb < 0) return false;
+      const bool two = b1.jb >= 0;
       f4 e0[RB], e1[RB];
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb) e0[rb] = est(rb, b0);
@@ -593,32 +922,28 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
           h0[rb] = __ballot(h0[rb]) != 0ull;
           h1[rb] = __ballot(h1[rb]) != 0ull;
         }
-        finish(jb, min(jb + 16, jlast), b0, e0, h0);
-        if (two) finish(jb + 16, min(jb + 32, jlast), b1, e1, h1);
+        finish(b0.jb, two ? b1.jb : jafter, b0, e0, h0);
+        if (two) finish(b1.jb, jafter, b1, e1, h1);
       }
       tick(two ? 2 : 1);
+      return two;
     };
-    // three pairs of buffers in turn, two pairs ahead (blocks past the slice
-    // re-read its last block, in bounds, and are not evaluated)
+    // three pairs of buffers in turn, two pairs ahead; a buffer past the last
+    // needed block holds jb = -1 and is neither loaded nor evaluated
     HB pa0, pa1, pb0, pb1, pc0, pc1;
-    auto fetch_pair = [&](int jb, HB &x0, HB &x1) {
-      fetch_h(min(jb, jlast), x0);
-      fetch_h(min(jb + 16, jlast), x1);
+    auto fetch_pair = [&](HB &x0, HB &x1) {
+      fetch_h(x0);
+      fetch_h(x1);
     };
-    if (j0 < j1) {
-      fetch_pair(j0, pa0, pa1);
-      fetch_pair(j0 + 32, pb0, pb1);
-      for (int jb = j0;; jb += 96) {
-        fetch_pair(jb + 64, pc0, pc1);
-        step(jb, pa0, pa1);
-        if (jb + 32 >= j1) break;
-        fetch_pair(jb + 96, pa0, pa1);
-        step(jb + 32, pb0, pb1);
-        if (jb + 64 >= j1) break;
-        fetch_pair(jb + 128, pb0, pb1);
-        step(jb + 64, pc0, pc1);
-        if (jb + 96 >= j1) break;
-      }
+    fetch_pair(pa0, pa1);
+    fetch_pair(pb0, pb1);
+    for (;;) {
+      fetch_pair(pc0, pc1);
+      if (!step(pa0, pa1, pb0.jb)) break;
+      fetch_pair(pa0, pa1);
+      if (!step(pb0, pb1, pc0.jb)) break;
+      fetch_pair(pb0, pb1);
+      if (!step(pc0, pc1, pa0.jb)) break;
     }
   } else {
     F64Frag f64p;  // persists across blocks: prefetched after a near block
@@ -669,26 +994,11 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
     if (writer && myrow < m) p.u[p.umap[pk.prow[myrow]]] += mine;
     return;
   }
+  // the slices' partial sums are added in slice order by mix_reduce_kernel,
+  // launched next: a fence and a counter here would make every workgroup
+  // write back and invalidate its XCD's L2 (device-scope release / acquire on
+  // gfx950), evicting the source slices the other workgroups are reading
   if (writer && myrow < m) A.partial[(long)split * m + myrow] = mine;
-  __threadfence();
-  __syncthreads();
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(&A.counters[rblk], 1u);
-    s_last = (prev == (unsigned)(A.nsplit - 1));
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  for (int q = threadIdx.x; q < wg_rows; q += XWG) {
-    int r = rblk * wg_rows + q;
-    if (r < m) {
-      double s = 0.0;
-      for (int sp = 0; sp < A.nsplit; ++sp) s += A.partial[(long)sp * m + r];
-      p.u[p.umap[pk.prow[r]]] += s;
-    }
-  }
-  if (threadIdx.x == 0) A.counters[rblk] = 0u;
 }
 
 // ------------------------------------------------------------------ launch
@@ -698,6 +1008,8 @@ static hipError_t mixed_go(const FusedArgs &A, const MixPack &pk, float xs, int 
                            hipStream_t s) {
   dim3 grid((unsigned)row_blocks * (unsigned)A.nsplit);
   hipLaunchKernelGGL((mixed_kernel<PREC, TYPE, NC>), grid, dim3(XWG), 0, s, A, pk, xs);
+  if (A.nsplit > 1)
+    return launch_split_reduce(A.partial, A.nsplit, A.p.m, A.p.u, A.p.umap, pk.prow, s);
   return hipGetLastError();
 }
 
@@ -725,6 +1037,9 @@ static hipError_t mixed_launch(const Problem &p, const FusedPlan &plan, double *
   const float xs = (float)(p.P.scal * (double)kLog2e);
   const long mp = (long)plan.row_blocks * kMixRowsPerWG, np = ((long)p.n + 15) / 16 * 16;
   dim3 pg((unsigned)((mp + np + 255) / 256));
+  const int kp = 32 * nc;
+  hipLaunchKernelGGL(mix_rows_kernel, dim3((unsigned)((mp * (kp / 2) + 255) / 256)), dim3(256), 0,
+                     s, p, kp, mp, pk.prow, reinterpret_cast<double2 *>(pk.Ad));
   if (p.type == GAUSSIAN) {
     hipLaunchKernelGGL((mixed_pack_kernel<PREC, GAUSSIAN>), pg, dim3(256), 0, s, p, nc, xs, mp, np,
                        pk);
@@ -733,6 +1048,16 @@ static hipError_t mixed_launch(const Problem &p, const FusedPlan &plan, double *
   hipLaunchKernelGGL((mixed_pack_kernel<PREC, VAR_BANDWIDTH>), pg, dim3(256), 0, s, p, nc, xs, mp,
                      np, pk);
   return mixed_nc<PREC, VAR_BANDWIDTH>(A, pk, xs, nc, plan.row_blocks, s);
+}
+
+template <int KP>
+static void launch_bounds(const Problem &p, int ntb, int nsb, const float *pivB, const float *pn,
+                          const unsigned *ka, const unsigned *kb, const int *prow,
+                          const int *pcol, float *Vt, float *Vs, unsigned *rt, unsigned *rs,
+                          hipStream_t s) {
+  const unsigned wgs = (unsigned)(((long)ntb + nsb + 3) / 4);
+  hipLaunchKernelGGL((mix_bounds_kernel<KP>), dim3(wgs), dim3(256), 0, s, p, ntb, nsb, pivB, pn, ka,
+                     kb, prow, pcol, Vt, Vs, rt, rs);
 }
 
 template <int KP>
@@ -784,6 +1109,7 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t off = 0;
   const size_t o_Ap = off; off += al(mp * nc * 128);
+  const size_t o_Ad = off; off += al(mp * nc * 32 * sizeof(double));
   const size_t o_Bp = off; off += al(np * nc * 128);
   const size_t o_Bd = off; off += al(np * nc * 256);
   const size_t o_ra = off; off += al(mp * sizeof(float2));
@@ -796,8 +1122,24 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   const size_t o_ids = off; off += al(mx * sizeof(int));
   const size_t o_ka = off; off += bucket ? al((size_t)p.m * sizeof(unsigned)) : 0;
   const size_t o_kb = off; off += bucket ? al((size_t)p.n * sizeof(unsigned)) : 0;
-  const size_t o_kout = off; off += bucket ? al(mx * sizeof(unsigned)) : 0;
+  const size_t o_kouta = off; off += bucket ? al((size_t)p.m * sizeof(unsigned)) : 0;
+  const size_t o_koutb = off; off += bucket ? al((size_t)p.n * sizeof(unsigned)) : 0;
   const size_t o_piv = off; off += bucket ? al((size_t)P * kp * sizeof(float)) : 0;
+  // pruning bounds of the skip tier (bucketed BF16x3 Gaussian with a decaying kernel)
+  const bool bounds = bucket && precision == KS_PREC_MIXED_BF16X3 && p.type == GAUSSIAN &&
+                      p.P.scal < 0.0;
+  const size_t ntb = mp / 16, nsb = np / 16;
+  const size_t ngrp = mp / 32;  // row groups of one wave
+  const size_t ngw = ((size_t)plan.cols_per_split / 16 + 63) / 64;
+  const size_t o_pivb = off; off += bounds ? al((size_t)P * kp * sizeof(float)) : 0;
+  const size_t o_pp = off; off += bounds ? al((size_t)P * P * sizeof(float)) : 0;
+  const size_t o_pn = off; off += bounds ? al((size_t)P * sizeof(float)) : 0;
+  const size_t o_vt = off; off += bounds ? al((size_t)P * ntb * sizeof(float)) : 0;
+  const size_t o_vs = off; off += bounds ? al((size_t)P * nsb * sizeof(float)) : 0;
+  const size_t o_rt = off; off += bounds ? al(ntb * sizeof(unsigned)) : 0;
+  const size_t o_rs = off; off += bounds ? al(nsb * sizeof(unsigned)) : 0;
+  const size_t o_need = off;
+  off += bounds ? al(ngrp * plan.nsplit * ngw * sizeof(unsigned long long)) : 0;
   const size_t o_sort = off; off += al(sort_bytes);
   const size_t npart = (size_t)(plan.nsplit > 1 ? plan.nsplit : 0) * p.m;
   double *partial = nullptr;
@@ -810,6 +1152,7 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   base = (char *)(((uintptr_t)base + 255) & ~(uintptr_t)255);
   MixPack pk;
   pk.Ap = (uint4 *)(base + o_Ap);
+  pk.Ad = (double *)(base + o_Ad);
   pk.Bp = (uint4 *)(base + o_Bp);
   pk.Bd = (double4 *)(base + o_Bd);
   pk.ra = (float2 *)(base + o_ra);
@@ -821,6 +1164,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   int *pcol = (int *)(base + o_pcol);
   pk.prow = prow;
   pk.pcol = pcol;
+  pk.need = nullptr;
+  pk.ngw = (int)ngw;
 
   // ---- target and source orders
   if (!bucket) {
@@ -829,10 +1174,11 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   } else {
     int *ids = (int *)(base + o_ids);
     unsigned *ka = (unsigned *)(base + o_ka), *kb = (unsigned *)(base + o_kb);
-    unsigned *kout = (unsigned *)(base + o_kout);
+    unsigned *kouta = (unsigned *)(base + o_kouta), *koutb = (unsigned *)(base + o_koutb);
     float *piv = (float *)(base + o_piv);
+    float *pivB = bounds ? (float *)(base + o_pivb) : nullptr;
     hipLaunchKernelGGL(mix_pivot_kernel, dim3((unsigned)((P * kp + 255) / 256)), dim3(256), 0, s, p,
-                       P, kp, piv);
+                       P, kp, piv, pivB);
     switch (nc) {
       case 1: launch_bucket<32>(p, P, piv, ka, kb, s); break;
       case 2: launch_bucket<64>(p, P, piv, ka, kb, s); break;
@@ -842,13 +1188,45 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
     hipLaunchKernelGGL(mix_identity_kernel, dim3((unsigned)((mx + 255) / 256)), dim3(256), 0, s,
                        ids, (long)mx);
     size_t tb = sort_bytes;
-    if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, tb, (const unsigned *)ka, kout,
+    if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, tb, (const unsigned *)ka, kouta,
                                            (const int *)ids, prow, p.m, 0, 8, s) != hipSuccess)
       return KS_ERR_DEVICE;
     tb = sort_bytes;
-    if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, tb, (const unsigned *)kb, kout,
+    if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, tb, (const unsigned *)kb, koutb,
                                            (const int *)ids, pcol, p.n, 0, 8, s) != hipSuccess)
       return KS_ERR_DEVICE;
+    if (bounds) {
+      if (P != kPivots) return KS_ERR_INVALID_ARG;  // the bound kernels take all kPivots tiles
+      float *pp = (float *)(base + o_pp), *pn = (float *)(base + o_pn);
+      float *Vt = (float *)(base + o_vt), *Vs = (float *)(base + o_vs);
+      unsigned *rt = (unsigned *)(base + o_rt), *rs = (unsigned *)(base + o_rs);
+      hipLaunchKernelGGL(mix_pivot_gram_kernel, dim3((unsigned)P), dim3((unsigned)P), 0, s, P, kp,
+                         (const float *)piv, pp, pn);
+      switch (nc) {
+        case 1: launch_bounds<32>(p, (int)ntb, (int)nsb, pivB, pn, kouta, koutb, prow, pcol, Vt, Vs, rt, rs, s); break;
+        case 2: launch_bounds<64>(p, (int)ntb, (int)nsb, pivB, pn, kouta, koutb, prow, pcol, Vt, Vs, rt, rs, s); break;
+        case 3: launch_bounds<96>(p, (int)ntb, (int)nsb, pivB, pn, kouta, koutb, prow, pcol, Vt, Vs, rt, rs, s); break;
+        default: launch_bounds<128>(p, (int)ntb, (int)nsb, pivB, pn, kouta, koutb, prow, pcol, Vt, Vs, rt, rs, s); break;
+      }
+      MixBounds mb;
+      mb.Vt = Vt;
+      mb.Vs = Vs;
+      mb.pp = pp;
+      mb.rt = rt;
+      mb.rs = rs;
+      mb.ntb = (int)ntb;
+      mb.nsb = (int)nsb;
+      // a block pair whose d^2 lower bound exceeds 60 / |xs| has every exponent
+      // xs d^2 below -60 (K < 2^-60); the factor covers xs = scal log2(e)
+      // rounded to FP32
+      const float xs = (float)(p.P.scal * (double)kLog2e);
+      mb.thr_d2 = (float)(60.0 / fabs((double)xs) * (1.0 + 0x1p-20));
+      unsigned long long *need = (unsigned long long *)(base + o_need);
+      const long nwaves = (long)ngrp * plan.nsplit;
+      hipLaunchKernelGGL(mix_need_kernel, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, s, mb,
+                         (int)ngrp, plan.nsplit, plan.cols_per_split, (int)ngw, p.n, need);
+      pk.need = need;
+    }
   }
   if (hipGetLastError() != hipSuccess) return KS_ERR_DEVICE;
 

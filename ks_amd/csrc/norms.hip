@@ -29,6 +29,24 @@ hipError_t launch_zero_u32(unsigned *x, long n, hipStream_t s) {
   return hipGetLastError();
 }
 
+__global__ void split_reduce_kernel(const double *__restrict__ partial, int nsplit, int m,
+                                    double *u, const int *__restrict__ umap,
+                                    const int *__restrict__ prow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double acc = 0.0;
+  for (int sp = 0; sp < nsplit; ++sp) acc += partial[(long)sp * m + i];
+  u[umap[prow ? prow[i] : i]] += acc;
+}
+
+hipError_t launch_split_reduce(const double *partial, int nsplit, int m, double *u,
+                               const int *umap, const int *prow, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(split_reduce_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
+                     partial, nsplit, m, u, umap, prow);
+  return hipGetLastError();
+}
+
 }  // namespace gsks
 
 extern "C" int ks_sqnorm_device(const double *X, long nx, int k, double *X2, void *stream) {
