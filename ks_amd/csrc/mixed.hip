@@ -594,6 +594,7 @@ __device__ __forceinline__ double row_sum16(double x) {
 
 constexpr int kFlush = 8;  // far-pair FP32 partials go to FP64 every 8 source blocks
 
+
 template <int PREC, int TYPE, int NC>
 __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A, MixPack pk,
                                                                   float xs) {
@@ -632,14 +633,11 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
     // and amap made each iteration wait for three dependent round trips)
     constexpr int PER = WROWS * 32 * NC / 128;  // 16-byte loads per lane
     const double2 *src = reinterpret_cast<const double2 *>(pk.Ad + (long)row0 * 32 * NC);
-    double2 v[PER];
-#pragma unroll
-    for (int i = 0; i < PER; ++i) v[i] = src[i * 64 + lane];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int e = 2 * (i * 64 + lane);
       const int rl = e / (32 * NC), q = e - rl * (32 * NC);
-      *reinterpret_cast<double2 *>(&arows[rl * KS + q]) = v[i];
+      *reinterpret_cast<double2 *>(&arows[rl * KS + q]) = src[i * 64 + lane];
     }
   }
 
