@@ -391,7 +391,11 @@ int enqueue_fused(const Problem &p, hipStream_t s, const FixedScratch *fixed) {
   }
   hipError_t e = mfma ? launch_fused_f64_mfma(p, plan, partial, counters, s)
                       : launch_fused_f64(q, plan, partial, counters, s);
-  return e == hipSuccess ? KS_OK : hip_failure(e, mfma ? "fused_f64_mfma launch" : "fused_f64 launch");
+  if (e != hipSuccess)
+    return hip_failure(e, mfma ? "fused_f64_mfma launch" : "fused_f64 launch");
+  // the source splits' partial sums, added in split order
+  if (plan.nsplit > 1) e = launch_split_reduce(partial, plan.nsplit, p.m, p.u, p.umap, nullptr, s);
+  return e == hipSuccess ? KS_OK : hip_failure(e, "split reduction launch");
 }
 
 int device_problem(const char *name, const ks_t *kernel, int m, int n, int k, double *u,

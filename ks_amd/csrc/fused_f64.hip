@@ -114,31 +114,14 @@ __global__ __launch_bounds__(WG) void fused_f64_kernel(FusedArgs A) {
     }
     return;
   }
+  // the splits' partials are added in split order by launch_split_reduce's
+  // kernel, launched next (no device-scope fence here: on gfx950 it would
+  // write back and invalidate this XCD's L2)
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     int i = row0 + q * WG + tid;
     if (i < m) A.partial[(long)split * m + i] = acc[q];
   }
-  __threadfence();  // release this workgroup's partials at device scope
-  __syncthreads();
-  __shared__ int s_last;
-  if (tid == 0) {
-    unsigned prev = atomicAdd(&A.counters[rblk], 1u);
-    s_last = (prev == (unsigned)(A.nsplit - 1));
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();  // acquire the other workgroups' partials
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    int i = row0 + q * WG + tid;
-    if (i < m) {
-      double s = 0.0;
-      for (int sp = 0; sp < A.nsplit; ++sp) s += A.partial[(long)sp * m + i];
-      p.u[p.umap[i]] += s;
-    }
-  }
-  if (tid == 0) A.counters[rblk] = 0u;  // leave the counter ready for the next launch
 }
 
 // ------------------------------------------------------------------ launch

@@ -185,30 +185,14 @@ __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
     }
     return;
   }
+  // the splits' partials are added in split order by launch_split_reduce's
+  // kernel, launched next (no device-scope fence here: on gfx950 it would
+  // write back and invalidate this XCD's L2)
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) {
     int r = row0 + rb * 16 + g + 4 * p16;
     if (writer && r < m) A.partial[(long)split * m + r] = mine[rb];
   }
-  __threadfence();
-  __syncthreads();
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(&A.counters[rblk], 1u);
-    s_last = (prev == (unsigned)(A.nsplit - 1));
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  for (int q = threadIdx.x; q < wg_rows; q += MWG) {
-    int r = rblk * wg_rows + q;
-    if (r < m) {
-      double s = 0.0;
-      for (int sp = 0; sp < A.nsplit; ++sp) s += A.partial[(long)sp * m + r];
-      p.u[p.umap[r]] += s;
-    }
-  }
-  if (threadIdx.x == 0) A.counters[rblk] = 0u;
 }
 
 // ------------------------------------------------------------------ symmetric sums
@@ -389,8 +373,7 @@ __global__ __launch_bounds__(SMWG) void fused_f64_sym_mfma_kernel(SymMArgs A) {
     mine[rb] = x;
   }
   const bool writer = p16 < 4;
-  const int active = A.nsplit - first;
-  if (active == 1) {
+  if (A.nsplit == 1) {
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
       const int r = wrow0 + rb * 16 + g + 4 * p16;
@@ -398,30 +381,15 @@ __global__ __launch_bounds__(SMWG) void fused_f64_sym_mfma_kernel(SymMArgs A) {
     }
     return;
   }
+  // the splits' partials (from `first` on) are added in split order by
+  // sym_colsum_kernel, launched next, before the column partials (no
+  // device-scope fence here: on gfx950 it would write back and invalidate this
+  // XCD's L2)
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) {
     const int r = wrow0 + rb * 16 + g + 4 * p16;
     if (writer && r < m) A.partial[(long)split * m + r] = mine[rb];
   }
-  __threadfence();
-  __syncthreads();
-  __shared__ int s_last;
-  if (tid == 0) {
-    const unsigned prev = atomicAdd(&A.counters[rblk], 1u);
-    s_last = (prev == (unsigned)(active - 1));
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  for (int q = tid; q < B; q += SMWG) {
-    const int r = row0 + q;
-    if (r < m) {
-      double s = 0.0;
-      for (int sp = first; sp < A.nsplit; ++sp) s += A.partial[(long)sp * m + r];
-      p.u[p.umap[r]] += s;
-    }
-  }
-  if (tid == 0) A.counters[rblk] = 0u;
 }
 
 // ------------------------------------------------------------------ launch
@@ -541,7 +509,7 @@ hipError_t launch_fused_f64_sym_mfma(const Problem &p, const SymPlan &pl, double
     default: return hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;
-  return launch_sym_colsum(p, pl.rows_per_wg, colp, s);
+  return launch_sym_colsum(p, pl, partial, colp, s);
 }
 
 }  // namespace gsks

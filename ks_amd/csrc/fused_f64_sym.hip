@@ -166,8 +166,7 @@ __global__ __launch_bounds__(SWG) void fused_f64_sym_kernel(SymArgs A) {
   }
 
   // ---- row results: the splits from `first` on, added in split order
-  const int active = A.nsplit - first;
-  if (active == 1) {
+  if (A.nsplit == 1) {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int i = row0 + q * SWG + tid;
@@ -175,42 +174,40 @@ __global__ __launch_bounds__(SWG) void fused_f64_sym_kernel(SymArgs A) {
     }
     return;
   }
+  // the splits' partials (from `first` on) are added in split order by
+  // sym_colsum_kernel, launched next, before the column partials (no
+  // device-scope fence here: on gfx950 it would write back and invalidate this
+  // XCD's L2)
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int i = row0 + q * SWG + tid;
     if (i < m) A.partial[(long)split * m + i] = acc[q];
   }
-  __threadfence();
-  __syncthreads();
-  __shared__ int s_last;
-  if (tid == 0) {
-    const unsigned prev = atomicAdd(&A.counters[rblk], 1u);
-    s_last = (prev == (unsigned)(active - 1));
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int i = row0 + q * SWG + tid;
-    if (i < m) {
-      double s = 0.0;
-      for (int sp = first; sp < A.nsplit; ++sp) s += A.partial[(long)sp * m + i];
-      p.u[p.umap[i]] += s;
-    }
-  }
-  if (tid == 0) A.counters[rblk] = 0u;
 }
 
-// u[umap[j]] += sum over the row blocks before j's of their column partials, in order.
-__global__ void sym_colsum_kernel(const Problem p, int B, const double *__restrict__ colp) {
+// The last step of a symmetric sum, one thread per target j of row block J:
+// with split partials (nsplit > 1), u[umap[j]] += the row partials of the
+// splits from J's first on, in split order; then u[umap[j]] += the column
+// partials of the row blocks before J, in order.
+__global__ void sym_colsum_kernel(const Problem p, int B, const double *__restrict__ partial,
+                                  int nsplit, int cols_per_split,
+                                  const double *__restrict__ colp) {
   const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= p.m) return;
   const int J = (int)(j / B);
-  if (J == 0) return;
-  double s = 0.0;
-  for (int I = 0; I < J; ++I) s += colp[(long)I * p.m + j];
-  p.u[p.umap[j]] += s;
+  if (!partial && J == 0) return;
+  double x = p.u[p.umap[j]];
+  if (partial) {
+    double s = 0.0;
+    for (int sp = (J * B) / cols_per_split; sp < nsplit; ++sp) s += partial[(long)sp * p.m + j];
+    x += s;
+  }
+  if (J > 0) {
+    double s = 0.0;
+    for (int I = 0; I < J; ++I) s += colp[(long)I * p.m + j];
+    x += s;
+  }
+  p.u[p.umap[j]] = x;
 }
 
 // ------------------------------------------------------------------ launch
@@ -380,12 +377,14 @@ hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *par
     default: return hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;
-  return launch_sym_colsum(p, pl.rows_per_wg, colp, s);
+  return launch_sym_colsum(p, pl, partial, colp, s);
 }
 
-hipError_t launch_sym_colsum(const Problem &p, int B, const double *colp, hipStream_t s) {
-  hipLaunchKernelGGL(sym_colsum_kernel, dim3((unsigned)((p.m + 255) / 256)), dim3(256), 0, s, p, B,
-                     colp);
+hipError_t launch_sym_colsum(const Problem &p, const SymPlan &pl, const double *partial,
+                             const double *colp, hipStream_t s) {
+  hipLaunchKernelGGL(sym_colsum_kernel, dim3((unsigned)((p.m + 255) / 256)), dim3(256), 0, s, p,
+                     pl.rows_per_wg, pl.nsplit > 1 ? partial : nullptr, pl.nsplit,
+                     pl.cols_per_split, colp);
   return hipGetLastError();
 }
 

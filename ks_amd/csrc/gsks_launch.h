@@ -51,9 +51,12 @@ bool sym_eligible(const Problem &p);
 constexpr int sym_rows_per_lane(int k) { return k <= 4 ? 6 : 4; }
 SymPlan plan_sym(int m, int k, int target_wgs);
 SymPlan plan_sym_rows(int m, int target_wgs, int rows_per_wg);
-// u[umap[j]] += sum_{I < J} colp[I][j] over row blocks of B targets (J = j's block)
-hipError_t launch_sym_colsum(const Problem &p, int B, const double *colp, hipStream_t s);
-// partial: [nsplit][m]; counters: [row_blocks] (zero); colp: [row_blocks][m]
+// The last step of a symmetric sum: u[umap[j]] += the split partials of j's row
+// block (nsplit > 1), then += sum_{I < J} colp[I][j] over row blocks of
+// pl.rows_per_wg targets (J = j's block)
+hipError_t launch_sym_colsum(const Problem &p, const SymPlan &pl, const double *partial,
+                             const double *colp, hipStream_t s);
+// partial: [nsplit][m]; counters: unused; colp: [row_blocks][m]
 hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *partial,
                                 unsigned *counters, double *colp, hipStream_t s);
 // Compact tables of a symmetric problem (the mapped rows, norms, weights and
@@ -77,6 +80,10 @@ hipError_t launch_zero_u32(unsigned *x, long n, hipStream_t s);
 // and no counters.
 hipError_t launch_split_reduce(const double *partial, int nsplit, int m, double *u,
                                const int *umap, const int *prow, hipStream_t s);
+// The same for nrhs right-hand sides: partial [nsplit][m][nrhs],
+// u[umap[i] nrhs + r] += sum over sp of partial[sp][i][r].
+hipError_t launch_split_reduce_rhs(const double *partial, int nsplit, int m, int nrhs, double *u,
+                                   const int *umap, hipStream_t s);
 
 // FP64 cross term on the matrix cores (fused_f64_mfma.hip)
 bool mfma_f64_eligible(int type, int k);

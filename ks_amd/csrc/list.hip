@@ -71,9 +71,7 @@ struct ListArgs {
   const int *amap, *bmap, *wmap;
   const double *XA, *XA2, *XB, *XB2, *w;
   double *partial;     // [ntiles][64]
-  unsigned *rbcount;   // [nrowblocks], zero between launches
-  unsigned *head;      // tile counter
-  unsigned *done;      // waves that have left
+  unsigned *head;      // tile counter (reset by list_rowsum_kernel after each run)
   double *R;           // [total_a + 64] per-(task, row) totals, then 64 sink entries
   long rsink;          // total_a: where the lanes past a row block's end write
 };
@@ -490,35 +488,39 @@ __global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void lis
       case MULTIQUADRATIC: s = run_tile<MULTIQUADRATIC, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
       default: s = run_tile<EPANECHNIKOV, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
     }
-    bool last = rb.nsplit == 1;  // wave-uniform
-    if (!last) {
+    // A row block of one slice writes its totals; a split one writes the
+    // tile's partials, which list_rowsum_kernel adds in slice order after the
+    // launch (a device-scope fence and counter here would write back and
+    // invalidate this XCD's L2 on gfx950, once per tile).  Lanes past the
+    // block's last row write the sink entries R[rsink + lane].
+    if (rb.nsplit == 1) {  // wave-uniform
+      const int pos = rb.row0 + lane;
+      L.R[pos < T.m ? T.a0 + pos : L.rsink + lane] = s;
+    } else {
       L.partial[(long)t * 64 + lane] = s;
-      __threadfence();
-      const unsigned prev = (unsigned)__builtin_amdgcn_readfirstlane(
-          (int)atomicAdd(&L.rbcount[tl.x], lane == 0 ? 1u : 0u));
-      last = prev == (unsigned)(rb.nsplit - 1);
-      if (last) {
-        __threadfence();
-        double tot = 0.0;
-        for (int sp = 0; sp < rb.nsplit; ++sp)
-          tot += L.partial[(long)(rb.tile_base + sp) * 64 + lane];
-        s = tot;
-        L.rbcount[tl.x] = 0u;  // every lane stores the same zero
-      }
-    }
-    // lanes past the block's last row write the sink entries R[rsink + lane]
-    const int pos = rb.row0 + lane;
-    if (last) L.R[pos < T.m ? T.a0 + pos : L.rsink + lane] = s;
-  }
-  // the last wave to leave resets the tile counter for the next launch
-  if (lane == 0) {
-    __threadfence();
-    unsigned d = atomicAdd(L.done, 1u);
-    if (d == gridDim.x * 4u - 1u) {
-      *L.head = 0u;
-      *L.done = 0u;
     }
   }
+}
+
+// After a batch's persistent launches: R of every split row block is the sum
+// of its tiles' partials in slice order (one wave per row block), and the
+// launches' tile counters are reset for the next run.
+__global__ __launch_bounds__(256) void list_rowsum_kernel(const RowBlock *__restrict__ rbs, int nrb,
+                                                          const DevTask *__restrict__ tasks,
+                                                          const double *__restrict__ partial,
+                                                          double *R, unsigned *heads) {
+  const int lane = threadIdx.x & 63;
+  const int b = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (blockIdx.x == 0 && threadIdx.x < 6) heads[threadIdx.x] = 0u;
+  if (b >= nrb) return;
+  const RowBlock rb = rbs[b];
+  if (rb.nsplit == 1) return;
+  const DevTask T = tasks[rb.task];
+  const int pos = rb.row0 + lane;
+  if (pos >= T.m) return;
+  double tot = 0.0;
+  for (int sp = 0; sp < rb.nsplit; ++sp) tot += partial[(long)(rb.tile_base + sp) * 64 + lane];
+  R[T.a0 + pos] = tot;
 }
 
 // The compact source pool of a plan run: position i (task offset b0 + j) gets
@@ -583,6 +585,7 @@ struct ListPlan {
   // launch [0], the matrix-core launch [1] and the Laplace matrix-core launch [2]
   ListArgs A[3]{};
   int grid[3] = {0, 0, 0};
+  int nrb = 0;                                // row blocks of the persistent launches
   const int *keys = nullptr, *pos = nullptr;  // umap sorted, with positions
   std::vector<TaskDesc> mixed;                // tasks the mixed kernels run
   // k < 8 without variable-bandwidth tasks: every run first copies the
@@ -663,6 +666,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
     nkind[dt[rbs[i].task].tile] += rbs[i].nsplit;
   }
   pl->ntiles = (long)tiles.size();
+  pl->nrb = (int)rbs.size();
   if (pl->ntiles >= (1l << 31)) {
     delete pl;
     return KS_ERR_INVALID_ARG;
@@ -682,9 +686,8 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   size_t off_rbs = off_tiles + al(pl->ntiles * sizeof(int2));
   size_t off_tasks = off_rbs + al(rbs.size() * sizeof(RowBlock));
   size_t off_part = off_tasks + al(dt.size() * sizeof(DevTask));
-  size_t off_cnt = off_part + al((size_t)pl->ntiles * 64 * sizeof(double));
-  size_t off_head = off_cnt + al(rbs.size() * sizeof(unsigned));
-  size_t off_R = off_head + 256;  // three launches' {head, done} counters
+  size_t off_head = off_part + al((size_t)pl->ntiles * 64 * sizeof(double));
+  size_t off_R = off_head + 256;  // the three launches' tile counters (two words each)
   size_t off_iota = off_R + al((size_t)(total_a + 64) * sizeof(double));  // + 64 sink entries
   size_t off_kout = off_iota + al((size_t)total_a * sizeof(int));
   size_t off_vout = off_kout + al((size_t)total_a * sizeof(int));
@@ -703,12 +706,13 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   }
   pl->buf = buf;
   int rc = KS_OK;
-  // counters and R start at zero (the kernel leaves its counters at zero after
-  // every run, and R rows of tasks without sources are never written)
+  // the tile counters and R start at zero (list_rowsum_kernel resets the
+  // counters after every run, and R rows of tasks without sources are never
+  // written)
   if (hipMemcpyAsync(buf + off_tiles, tiles.data(), pl->ntiles * sizeof(int2), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(buf + off_rbs, rbs.data(), rbs.size() * sizeof(RowBlock), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(buf + off_tasks, dt.data(), dt.size() * sizeof(DevTask), hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemsetAsync(buf + off_cnt, 0, off_iota - off_cnt, s) != hipSuccess)
+      hipMemsetAsync(buf + off_head, 0, off_iota - off_head, s) != hipSuccess)
     rc = KS_ERR_DEVICE;
   for (int kind = 0; kind < 3; ++kind) {
     ListArgs &A = pl->A[kind];
@@ -719,9 +723,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
     A.rbs = (const RowBlock *)(buf + off_rbs);
     A.tasks = (const DevTask *)(buf + off_tasks);
     A.partial = (double *)(buf + off_part);
-    A.rbcount = (unsigned *)(buf + off_cnt);
     A.head = (unsigned *)(buf + off_head) + 2 * kind;
-    A.done = A.head + 1;
     A.R = (double *)(buf + off_R);
     A.rsink = total_a;
   }
@@ -821,6 +823,8 @@ int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
       else launch_list<0, 0>(A, g, s);
     }
     const double *R = pl.A[0].R;
+    hipLaunchKernelGGL(list_rowsum_kernel, dim3((unsigned)((pl.nrb + 3) / 4)), dim3(256), 0, s,
+                       pl.A[0].rbs, pl.nrb, pl.A[0].tasks, pl.A[0].partial, pl.A[0].R, pl.A[0].head);
     const long nb = (pl.total_a + 255) / 256;
     hipLaunchKernelGGL(merge_kernel, dim3((unsigned)nb), dim3(256), 0, s, pl.keys, pl.pos, R,
                        pl.total_a, L.u);

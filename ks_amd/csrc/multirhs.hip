@@ -111,25 +111,9 @@ __global__ __launch_bounds__(MR_WG) void multirhs_kernel(MrhsArgs A) {
     for (int r = 0; r < RMAX; ++r)
       if (r < nr) pr[r] = acc[r];
   }
-  __threadfence();
-  __syncthreads();
-  __shared__ int s_last;
-  if (tid == 0) {
-    unsigned prev = atomicAdd(&A.counters[blockIdx.x], 1u);
-    s_last = (prev == (unsigned)(A.nsplit - 1));
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  if (i < m) {
-    double *ur = p.u + (long)p.umap[i] * nr;
-    for (int r = 0; r < nr; ++r) {
-      double s = 0.0;
-      for (int sp = 0; sp < A.nsplit; ++sp) s += A.partial[((long)sp * m + i) * nr + r];
-      ur[r] += s;
-    }
-  }
-  if (tid == 0) A.counters[blockIdx.x] = 0u;
+  // the splits' partials are added in split order by a reduction kernel
+  // launched next (no device-scope fence here: on gfx950 it would write back
+  // and invalidate this XCD's L2)
 }
 
 // Matrix-core form (9 <= k <= 64, every type but Laplace, nrhs <= 16 NT): the
@@ -308,26 +292,9 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
       if (g == 0 && i < m) A.partial[((long)split * m + i) * nr + 16 * NT] = ux[rb];
     }
   }
-  __threadfence();
-  __syncthreads();
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    const unsigned prev = atomicAdd(&A.counters[rblk], 1u);
-    s_last = (prev == (unsigned)(A.nsplit - 1));
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  const int base = rblk * MRM_ROWS;
-  for (int e = threadIdx.x; e < MRM_ROWS * nr; e += MR_WG) {
-    const int i = base + e / nr, r = e % nr;
-    if (i < m) {
-      double s = 0.0;
-      for (int sp = 0; sp < A.nsplit; ++sp) s += A.partial[((long)sp * m + i) * nr + r];
-      p.u[(long)p.umap[i] * nr + r] += s;
-    }
-  }
-  if (threadIdx.x == 0) A.counters[rblk] = 0u;
+  // the splits' partials are added in split order by a reduction kernel
+  // launched next (no device-scope fence here: on gfx950 it would write back
+  // and invalidate this XCD's L2)
 }
 
 template <int TYPE, int KC, int NT, bool XR>
@@ -410,6 +377,8 @@ int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s) {
       case MULTIQUADRATIC: e = mrm_nt<MULTIQUADRATIC>(A, plan.row_blocks, s); break;
       default: e = mrm_nt<EPANECHNIKOV>(A, plan.row_blocks, s); break;
     }
+    if (e == hipSuccess && plan.nsplit > 1)
+      e = launch_split_reduce_rhs(partial, plan.nsplit, p.m, nrhs, p.u, p.umap, s);
     return e == hipSuccess ? KS_OK : KS_ERR_DEVICE;
   }
   switch (p.type) {
@@ -422,6 +391,8 @@ int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s) {
     case MULTIQUADRATIC: e = mr_r<MULTIQUADRATIC>(A, plan.row_blocks, s); break;
     default: e = mr_r<EPANECHNIKOV>(A, plan.row_blocks, s); break;
   }
+  if (e == hipSuccess && plan.nsplit > 1)
+    e = launch_split_reduce_rhs(partial, plan.nsplit, p.m, nrhs, p.u, p.umap, s);
   return e == hipSuccess ? KS_OK : KS_ERR_DEVICE;
 }
 

@@ -444,25 +444,9 @@ __global__ __launch_bounds__(MX_WG, (KC <= 1 && NT <= 1) ? 3 : 2) void multirhs_
     if (i < m && (cc == 16 * NT || cc < nr - 1))
       A.partial[((long)split * m + i) * nr + r] = Us[rr][cc];
   }
-  __threadfence();
-  __syncthreads();
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    const unsigned prev = atomicAdd(&A.counters[rblk], 1u);
-    s_last = (prev == (unsigned)(A.nsplit - 1));
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  for (int e = threadIdx.x; e < MX_ROWS * nr; e += MX_WG) {
-    const int i = base + e / nr, r = e % nr;
-    if (i < m) {
-      double s = 0.0;
-      for (int sp = 0; sp < A.nsplit; ++sp) s += A.partial[((long)sp * m + i) * nr + r];
-      p.u[(long)p.umap[i] * nr + r] += s;
-    }
-  }
-  if (threadIdx.x == 0) A.counters[rblk] = 0u;
+  // the splits' partials are added in split order by a reduction kernel
+  // launched next (no device-scope fence here: on gfx950 it would write back
+  // and invalidate this XCD's L2)
 }
 
 inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -605,6 +589,8 @@ int enqueue_multirhs_mixed(const Problem &p, int nrhs, const double *centre, hip
     case 3: e = mx_nt<3>(A, NT, plan.row_blocks, s); break;
     default: e = mx_nt<4>(A, NT, plan.row_blocks, s); break;
   }
+  if (e == hipSuccess && nsplit > 1)
+    e = launch_split_reduce_rhs(A.partial, nsplit, m, nrhs, p.u, p.umap, s);
   return e == hipSuccess ? KS_OK : hip_failure(e, "mixed multi-RHS launch");
 }
 

@@ -47,6 +47,26 @@ hipError_t launch_split_reduce(const double *partial, int nsplit, int m, double 
   return hipGetLastError();
 }
 
+__global__ void split_reduce_rhs_kernel(const double *__restrict__ partial, int nsplit, int m,
+                                        int nrhs, double *u, const int *__restrict__ umap) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)m * nrhs) return;
+  const long i = e / nrhs;
+  const int r = (int)(e - i * nrhs);
+  double acc = 0.0;
+  for (int sp = 0; sp < nsplit; ++sp) acc += partial[(long)sp * m * nrhs + e];
+  u[(long)umap[i] * nrhs + r] += acc;
+}
+
+hipError_t launch_split_reduce_rhs(const double *partial, int nsplit, int m, int nrhs, double *u,
+                                   const int *umap, hipStream_t s) {
+  const long tot = (long)m * nrhs;
+  if (tot <= 0) return hipSuccess;
+  hipLaunchKernelGGL(split_reduce_rhs_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                     partial, nsplit, m, nrhs, u, umap);
+  return hipGetLastError();
+}
+
 }  // namespace gsks
 
 extern "C" int ks_sqnorm_device(const double *X, long nx, int k, double *X2, void *stream) {
