@@ -239,8 +239,10 @@ def bench_list(args, d, L):
 
 
 def bench_grad(args, d):
-    """One step = the forward summation and the whole gradient of L = g.u
-    (dXA, dXB, dw, dh: two multi-right-hand-side passes that recompute K).
+    """One step = the forward u = K w and the whole gradient of L = g.u (dXA,
+    dXB, dw, dh): two multi-right-hand-side summations that recompute K, the
+    first of which (k + 1 right-hand sides [w b, w]) also yields u, as the
+    autograd forward runs it when XA or h needs a gradient.
     Precision auto / mixed_fp32: the mixed form (FP32 matrix cores on centred
     points where the block bound allows, FP64 elsewhere) beside the FP64 form,
     with the largest relative L2 error of u, dXA, dXB, dw and dh against the
@@ -259,8 +261,8 @@ def bench_grad(args, d):
     want = "mixed_fp32" if args.precision in ("auto", "mixed_bf16x3") else args.precision
 
     def step(prec):
-        u = AG.gaussian_summation_forward(XA, XB, w, h, prec)
-        return (u,) + AG.gaussian_summation_grad(XA, XB, w, g, h, prec)
+        u, state = AG.gaussian_summation_forward_state(XA, XB, w, h, prec)
+        return (u,) + AG.gaussian_summation_backward(XA, XB, w, g, h, state, prec)
 
     def timed(prec, steps, warmup):
         for _ in range(warmup):
@@ -316,10 +318,10 @@ def bench_grad(args, d):
                         "traffic": None, "flops_per_eval": fpe,
                         "flop_model": "2k + 37 forward (test/test_dgsks.c:254-283) + 4k + 4 "
                                       "backward", "kernel_ms": kms,
-                        "kernel": ("gsks::multirhs_mixed_kernel (forward with 1 and both "
-                                   "gradient passes with k + 1 right-hand sides)" if mixed
-                                   else "gsks::fused_f64_mfma_kernel + "
-                                        "gsks::multirhs_mfma_kernel (2 passes)")}}
+                        "kernel": ("gsks::multirhs_mixed_kernel (2 passes of k + 1 "
+                                   "right-hand sides)" if mixed
+                                   else "gsks::multirhs_mfma_kernel (2 passes of k + 1 "
+                                        "right-hand sides)")}}
     out.update(extra)
     print(json.dumps(out), flush=True)
 

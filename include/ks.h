@@ -252,6 +252,23 @@ int ks_gauss_grad_device(int m, int n, int k, const double *XA, const double *XB
                          double *dXA, double *dXB, double *dw, double *dh, int precision,
                          void *stream);
 
+/* The two halves of ks_gauss_grad_device, for a training step (no counterpart
+ * in the reference, whose summations have no gradient).  The forward half
+ * computes u = K w (u may be NULL) and writes the gradient state -- V = K [w b]
+ * and u as [m][k + 1] rows, then the k coordinates of the frame the precision
+ * works in -- into `state` (m (k + 1) + k doubles) from one summation of k + 1
+ * right-hand sides.  The backward half takes g and the same points, weights,
+ * h and precision, runs the second summation and forms dXA, dXB, dw and dh
+ * (any may be NULL); state may be NULL when dXA and dh are.  A step is then
+ * two summations instead of three.  Device pointers, enqueued on `stream`. */
+int ks_gauss_grad_forward_device(int m, int n, int k, const double *XA, const double *XB,
+                                 const double *w, double h, double *u, double *state,
+                                 int precision, void *stream);
+int ks_gauss_grad_backward_device(int m, int n, int k, const double *XA, const double *XB,
+                                  const double *w, const double *g, double h,
+                                  const double *state, double *dXA, double *dXB,
+                                  double *dw, double *dh, int precision, void *stream);
+
 /* Batched summation: ntask independent summations over shared point pools,
  * one persistent launch.  Task t owns the ranges [a_off[t], a_off[t+1]) of
  * amap/umap and [b_off[t], b_off[t+1]) of bmap/wmap and uses kernels[t]

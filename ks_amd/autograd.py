@@ -17,6 +17,13 @@ elsewhere), the form for unclustered points, whose pairs are mostly near.
 "mixed_bf16x3" runs the forward through dgsks_device's split-BF16 form (for
 clustered points, where most blocks are far) and the backward as
 "mixed_fp32".
+
+When XA or h needs a gradient (and grad mode is on), the "fp64" and
+"mixed_fp32" forwards run the gradient's first summation instead
+(ks_gauss_grad_forward_device: k + 1 right-hand sides [w b, w] -> [V, u]) and
+keep V, so the backward runs the second summation only: a training step costs
+two summations, not three.  Otherwise the forward is the plain summation and
+the backward, which then needs no V, runs the second summation only.
 """
 from __future__ import annotations
 
@@ -76,11 +83,16 @@ def gaussian_summation_forward(XA, XB, w, h: float, precision: str = "fp64"):
     return u
 
 
-def gaussian_summation_grad(XA, XB, w, g, h: float, precision: str = "fp64"):
-    """(dXA, dXB, dw, dh) of L = g . u through the hand-written HIP gradient
-    (FP64, or the mixed form for either mixed precision)."""
+def _prec_code(precision):
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+    return _lib.KS_PREC_FP64 if precision == "fp64" else _lib.KS_PREC_MIXED_FP32
+
+
+def gaussian_summation_grad(XA, XB, w, g, h: float, precision: str = "fp64"):
+    """(dXA, dXB, dw, dh) of L = g . u through the hand-written HIP gradient
+    (FP64, or the mixed form for either mixed precision): both summations."""
+    code = _prec_code(precision)
     L = _lib.lib()
     XA, XB, w, g = (t.contiguous() for t in (XA, XB, w, g))
     m, k = XA.shape
@@ -90,38 +102,98 @@ def gaussian_summation_grad(XA, XB, w, g, h: float, precision: str = "fp64"):
     dXB = torch.empty_like(XB)
     dw = torch.empty(n, dtype=torch.float64, device=dev)
     dh = torch.empty(1, dtype=torch.float64, device=dev)
-    code = _lib.KS_PREC_FP64 if precision == "fp64" else _lib.KS_PREC_MIXED_FP32
     rc = L.ks_gauss_grad_device(m, n, k, _p(XA), _p(XB), _p(w), _p(g), float(h), _p(dXA),
                                 _p(dXB), _p(dw), _p(dh), code, _stream(dev))
     _lib.check(rc, "ks_gauss_grad_device")
     return dXA, dXB, dw, dh
 
 
+def gaussian_summation_forward_state(XA, XB, w, h: float, precision: str = "fp64"):
+    """(u, state): u = K w and the gradient state ([V | u] rows and the frame
+    origin, m (k + 1) + k doubles) from the gradient's first summation."""
+    code = _prec_code(precision)
+    L = _lib.lib()
+    XA, XB, w = (t.contiguous() for t in (XA, XB, w))
+    m, k = XA.shape
+    n = XB.shape[0]
+    dev = XA.device
+    u = torch.empty(m, dtype=torch.float64, device=dev)
+    state = torch.empty(m * (k + 1) + k, dtype=torch.float64, device=dev)
+    rc = L.ks_gauss_grad_forward_device(m, n, k, _p(XA), _p(XB), _p(w), float(h), _p(u),
+                                        _p(state), code, _stream(dev))
+    _lib.check(rc, "ks_gauss_grad_forward_device")
+    return u, state
+
+
+def gaussian_summation_backward(XA, XB, w, g, h: float, state=None, precision: str = "fp64",
+                                want=(True, True, True, True)):
+    """The gradient's second summation and finishing kernels: (dXA, dXB, dw,
+    dh), None where `want` is False.  dXA and dh need the state that
+    gaussian_summation_forward_state returned for the same inputs."""
+    code = _prec_code(precision)
+    L = _lib.lib()
+    XA, XB, w, g = (t.contiguous() for t in (XA, XB, w, g))
+    m, k = XA.shape
+    n = XB.shape[0]
+    dev = XA.device
+    if (want[0] or want[3]) and state is None:
+        raise ValueError("dXA and dh need the forward state")
+    out = [torch.empty_like(XA) if want[0] else None,
+           torch.empty_like(XB) if want[1] else None,
+           torch.empty(n, dtype=torch.float64, device=dev) if want[2] else None,
+           torch.empty(1, dtype=torch.float64, device=dev) if want[3] else None]
+    ptr = [_p(t) if t is not None else None for t in out]
+    rc = L.ks_gauss_grad_backward_device(m, n, k, _p(XA), _p(XB), _p(w), _p(g), float(h),
+                                         _p(state) if state is not None else None, *ptr, code,
+                                         _stream(dev))
+    _lib.check(rc, "ks_gauss_grad_backward_device")
+    return tuple(out)
+
+
 class GaussianSummation(torch.autograd.Function):
     """u = K(XA, XB; h) w.  `h` is a 0-d or 1-element float64 tensor (it may
-    require grad); `precision` selects the forward and backward forms."""
+    require grad); `precision` selects the forward and backward forms;
+    `for_grad` (grad mode was on) lets the forward keep the gradient state."""
 
     @staticmethod
-    def forward(ctx, XA, XB, w, h, precision="fp64"):
+    def forward(ctx, XA, XB, w, h, precision="fp64", for_grad=True):
         _check(XA, "XA", 2)
         _check(XB, "XB", 2)
         _check(w, "w", 1)
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         hv = float(h.detach().reshape(-1)[0].item()) if torch.is_tensor(h) else float(h)
-        ctx.save_for_backward(XA, XB, w)
+        need = ctx.needs_input_grad
         ctx.h = hv
         ctx.h_is_tensor = torch.is_tensor(h)
         ctx.h_shape = h.shape if torch.is_tensor(h) else None
         ctx.precision = precision
+        ctx.with_state = bool(for_grad and (need[0] or need[3]) and precision != "mixed_bf16x3")
+        if ctx.with_state:
+            u, state = gaussian_summation_forward_state(XA, XB, w, hv, precision)
+            ctx.save_for_backward(XA, XB, w, state)
+            return u
+        ctx.save_for_backward(XA, XB, w)
         return gaussian_summation_forward(XA, XB, w, hv, precision)
 
     @staticmethod
     def backward(ctx, g):
-        XA, XB, w = ctx.saved_tensors
-        dXA, dXB, dw, dh = gaussian_summation_grad(XA, XB, w, g.to(torch.float64), ctx.h,
-                                                   ctx.precision)
-        dh_out = dh.reshape(ctx.h_shape) if ctx.h_is_tensor else None
-        return dXA, dXB, dw, dh_out, None
+        need = ctx.needs_input_grad
+        g = g.to(torch.float64)
+        if ctx.with_state:
+            XA, XB, w, state = ctx.saved_tensors
+            dXA, dXB, dw, dh = gaussian_summation_backward(
+                XA, XB, w, g, ctx.h, state, ctx.precision, want=tuple(need[:4]))
+        elif need[0] or need[3]:  # mixed_bf16x3: its forward kept no state
+            XA, XB, w = ctx.saved_tensors
+            dXA, dXB, dw, dh = gaussian_summation_grad(XA, XB, w, g, ctx.h, ctx.precision)
+        else:
+            XA, XB, w = ctx.saved_tensors
+            dXA, dXB, dw, dh = gaussian_summation_backward(
+                XA, XB, w, g, ctx.h, None, ctx.precision, want=(False, need[1], need[2], False))
+        dh_out = dh.reshape(ctx.h_shape) if (ctx.h_is_tensor and dh is not None) else None
+        return dXA, dXB, dw, dh_out, None, None
 
 
 def gaussian_summation(XA, XB, w, h, precision="fp64"):
-    return GaussianSummation.apply(XA, XB, w, h, precision)
+    return GaussianSummation.apply(XA, XB, w, h, precision, torch.is_grad_enabled())

@@ -408,7 +408,8 @@ __global__ void grad_rhs(const double *X, const double *s, int n, int k, double 
   W[j * R + k] = sj;
 }
 
-// dXA, and per-row dh terms (summed afterwards in a fixed order)
+// dXA, and per-row dh terms (summed afterwards in a fixed order); either output
+// may be NULL
 __global__ void grad_fin_a(const double *XA, const double *XA2, const double *g, const double *U1,
                            int m, int k, double h, double *dXA, double *dh_rows) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -420,13 +421,13 @@ __global__ void grad_fin_a(const double *XA, const double *XA2, const double *g,
   double av = 0.0;
   for (int q = 0; q < k; ++q) {
     const double a = XA[i * k + q], V = u[q];
-    dXA[i * k + q] = f * (ui * a - V);
+    if (dXA) dXA[i * k + q] = f * (ui * a - V);
     av = fma(a, V, av);
   }
-  dh_rows[i] = g[i] * (XA2[i] * ui - 2.0 * av);
+  if (dh_rows) dh_rows[i] = g[i] * (XA2[i] * ui - 2.0 * av);
 }
 
-// dXB, dw, and per-source dh terms w_j |b_j|^2 dw_j
+// dXB, dw, and per-source dh terms w_j |b_j|^2 dw_j; any output may be NULL
 __global__ void grad_fin_b(const double *XB, const double *XB2, const double *w, const double *U2,
                            int n, int k, double h, double *dXB, double *dw, double *dh_rows) {
   long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -434,9 +435,10 @@ __global__ void grad_fin_b(const double *XB, const double *XB2, const double *w,
   const int R = k + 1;
   const double *u = U2 + j * R;
   const double f = w[j] / (h * h), dwj = u[k];
-  dw[j] = dwj;
-  for (int q = 0; q < k; ++q) dXB[j * k + q] = f * (u[q] - dwj * XB[j * k + q]);
-  dh_rows[j] = w[j] * XB2[j] * dwj;
+  if (dw) dw[j] = dwj;
+  if (dXB)
+    for (int q = 0; q < k; ++q) dXB[j * k + q] = f * (u[q] - dwj * XB[j * k + q]);
+  if (dh_rows) dh_rows[j] = w[j] * XB2[j] * dwj;
 }
 
 // deterministic sum of the per-point dh terms: one workgroup of 1024, each lane
@@ -501,6 +503,190 @@ extern "C" int dgsks_multirhs_device(const ks_t *kernel, int m, int n, int k, in
   return rc == KS_OK ? KS_OK : raise_error(rc, false, "dgsks_multirhs_device(): %s\n", err_string(rc));
 }
 
+namespace gsks {
+namespace {
+
+// Gradient state: [m][k + 1] rows (V_i = sum_j K_ij w_j b_j, then
+// u_i = sum_j K_ij w_j, in the frame the precision works in), then the k
+// coordinates of that frame's origin (zeros for FP64, the mean of both point
+// sets for the mixed form).
+size_t grad_state_elems(int m, int k) { return (size_t)m * (k + 1) + k; }
+
+int grad_check(const char *name, int m, int n, int k, double h, int precision, bool &mixed) {
+  if (m <= 0 || n <= 0 || k <= 0 || k > 70 || !(h > 0.0))
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): invalid argument\n", name);
+  mixed = precision == KS_PREC_MIXED_FP32;
+  if (!mixed && precision != KS_PREC_FP64)
+    return raise_error(KS_ERR_INVALID_ARG, false,
+                       "%s(): precision must be KS_PREC_FP64 or KS_PREC_MIXED_FP32\n", name);
+  if (mixed && k + 1 > kMixedMrhsMaxRhs)
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): the mixed gradient takes k <= %d\n",
+                       name, kMixedMrhsMaxRhs - 1);
+  if (ensure_device() != KS_OK)
+    return raise_error(KS_ERR_DEVICE, false, "%s(): no GPU available\n", name);
+  return KS_OK;
+}
+
+// Carves one stream-ordered allocation into 256-byte aligned arrays.  A layout
+// function runs twice: on a null base to measure, then on the allocation.
+struct Carve {
+  char *base = nullptr;
+  size_t used = 0;
+  template <class T>
+  T *take(size_t count) {
+    T *r = base ? reinterpret_cast<T *>(base + used) : nullptr;
+    used += (sizeof(T) * count + 255) & ~(size_t)255;
+    return r;
+  }
+};
+
+template <class Layout>
+int carve_alloc(Layout &&layout, char **buf, hipStream_t s) {
+  Carve c;
+  layout(c);
+  if (hipMallocAsync((void **)buf, c.used, s) != hipSuccess) {
+    (void)hipGetLastError();
+    *buf = nullptr;
+    return KS_ERR_ALLOC;
+  }
+  c.base = *buf;
+  c.used = 0;
+  layout(c);
+  return KS_OK;
+}
+
+__global__ void grad_take_u(const double *state, int m, int k, double *u) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) u[i] = state[i * (k + 1) + k];
+}
+
+// The coordinates both passes work in and their squared norms.  FP64: the
+// caller's points.  Mixed: both sets translated by `origin` (written here from
+// the mean when `centre_out`, else read), so the FP32 cross terms see the
+// smallest coordinates the data allows; the gradient does not change.
+struct GradFrame {
+  const double *A = nullptr, *B = nullptr;
+  double *A2 = nullptr, *B2 = nullptr, *Ac = nullptr, *Bc = nullptr, *zc = nullptr;
+  double *cwork = nullptr;
+  int *ident = nullptr;
+
+  void layout(Carve &c, int m, int n, int k, bool mixed, bool centre_out) {
+    A2 = c.take<double>(m);
+    B2 = c.take<double>(n);
+    ident = c.take<int>(m > n ? m : n);
+    if (mixed) {
+      Ac = c.take<double>((size_t)m * k);
+      Bc = c.take<double>((size_t)n * k);
+      zc = c.take<double>(64);
+      if (centre_out) cwork = c.take<double>(centre_work_elems((long)m + n));
+    }
+  }
+
+  int setup(const double *XA, int m, const double *XB, int n, int k, bool mixed, double *origin,
+            bool centre_out, hipStream_t s) {
+    const int mx = m > n ? m : n;
+    int rc = KS_OK;
+    auto chain = [&](int r) {
+      if (rc == KS_OK) rc = r;
+    };
+    hipLaunchKernelGGL(iota_i32, dim3((mx + 255) / 256), dim3(256), 0, s, ident, mx);
+    A = XA;
+    B = XB;
+    if (mixed) {
+      hipLaunchKernelGGL(zero_f64, dim3(1), dim3(64), 0, s, zc, 64l);
+      if (centre_out) chain(enqueue_centre(XA, m, XB, n, k, ident, cwork, origin, s));
+      chain(enqueue_shift(XA, m, k, origin, Ac, s));
+      chain(enqueue_shift(XB, n, k, origin, Bc, s));
+      A = Ac;
+      B = Bc;
+    } else if (centre_out) {
+      hipLaunchKernelGGL(zero_f64, dim3((k + 63) / 64), dim3(64), 0, s, origin, (long)k);
+    }
+    chain(ks_sqnorm_device(A, m, k, A2, s));
+    chain(ks_sqnorm_device(B, n, k, B2, s));
+    return rc;
+  }
+
+  // One summation of R = k + 1 right-hand sides from sources S to targets T.
+  int pass(bool mixed, const double *T, const double *T2, int mt, const double *S,
+           const double *S2, int ns, int k, double h, const double *W, double *U,
+           hipStream_t s) const {
+    const int R = k + 1;
+    hipLaunchKernelGGL(zero_f64, dim3((unsigned)(((long)mt * R + 255) / 256)), dim3(256), 0, s,
+                       U, (long)mt * R);
+    Problem p;
+    p.m = mt; p.n = ns; p.k = k; p.type = GAUSSIAN;
+    p.P = KerParams{-1.0 / (2.0 * h * h), 0.0, 0.0, 0, POW_GENERAL};
+    p.u = U; p.umap = ident; p.XA = T; p.XA2 = T2; p.amap = ident;
+    p.XB = S; p.XB2 = S2; p.bmap = ident; p.w = W; p.wmap = ident;
+    p.hi = p.hj = nullptr;
+    return mixed ? enqueue_multirhs_mixed(p, R, zc, s) : enqueue_multirhs(p, R, s);
+  }
+};
+
+// Pass 1: state = K [w b, w] (and its origin); u = the last column.
+int grad_forward(int m, int n, int k, const double *XA, const double *XB, const double *w,
+                 double h, double *u, double *state, bool mixed, hipStream_t s) {
+  GradFrame F;
+  double *W1 = nullptr;
+  char *buf = nullptr;
+  int rc = carve_alloc([&](Carve &c) {
+    F.layout(c, m, n, k, mixed, true);
+    W1 = c.take<double>((size_t)n * (k + 1));
+  }, &buf, s);
+  if (rc != KS_OK) return rc;
+  double *origin = state + (size_t)m * (k + 1);
+  rc = F.setup(XA, m, XB, n, k, mixed, origin, true, s);
+  hipLaunchKernelGGL(grad_rhs, dim3((n + 255) / 256), dim3(256), 0, s, F.B, w, n, k, W1);
+  if (rc == KS_OK) rc = F.pass(mixed, F.A, F.A2, m, F.B, F.B2, n, k, h, W1, state, s);
+  if (rc == KS_OK && u)
+    hipLaunchKernelGGL(grad_take_u, dim3((m + 255) / 256), dim3(256), 0, s, state, m, k, u);
+  if (rc == KS_OK && hipGetLastError() != hipSuccess) rc = KS_ERR_DEVICE;
+  (void)hipFreeAsync(buf, s);
+  return rc;
+}
+
+// Pass 2: [Q, dw] = K^T [g a, g], then the finishing kernels.  Without a state
+// only dXB and dw can be formed (dXA and dh need pass 1).
+int grad_backward(int m, int n, int k, const double *XA, const double *XB, const double *w,
+                  const double *g, double h, const double *state, double *dXA, double *dXB,
+                  double *dw, double *dh, bool mixed, hipStream_t s) {
+  const bool need_a = dXA || dh;
+  if (need_a && !state) return KS_ERR_INVALID_ARG;
+  if (!need_a && !dXB && !dw) return KS_OK;
+  const int R = k + 1;
+  GradFrame F;
+  double *W2 = nullptr, *U2 = nullptr, *hrows = nullptr, *cen = nullptr;
+  char *buf = nullptr;
+  const bool centre_out = mixed && !state;  // no stored origin: take the mean again
+  int rc = carve_alloc([&](Carve &c) {
+    F.layout(c, m, n, k, mixed, centre_out);
+    W2 = c.take<double>((size_t)m * R);
+    U2 = c.take<double>((size_t)n * R);
+    if (dh) hrows = c.take<double>((size_t)m + n);
+    if (centre_out) cen = c.take<double>(64);
+  }, &buf, s);
+  if (rc != KS_OK) return rc;
+  double *origin = state ? const_cast<double *>(state) + (size_t)m * R : cen;
+  rc = F.setup(XA, m, XB, n, k, mixed, origin, centre_out, s);
+  hipLaunchKernelGGL(grad_rhs, dim3((m + 255) / 256), dim3(256), 0, s, F.A, g, m, k, W2);
+  if (rc == KS_OK) rc = F.pass(mixed, F.B, F.B2, n, F.A, F.A2, m, k, h, W2, U2, s);
+  if (rc == KS_OK) {
+    if (need_a)
+      hipLaunchKernelGGL(grad_fin_a, dim3((m + 255) / 256), dim3(256), 0, s, F.A, F.A2, g, state,
+                         m, k, h, dXA, hrows);
+    hipLaunchKernelGGL(grad_fin_b, dim3((n + 255) / 256), dim3(256), 0, s, F.B, F.B2, w, U2, n, k,
+                       h, dXB, dw, dh ? hrows + m : nullptr);
+    if (dh) hipLaunchKernelGGL(grad_fin_h, dim3(1), dim3(GF_WG), 0, s, hrows, m + n, h, dh);
+    if (hipGetLastError() != hipSuccess) rc = KS_ERR_DEVICE;
+  }
+  (void)hipFreeAsync(buf, s);
+  return rc;
+}
+
+}  // namespace
+}  // namespace gsks
+
 // Gradient of L = g . (K w) for the Gaussian kernel with bandwidth h and identity
 // maps; all arrays device memory, XA [m][k], XB [n][k].  Outputs dXA [m][k],
 // dXB [n][k], dw [n], dh [1] (any output may be NULL).  Temporary buffers come from
@@ -513,103 +699,64 @@ extern "C" int ks_gauss_grad_device(int m, int n, int k, const double *XA, const
                                     const double *w, const double *g, double h, double *dXA,
                                     double *dXB, double *dw, double *dh, int precision,
                                     void *stream) {
-  if (m <= 0 || n <= 0 || k <= 0 || k > 70 || !(h > 0.0) || !XA || !XB || !w || !g)
-    return raise_error(KS_ERR_INVALID_ARG, false, "ks_gauss_grad_device(): invalid argument\n");
-  const bool mixed = precision == KS_PREC_MIXED_FP32;
-  if (!mixed && precision != KS_PREC_FP64)
-    return raise_error(KS_ERR_INVALID_ARG, false,
-                       "ks_gauss_grad_device(): precision must be KS_PREC_FP64 or "
-                       "KS_PREC_MIXED_FP32\n");
-  if (mixed && k + 1 > kMixedMrhsMaxRhs)
-    return raise_error(KS_ERR_INVALID_ARG, false,
-                       "ks_gauss_grad_device(): the mixed gradient takes k <= %d\n",
-                       kMixedMrhsMaxRhs - 1);
-  if (ensure_device() != KS_OK)
-    return raise_error(KS_ERR_DEVICE, false, "ks_gauss_grad_device(): no GPU available\n");
+  const char *name = "ks_gauss_grad_device";
+  bool mixed = false;
+  int rc = grad_check(name, m, n, k, h, precision, mixed);
+  if (rc != KS_OK) return rc;
+  if (!XA || !XB || !w || !g)
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): invalid argument\n", name);
   hipStream_t s = (hipStream_t)stream;
-  const int R1 = k + 1, R2 = k + 1;
-  size_t bytes = sizeof(double) * ((size_t)m + n + (size_t)n * R1 + (size_t)m * R1 +
-                                   (size_t)m * R2 + (size_t)n * R2 + m + n + (size_t)m * k +
-                                   (size_t)n * k + n + 1) +
-                 sizeof(int) * ((size_t)(m > n ? m : n)) + 24 * 256;
-  if (mixed)
-    bytes += sizeof(double) * ((size_t)m * k + (size_t)n * k + 2 * 64 +
-                               centre_work_elems((long)m + n));
-  char *buf = nullptr;
-  if (hipMallocAsync((void **)&buf, bytes, s) != hipSuccess) {
-    (void)hipGetLastError();
-    return raise_error(KS_ERR_ALLOC, false, "ks_gauss_grad_device(): allocation failed\n");
+  double *state = nullptr;
+  if (dXA || dh) {
+    if (hipMallocAsync((void **)&state, sizeof(double) * grad_state_elems(m, k), s) !=
+        hipSuccess) {
+      (void)hipGetLastError();
+      return raise_error(KS_ERR_ALLOC, false, "%s(): allocation failed\n", name);
+    }
+    rc = grad_forward(m, n, k, XA, XB, w, h, nullptr, state, mixed, s);
   }
-  char *q = buf;
-  auto take = [&](size_t b) {
-    char *r = q;
-    q += (b + 255) & ~(size_t)255;
-    return r;
-  };
-  double *XA2 = (double *)take(sizeof(double) * m);
-  double *XB2 = (double *)take(sizeof(double) * n);
-  double *W1 = (double *)take(sizeof(double) * (size_t)n * R1);
-  double *U1 = (double *)take(sizeof(double) * (size_t)m * R1);
-  double *W2 = (double *)take(sizeof(double) * (size_t)m * R2);
-  double *U2 = (double *)take(sizeof(double) * (size_t)n * R2);
-  double *hrows = (double *)take(sizeof(double) * ((size_t)m + n));  // per target, per source
-  double *tXA = (double *)take(sizeof(double) * (size_t)m * k);
-  double *tXB = (double *)take(sizeof(double) * (size_t)n * k);
-  double *tdw = (double *)take(sizeof(double) * n);
-  double *tdh = (double *)take(sizeof(double));
-  int *ident = (int *)take(sizeof(int) * (size_t)(m > n ? m : n));
-  const int mx = m > n ? m : n;
-  int rc = KS_OK;
-  auto chain = [&](int r) {  // keeps the first failure
-    if (rc == KS_OK) rc = r;
-  };
-  hipLaunchKernelGGL(iota_i32, dim3((mx + 255) / 256), dim3(256), 0, s, ident, mx);
-  const double *A = XA, *B = XB;  // the coordinates every later step uses
-  double *zc = nullptr;
-  if (mixed) {
-    double *XAc = (double *)take(sizeof(double) * (size_t)m * k);
-    double *XBc = (double *)take(sizeof(double) * (size_t)n * k);
-    double *cen = (double *)take(sizeof(double) * 64);
-    zc = (double *)take(sizeof(double) * 64);
-    double *cwork = (double *)take(sizeof(double) * centre_work_elems((long)m + n));
-    hipLaunchKernelGGL(zero_f64, dim3(1), dim3(64), 0, s, zc, 64l);
-    chain(enqueue_centre(XA, m, XB, n, k, ident, cwork, cen, s));
-    chain(enqueue_shift(XA, m, k, cen, XAc, s));
-    chain(enqueue_shift(XB, n, k, cen, XBc, s));
-    A = XAc;
-    B = XBc;
-  }
-  chain(ks_sqnorm_device(A, m, k, XA2, s));
-  chain(ks_sqnorm_device(B, n, k, XB2, s));
-  hipLaunchKernelGGL(grad_rhs, dim3((n + 255) / 256), dim3(256), 0, s, B, w, n, k, W1);
-  hipLaunchKernelGGL(grad_rhs, dim3((m + 255) / 256), dim3(256), 0, s, A, g, m, k, W2);
-  hipLaunchKernelGGL(zero_f64, dim3((unsigned)(((long)m * R1 + 255) / 256)), dim3(256), 0, s, U1,
-                     (long)m * R1);
-  hipLaunchKernelGGL(zero_f64, dim3((unsigned)(((long)n * R2 + 255) / 256)), dim3(256), 0, s, U2,
-                     (long)n * R2);
-  Problem p1;
-  p1.m = m; p1.n = n; p1.k = k; p1.type = GAUSSIAN;
-  p1.P = KerParams{-1.0 / (2.0 * h * h), 0.0, 0.0, 0, POW_GENERAL};
-  p1.u = U1; p1.umap = ident; p1.XA = A; p1.XA2 = XA2; p1.amap = ident;
-  p1.XB = B; p1.XB2 = XB2; p1.bmap = ident; p1.w = W1; p1.wmap = ident;
-  p1.hi = p1.hj = nullptr;
-  auto pass = [&](const Problem &pp, int R) {
-    return mixed ? enqueue_multirhs_mixed(pp, R, zc, s) : enqueue_multirhs(pp, R, s);
-  };
-  if (rc == KS_OK) rc = pass(p1, R1);
-  Problem p2 = p1;
-  p2.m = n; p2.n = m;
-  p2.u = U2; p2.XA = B; p2.XA2 = XB2; p2.XB = A; p2.XB2 = XA2; p2.w = W2;
-  if (rc == KS_OK) rc = pass(p2, R2);
-  if (rc == KS_OK) {
-    hipLaunchKernelGGL(grad_fin_a, dim3((m + 255) / 256), dim3(256), 0, s, A, XA2, g, U1, m, k, h,
-                       dXA ? dXA : tXA, hrows);
-    hipLaunchKernelGGL(grad_fin_b, dim3((n + 255) / 256), dim3(256), 0, s, B, XB2, w, U2, n, k, h,
-                       dXB ? dXB : tXB, dw ? dw : tdw, hrows + m);
-    hipLaunchKernelGGL(grad_fin_h, dim3(1), dim3(GF_WG), 0, s, hrows, m + n, h, dh ? dh : tdh);
-    if (hipGetLastError() != hipSuccess) rc = KS_ERR_DEVICE;
-  }
-  (void)hipFreeAsync(buf, s);
-  if (rc != KS_OK) return raise_error(rc, false, "ks_gauss_grad_device(): %s\n", err_string(rc));
+  if (rc == KS_OK) rc = grad_backward(m, n, k, XA, XB, w, g, h, state, dXA, dXB, dw, dh, mixed, s);
+  if (state) (void)hipFreeAsync(state, s);
+  if (rc != KS_OK) return raise_error(rc, false, "%s(): %s\n", name, err_string(rc));
+  return KS_OK;
+}
+
+// The first half of ks_gauss_grad_device for a training step: u = K w (u may be
+// NULL) and the gradient state (m (k + 1) + k doubles) that
+// ks_gauss_grad_backward_device reads, from one summation of k + 1 right-hand
+// sides, so the step costs two summations in all.
+extern "C" int ks_gauss_grad_forward_device(int m, int n, int k, const double *XA,
+                                            const double *XB, const double *w, double h,
+                                            double *u, double *state, int precision,
+                                            void *stream) {
+  const char *name = "ks_gauss_grad_forward_device";
+  bool mixed = false;
+  int rc = grad_check(name, m, n, k, h, precision, mixed);
+  if (rc != KS_OK) return rc;
+  if (!XA || !XB || !w || !state)
+    return raise_error(KS_ERR_INVALID_ARG, false, "%s(): invalid argument\n", name);
+  rc = grad_forward(m, n, k, XA, XB, w, h, u, state, mixed, (hipStream_t)stream);
+  if (rc != KS_OK) return raise_error(rc, false, "%s(): %s\n", name, err_string(rc));
+  return KS_OK;
+}
+
+// The second half: the gradient of L = g . u from the state the forward half
+// wrote with the same points, weights, h and precision.  state may be NULL
+// when dXA and dh are (dXB and dw need the second summation only).
+extern "C" int ks_gauss_grad_backward_device(int m, int n, int k, const double *XA,
+                                             const double *XB, const double *w, const double *g,
+                                             double h, const double *state, double *dXA,
+                                             double *dXB, double *dw, double *dh, int precision,
+                                             void *stream) {
+  const char *name = "ks_gauss_grad_backward_device";
+  bool mixed = false;
+  int rc = grad_check(name, m, n, k, h, precision, mixed);
+  if (rc != KS_OK) return rc;
+  if (!XA || !XB || !w || !g || (!state && (dXA || dh)))
+    return raise_error(KS_ERR_INVALID_ARG, false,
+                       "%s(): invalid argument (dXA and dh need the forward state)\n", name);
+  rc = grad_backward(m, n, k, XA, XB, w, g, h, state, dXA, dXB, dw, dh, mixed,
+                     (hipStream_t)stream);
+  if (rc != KS_OK) return raise_error(rc, false, "%s(): %s\n", name, err_string(rc));
   return KS_OK;
 }

@@ -85,6 +85,100 @@ def test_reference_gradient_full_shape(torch_cuda, oracle):
     assert eA <= 1e-11 and eB <= 1e-11 and ew <= 1e-12 and eh <= 1e-10
 
 
+def test_central_differences_full_shape(torch_cuda):
+    """m = n = 32768, k = 16, h = 0.4 (the gradient workload's points, weights
+    and g): the autograd gradient, whose forward keeps the state of the first
+    summation, against central differences of the FP64 forward (eps = 1e-6) on
+    64 seeded coordinates: 24 of XA, 24 of XB, 15 of w and h.  Each difference
+    is g . (u+ - u-), the unchanged rows cancelling exactly (the forward is
+    deterministic), so its rounding stays far below the 1e-6 bar."""
+    torch = torch_cuda
+    from ks_amd import workloads as W
+    from ks_amd.autograd import gaussian_summation, gaussian_summation_forward
+    d = W.grad_gauss16()
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    XA, XB, w, g = dev(d["XA"]), dev(d["XB"]), dev(d["w"]), dev(d["g"])
+    h, (m, k), n = d["h"], d["XA"].shape, d["XB"].shape[0]
+    XA.requires_grad_(True)
+    XB.requires_grad_(True)
+    w.requires_grad_(True)
+    hh = torch.tensor(h, dtype=torch.float64, device="cuda", requires_grad=True)
+    u = gaussian_summation(XA, XB, w, hh)
+    (u * g).sum().backward()
+    grads = {"XA": XA.grad.clone(), "XB": XB.grad.clone(), "w": w.grad.clone(),
+             "h": hh.grad.reshape(1).clone()}
+    A, B, ww = XA.detach().clone(), XB.detach().clone(), w.detach().clone()
+    u0 = gaussian_summation_forward(A, B, ww, h)
+    assert float(torch.linalg.norm(u.detach() - u0) / torch.linalg.norm(u0)) <= 1e-14
+
+    rng = np.random.default_rng(21)
+    eps = 1e-6
+    worst = {}
+
+    def diff(t, idx, hv=h):
+        x = t[idx].item()
+        t[idx] = x + eps
+        up = gaussian_summation_forward(A, B, ww, hv)
+        t[idx] = x - eps
+        um = gaussian_summation_forward(A, B, ww, hv)
+        t[idx] = x
+        return float(torch.dot(g, up - um)) / (2 * eps)
+
+    picks = ([("XA", (int(i), int(p))) for i, p in zip(rng.integers(0, m, 24),
+                                                        rng.integers(0, k, 24))] +
+             [("XB", (int(j), int(p))) for j, p in zip(rng.integers(0, n, 24),
+                                                        rng.integers(0, k, 24))] +
+             [("w", (int(j),)) for j in rng.integers(0, n, 15)])
+    with torch.no_grad():
+        for name, idx in picks:
+            fd = diff({"XA": A, "XB": B, "w": ww}[name], idx)
+            an = grads[name][idx].item()
+            scale = float(grads[name].abs().max())
+            err = abs(an - fd) / max(abs(fd), 1e-3 * scale)
+            worst[name] = max(worst.get(name, 0.0), err)
+            assert err <= 1e-6, (name, idx, an, fd)
+        fd = (float(torch.dot(g, gaussian_summation_forward(A, B, ww, h + eps))) -
+              float(torch.dot(g, gaussian_summation_forward(A, B, ww, h - eps)))) / (2 * eps)
+        worst["h"] = abs(grads["h"].item() - fd) / abs(fd)
+        assert worst["h"] <= 1e-6, (grads["h"].item(), fd)
+    print("full-shape central differences, worst relative error:",
+          {a: f"{b:.1e}" for a, b in worst.items()})
+
+
+def test_gradient_halves(torch_cuda):
+    """The forward half (u and the state) and the backward half against the
+    one-shot gradient, which runs the same two summations: identical bits;
+    u against the plain summation; the backward without a state (dXB and dw
+    only) identical too; autograd with only XB and w requiring grad."""
+    torch = torch_cuda
+    from ks_amd import autograd as AG
+    rng = np.random.default_rng(31)
+    m, n, k, h = 1500, 1237, 12, 0.5
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    XA, XB = dev(rng.random((m, k))), dev(rng.random((n, k)))
+    w, g = dev(rng.standard_normal(n)), dev(rng.standard_normal(m))
+    for prec in ("fp64", "mixed_fp32"):
+        u, state = AG.gaussian_summation_forward_state(XA, XB, w, h, prec)
+        ref_u = AG.gaussian_summation_forward(XA, XB, w, h, "fp64")
+        tol = 1e-14 if prec == "fp64" else 1e-6
+        assert float(torch.linalg.norm(u - ref_u) / torch.linalg.norm(ref_u)) <= tol, prec
+        assert torch.equal(state[m * (k + 1) - 1::k + 1][:1], u[-1:])
+        halves = AG.gaussian_summation_backward(XA, XB, w, g, h, state, prec)
+        whole = AG.gaussian_summation_grad(XA, XB, w, g, h, prec)
+        for a, b in zip(halves, whole):
+            assert torch.equal(a, b), prec
+        part = AG.gaussian_summation_backward(XA, XB, w, g, h, None, prec,
+                                              want=(False, True, True, False))
+        assert part[0] is None and part[3] is None
+        assert torch.equal(part[1], whole[1]) and torch.equal(part[2], whole[2]), prec
+        with pytest.raises(ValueError):
+            AG.gaussian_summation_backward(XA, XB, w, g, h, None, prec)
+    XBr, wr = XB.clone().requires_grad_(True), w.clone().requires_grad_(True)
+    (AG.gaussian_summation(XA, XBr, wr, h) * g).sum().backward()
+    whole = AG.gaussian_summation_grad(XA, XB, w, g, h, "fp64")
+    assert torch.equal(XBr.grad, whole[1]) and torch.equal(wr.grad, whole[2])
+
+
 def test_multirhs_matches_single_rhs(torch_cuda, oracle):
     """Each column of the multi-RHS result equals a single-RHS summation."""
     torch = torch_cuda
