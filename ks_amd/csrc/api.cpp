@@ -75,8 +75,6 @@ struct Scratch {
   std::recursive_mutex mu;  // held by a ScratchLease
   double *partial = nullptr;
   size_t partial_elems = 0;
-  unsigned *counters = nullptr;
-  size_t counter_elems = 0;
   char *aux = nullptr;
   size_t aux_bytes = 0;
   bool captured = false;        // a stream capture has handed out these buffers
@@ -163,11 +161,10 @@ static int scratch_drop(Scratch &sc, hipStream_t s, void *p) {
   return KS_OK;
 }
 
-int ScratchLease::partials(size_t partial_elems, size_t counter_elems, double **partial,
-                           unsigned **counters) {
+int ScratchLease::partials(size_t partial_elems, double **partial) {
   Scratch &sc = *sc_;
-  const bool grow_p = sc.partial_elems < partial_elems, grow_c = sc.counter_elems < counter_elems;
-  int rc = scratch_prepare(sc, s_, grow_p || grow_c, "scratch partials");
+  const bool grow_p = sc.partial_elems < partial_elems;
+  int rc = scratch_prepare(sc, s_, grow_p, "scratch partials");
   if (rc != KS_OK) return rc;
   if (grow_p) {
     rc = scratch_drop(sc, s_, sc.partial);
@@ -182,23 +179,7 @@ int ScratchLease::partials(size_t partial_elems, size_t counter_elems, double **
     }
     sc.partial_elems = want;
   }
-  if (grow_c) {
-    rc = scratch_drop(sc, s_, sc.counters);
-    if (rc != KS_OK) return rc;
-    sc.counters = nullptr;
-    sc.counter_elems = 0;
-    size_t want = counter_elems + counter_elems / 4 + 64;
-    if (hipMalloc(&sc.counters, want * sizeof(unsigned)) != hipSuccess) {
-      (void)hipGetLastError();
-      sc.counters = nullptr;
-      return KS_ERR_ALLOC;
-    }
-    hipError_t me = hipMemset(sc.counters, 0, want * sizeof(unsigned));
-    if (me != hipSuccess) return hip_failure(me, "scratch: hipMemset");
-    sc.counter_elems = want;
-  }
   *partial = sc.partial;
-  *counters = sc.counters;
   return KS_OK;
 }
 
@@ -295,43 +276,39 @@ int make_params(const ks_t *ker, int k, KerParams &P, bool warn) {
 
 constexpr int kTargetWGs = 2048;  // ~8 workgroups (32 waves) per CU on 256 CUs
 constexpr int kMinCols = 128;     // keep each workgroup's source slice >= 128 points
-// The matrix-core kernel streams 16-source blocks with a two-stage prefetch; a
-// workgroup needs about 64 blocks per wave to amortise its set-up (A fragments,
-// norms) and its split hand-off.  Measured at m = n = 8192, k = 16: 0.205 ms
-// with 256-source slices (2048 workgroups), 0.155 ms with 1024.
+// The matrix-core kernel stages its sources through LDS 64 (k > 32: 32) at a
+// time; a workgroup needs about 16 stages to amortise its set-up (A fragments,
+// norms, the first stage's two dependent loads).  Measured at m = n = 8192,
+// k = 16: 0.1001 ms with 1024-source slices (512 workgroups), 0.1002 ms with
+// 512; before the kernel staged its sources, 0.205 ms with 256 and 0.155 ms
+// with 1024.
 constexpr int kMinColsMfma = 1024;
 // the vector kernel gathers its sources into compact tables from this many targets
 constexpr int kGatherMinRows = 1024;
 // symmetric sums keep one column partial per (row block, source): at most 4 GiB
 constexpr size_t kSymMaxColPartials = (size_t)1 << 29;
 
-size_t fixed_scratch_need(size_t partial_elems, size_t counter_elems) {
-  return ((counter_elems * sizeof(unsigned) + 255) & ~(size_t)255) + partial_elems * sizeof(double);
-}
+size_t fixed_scratch_need(size_t partial_elems) { return partial_elems * sizeof(double); }
 
-// Hands out enqueue_fused's partial sums and counters: from the stream's
-// scratch (held by a lease until the caller's launches are enqueued), or carved
-// from a caller-owned fixed buffer, whose counters are cleared first.
+// Hands out enqueue_fused's partial sums and tables: from the stream's scratch
+// (held by a lease until the caller's launches are enqueued), or from a
+// caller-owned fixed buffer.
 class FusedScratch {
  public:
-  FusedScratch(hipStream_t s, const FixedScratch *fixed) : s_(s), fixed_(fixed) {
+  FusedScratch(hipStream_t s, const FixedScratch *fixed) : fixed_(fixed) {
     if (!fixed_) lease_.emplace(s);
   }
-  bool fits(size_t partial_elems, size_t counter_elems) const {
-    return !fixed_ || fixed_scratch_need(partial_elems, counter_elems) <= fixed_->bytes;
+  bool fits(size_t partial_elems) const {
+    return !fixed_ || fixed_scratch_need(partial_elems) <= fixed_->bytes;
   }
-  int get(size_t partial_elems, size_t counter_elems, double **partial, unsigned **counters) {
-    if (lease_) return lease_->partials(partial_elems, counter_elems, partial, counters);
-    if (!fits(partial_elems, counter_elems)) return KS_ERR_INVALID_ARG;
-    *counters = reinterpret_cast<unsigned *>(fixed_->base);
-    *partial = reinterpret_cast<double *>(
-        fixed_->base + fixed_scratch_need(0, counter_elems));
-    hipError_t e = launch_zero_u32(*counters, (long)counter_elems, s_);
-    return e == hipSuccess ? KS_OK : hip_failure(e, "counter clear launch");
+  int get(size_t partial_elems, double **partial) {
+    if (lease_) return lease_->partials(partial_elems, partial);
+    if (!fits(partial_elems)) return KS_ERR_INVALID_ARG;
+    *partial = reinterpret_cast<double *>(fixed_->base);
+    return KS_OK;
   }
 
  private:
-  hipStream_t s_;
   const FixedScratch *fixed_;
   std::optional<ScratchLease> lease_;
 };
@@ -350,10 +327,9 @@ int enqueue_fused(const Problem &p, hipStream_t s, const FixedScratch *fixed) {
     const size_t tab = sym_valu ? sym_table_doubles(p) + 32 : 0;
     const size_t need = (size_t)pl.nsplit * p.m + ncol + tab;
     // a fixed buffer too small for the column partials takes the general path
-    if (ncol <= kSymMaxColPartials && scratch.fits(need, (size_t)pl.row_blocks)) {
+    if (ncol <= kSymMaxColPartials && scratch.fits(need)) {
       double *partial = nullptr;
-      unsigned *counters = nullptr;
-      int rc = scratch.get(need, (size_t)pl.row_blocks, &partial, &counters);
+      int rc = scratch.get(need, &partial);
       if (rc != KS_OK) return rc;
       double *colp = partial + (size_t)pl.nsplit * p.m;
       Problem q = p;
@@ -363,8 +339,8 @@ int enqueue_fused(const Problem &p, hipStream_t s, const FixedScratch *fixed) {
         hipError_t ge = launch_sym_gather(p, t, &q, s);
         if (ge != hipSuccess) return hip_failure(ge, "symmetric gather launch");
       }
-      hipError_t e = sym_valu ? launch_fused_f64_sym(q, pl, partial, counters, colp, s)
-                              : launch_fused_f64_sym_mfma(p, pl, partial, counters, colp, s);
+      hipError_t e = sym_valu ? launch_fused_f64_sym(q, pl, partial, colp, s)
+                              : launch_fused_f64_sym_mfma(p, pl, partial, colp, s);
       return e == hipSuccess ? KS_OK : hip_failure(e, "symmetric launch");
     }
   }
@@ -377,9 +353,8 @@ int enqueue_fused(const Problem &p, hipStream_t s, const FixedScratch *fixed) {
   const size_t npart = plan.nsplit > 1 ? (size_t)plan.nsplit * p.m : 0;
   const size_t tab = gather ? gather_table_doubles(p.n, p.k, p.type) + 32 : 0;
   double *partial = nullptr;
-  unsigned *counters = nullptr;
   if (npart + tab > 0) {
-    int rc = scratch.get(npart + tab, (size_t)plan.row_blocks, &partial, &counters);
+    int rc = scratch.get(npart + tab, &partial);
     if (rc != KS_OK) return rc;
   }
   Problem q = p;
@@ -389,8 +364,8 @@ int enqueue_fused(const Problem &p, hipStream_t s, const FixedScratch *fixed) {
     hipError_t ge = launch_source_gather(p, t, &q, s);
     if (ge != hipSuccess) return hip_failure(ge, "source gather launch");
   }
-  hipError_t e = mfma ? launch_fused_f64_mfma(p, plan, partial, counters, s)
-                      : launch_fused_f64(q, plan, partial, counters, s);
+  hipError_t e = mfma ? launch_fused_f64_mfma(p, plan, partial, s)
+                      : launch_fused_f64(q, plan, partial, s);
   if (e != hipSuccess)
     return hip_failure(e, mfma ? "fused_f64_mfma launch" : "fused_f64 launch");
   // the source splits' partial sums, added in split order

@@ -30,6 +30,11 @@ template <int TYPE, int K, int R>
 __global__ __launch_bounds__(WG) void fused_f64_kernel(FusedArgs A) {
   const Problem &p = A.p;
   const int m = p.m;
+  __shared__ double2 etab[64];
+  if constexpr (uses_exp<TYPE>) {
+    load_exp_table(etab);
+    __syncthreads();
+  }
   const int kdim = (K > 0) ? K : p.k;
   const int tid = threadIdx.x;
   const int rblk = blockIdx.x;
@@ -88,7 +93,7 @@ __global__ __launch_bounds__(WG) void fused_f64_kernel(FusedArgs A) {
 #pragma unroll
           for (int pp = 1; pp < K; ++pp) c = fma(a[q][pp], bv[pp], c);
         }
-        acc[q] = fma(kval_f64<TYPE>(c, aa[q], bb, hi[q], hj, P), wj, acc[q]);
+        acc[q] = fma(kval_f64<TYPE, uses_exp<TYPE>>(c, aa[q], bb, hi[q], hj, P, etab), wj, acc[q]);
       }
     } else {
 #pragma unroll
@@ -100,7 +105,7 @@ __global__ __launch_bounds__(WG) void fused_f64_kernel(FusedArgs A) {
           c = arow[q][0] * b[0];
           for (int pp = 1; pp < kdim; ++pp) c = fma(arow[q][pp], b[pp], c);
         }
-        acc[q] = fma(kval_f64<TYPE>(c, aa[q], bb, hi[q], hj, P), wj, acc[q]);
+        acc[q] = fma(kval_f64<TYPE, uses_exp<TYPE>>(c, aa[q], bb, hi[q], hj, P, etab), wj, acc[q]);
       }
     }
   }
@@ -188,13 +193,12 @@ FusedPlan plan_fused(int m, int n, int k, int target_wgs, int min_cols) {
 }
 
 hipError_t launch_fused_f64(const Problem &p, const FusedPlan &plan, double *partial,
-                            unsigned *counters, hipStream_t s) {
+                            hipStream_t s) {
   FusedArgs A;
   A.p = p;
   A.cols_per_split = plan.cols_per_split;
   A.nsplit = plan.nsplit;
   A.partial = partial;
-  A.counters = counters;
   A.vec2 = 0;
   switch (p.type) {
     case GAUSSIAN: return launch_type<GAUSSIAN>(A, plan, s);

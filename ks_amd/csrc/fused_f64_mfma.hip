@@ -1,25 +1,32 @@
 // FP64 fused kernel summation with the cross term on the matrix cores
 // (v_mfma_f64_16x16x4_f64), for 9 <= k <= 64.
 //
-// On MI355X the FP64 matrix and FP64 vector peaks are equal, but they are
-// separate pipes: here the matrix core forms a.b for a 16 x 16 block of pairs
-// while the VALU evaluates the kernel function of the previous block, so the
-// k-long cross term no longer competes with exp() for VALU issue slots.
-//
 // Mapping:
-//   * a wave owns RB blocks of 16 target rows; their coordinates are loaded once
-//     into the MFMA A-operand layout and stay in VGPRs;
-//   * the wave streams 16-point source blocks of its slice; each lane loads its
-//     B fragment straight from HBM/L2 through bmap (lane l reads point l%16,
-//     coordinates 4*(l/16) .. +3 of each 16-coordinate chunk -- one contiguous
-//     32-byte piece per chunk; A uses the same coordinate order, so the MFMA k
-//     order is a permutation of 0..k-1 shared by both operands);
+//   * a workgroup of 4 waves owns 4 RB blocks of 16 target rows; each wave
+//     loads the coordinates of its RB blocks once, in the MFMA A-operand
+//     layout, and keeps them in VGPRs;
+//   * the workgroup walks its source slice in stages of S sources.  Its 256
+//     lanes stage the sources' coordinates (zero-padded to 16 KC), norms,
+//     weights, bandwidths and rows into LDS once (double-buffered), and the four
+//     waves read their B fragments from there instead of each wave fetching
+//     the same fragments from L2 (lane l reads point l%16, coordinates
+//     4*(l/16) .. +3 of each 16-coordinate chunk; A uses the same coordinate
+//     order, so the MFMA k order is a permutation of 0..k-1 shared by both
+//     operands).  Map entries load two stages ahead and coordinates one stage
+//     ahead, so no stage waits on a dependent pair of round trips;
 //   * the 16 x 16 result block (lane l holds rows l/16 + 4v, v = 0..3, of
-//     column l%16 -- the layout measured on gfx950) is turned into d^2 = |a|^2 - 2c + |b|^2, K(d^2) and K*w in
-//     registers; each lane keeps 4*RB row partials across the whole slice and
-//     the 16 lanes of a row group are summed once at the end;
-//   * the split-source reduction is the one of fused_f64.hip (partials + the
-//     last workgroup of a row block adds them in split order).
+//     column l%16 -- the layout measured on gfx950) is turned into
+//     d^2 = |a|^2 - 2c + |b|^2, K(d^2) and K*w in registers; each lane keeps
+//     4*RB row partials across the whole slice, and the 16 lanes of a row
+//     group are summed once at the end;
+//   * within a stage the products of row block rb for block b + 1 are issued
+//     as soon as its kernel values for block b are formed (one accumulator per
+//     row block): the FP64 matrix and vector instructions share the SIMD's
+//     FP64 issue, so the gain is in latency hidden, not in overlap;
+//   * Gaussian kernel values take exp_tab (a table-driven exp, about 15
+//     vector instructions instead of about 22);
+//   * split sources leave partial sums that launch_split_reduce adds in split
+//     order.
 // Laplace: the reference drops pairs with d^2 < 1e-15 (frame/dgsks_ref.c:343-345),
 // so a near pair's d^2 must be the reference's, bit for bit.  The matrix-core
 // and the reference's sequential sums differ by at most
@@ -40,20 +47,35 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int MWG = 256;  // 4 waves
 
+template <int KC>
+struct Stage {
+  static constexpr int NB = KC <= 2 ? 4 : 2;  // 16-source blocks per stage
+  static constexpr int S = 16 * NB;           // sources per stage
+  static constexpr int KP = 16 * KC + 2;      // LDS row of one source (doubles, padded)
+  static constexpr int TPP = MWG / S;         // lanes per staged source
+  static constexpr int CPT = 16 * KC / TPP;   // coordinates per lane (even)
+};
+
+// k <= 16 (one chunk) fits 3 waves per SIMD in 168 registers without spills,
+// the variable-bandwidth form aside; larger chunks run at 2.
 template <int TYPE, int KC, int RB, bool EXACT>
-__global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
+__global__ __launch_bounds__(MWG, (KC == 1 && TYPE != VAR_BANDWIDTH ? 3 : 1)) void fused_f64_mfma_kernel(FusedArgs A) {
+  using St = Stage<KC>;
+  constexpr int S = St::S, KP = St::KP, NB = St::NB, CPT = St::CPT;
+  __shared__ __align__(16) double sx[2][S * KP];
+  __shared__ double sb[2][S], sw[2][S], sh[2][S];
+  __shared__ int si[2][S];
+  __shared__ double2 etab[64];
+  load_exp_table(etab);
+
   const Problem &p = A.p;
   const int m = p.m, k = p.k;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int p16 = lane & 15;
   const int g = lane >> 4;
-  const int rblk = blockIdx.x;
-  const int split = blockIdx.y;
-  const int wg_rows = 4 * RB * 16;
-  const int row0 = rblk * wg_rows + wave * (RB * 16);
+  const int row0 = blockIdx.x * (4 * RB * 16) + wave * (RB * 16);
 
-  // ---- A fragments and the norms / bandwidths of the rows this lane's results cover
   double af[RB][KC][4];
   double aa[RB][4], hi[RB][4], racc[RB][4];
 #pragma unroll
@@ -71,127 +93,182 @@ __global__ __launch_bounds__(MWG) void fused_f64_mfma_kernel(FusedArgs A) {
     }
   }
 
-  const int j0 = split * A.cols_per_split;
+  const int j0 = blockIdx.y * A.cols_per_split;
   const int j1 = min(p.n, j0 + A.cols_per_split);
+  const int nst = (j1 - j0 + S - 1) / S;
   const KerParams P = p.P;
 
-  // ---- stream 16-point source blocks; two register buffers used in turn.  A
-  // block's map entries (bmap, wmap) load two blocks ahead and the data they
-  // address one block ahead, so no block waits on two dependent round trips.
-  double bf0[KC][4], bf1[KC][4];
-  double bb0 = 0.0, wj0 = 0.0, hj0 = 0.0, bb1 = 0.0, wj1 = 0.0, hj1 = 0.0;
-  int ib0 = 0, iw0 = -1, ib1 = 0, iw1 = -1;
-  auto fetch_idx = [&](int jb, int &ib, int &iw) {
-    const int col = jb + p16;
+  // this lane's part of a stage: source `pt`, coordinates [q0, q0 + CPT)
+  const int pt = threadIdx.x / St::TPP;
+  const int q0 = (threadIdx.x % St::TPP) * CPT;
+  const bool head = q0 == 0;  // also loads the source's norm, weight, bandwidth
+  auto fetch_idx = [&](int st, int &ib, int &iw) {
+    const int col = j0 + st * S + pt;
     const bool ok = col < j1;
     ib = p.bmap[ok ? col : j1 - 1];
-    iw = ok ? p.wmap[col] : -1;
+    iw = (ok && head) ? p.wmap[col] : -1;
   };
-  auto fetch = [&](int ib, int iw, double (&f)[KC][4], double &b2, double &ww, double &hh) {
-    load_frag<KC, EXACT>(p.XB, (long)ib * k, k, g, f);
-    b2 = p.XB2[ib];
-    ww = iw >= 0 ? p.w[iw] : 0.0;
-    hh = (TYPE == VAR_BANDWIDTH) ? p.hj[ib] : 0.0;
+  double cx[CPT], cb = 0.0, cw = 0.0, ch = 0.0;
+  int cib = 0;
+  auto fetch = [&](int ib, int iw) {
+    const double *src = p.XB + (long)ib * k + q0;
+    if constexpr (EXACT) {
+      const double2 *q = reinterpret_cast<const double2 *>(src);
+#pragma unroll
+      for (int i = 0; i < CPT / 2; ++i) {
+        const double2 x = q[i];
+        cx[2 * i] = x.x;
+        cx[2 * i + 1] = x.y;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < CPT; ++i) cx[i] = (q0 + i < k) ? src[i] : 0.0;
+    }
+    cib = ib;
+    if (head) {
+      cb = p.XB2[ib];
+      cw = iw >= 0 ? p.w[iw] : 0.0;
+      ch = (TYPE == VAR_BANDWIDTH) ? p.hj[ib] : 0.0;
+    }
   };
-  // Laplace: exact-recompute threshold on d^2, as a multiple of |a|^2 + |b|^2
+  auto stash = [&](int buf) {
+    double2 *d = reinterpret_cast<double2 *>(&sx[buf][pt * KP + q0]);
+#pragma unroll
+    for (int i = 0; i < CPT / 2; ++i) d[i] = make_double2(cx[2 * i], cx[2 * i + 1]);
+    if (head) {
+      sb[buf][pt] = cb;
+      sw[buf][pt] = cw;
+      sh[buf][pt] = ch;
+      si[buf][pt] = cib;
+    }
+  };
+
   const double lap_b = (2.0 * (k + 2) + 4.0) * 0x1p-53;
   const double lap_t1 = lap_b * (1.0 + fmax(0.5 * k - 1.0, 1.0) * 1e11) + lap_b;
-  auto block = [&](const double (&f)[KC][4], double bb, double wj, double hj, int ib) {
+  auto frag = [&](int buf, int b, double (&f)[KC][4]) {
+    const double *row = &sx[buf][(b * 16 + p16) * KP + 4 * g];
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < KC; ++c) {
+      const double2 x0 = *reinterpret_cast<const double2 *>(row + 16 * c);
+      const double2 x1 = *reinterpret_cast<const double2 *>(row + 16 * c + 2);
+      f[c][0] = x0.x;
+      f[c][1] = x0.y;
+      f[c][2] = x1.x;
+      f[c][3] = x1.y;
+    }
+  };
+  auto cross = [&](const double (&f)[KC][4], int rb) {
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int c = 0; c < KC; ++c) {
+    for (int c = 0; c < KC; ++c)
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[rb][c][t], f[c][t], acc, 0, 0, 0);
+      for (int t = 0; t < 4; ++t)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[rb][c][t], f[c][t], acc, 0, 0, 0);
+    return acc;
+  };
+  auto values = [&](int buf, int b, int rb, const d4 &acc) {
+    const int col = b * 16 + p16;
+    const double bb = sb[buf][col], wj = sw[buf][col];
+    const double hj = (TYPE == VAR_BANDWIDTH) ? sh[buf][col] : 0.0;
+    if constexpr (TYPE == LAPLACE) {
+      double d2[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        d2[v] = fma(-2.0, acc[v], aa[rb][v]) + bb;  // the assembly of kval_f64
+        if (d2[v] < 1e-15 + lap_t1 * (aa[rb][v] + bb)) {
+          const int ro = row0 + rb * 16 + g + 4 * v;
+          const int ao = p.amap[ro < m ? ro : m - 1];
+          const double c = dot_exact_n(p.XA + (long)ao * k, p.XB + (long)si[buf][col] * k, k);
+          d2[v] = fma(-2.0, c, aa[rb][v]) + bb;
+        }
       }
-      if constexpr (TYPE == LAPLACE) {
-        double d2[4];
+      laplace_n_f64<4>(d2, P);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          d2[v] = fma(-2.0, acc[v], aa[rb][v]) + bb;  // the assembly of kval_f64
-          if (d2[v] < 1e-15 + lap_t1 * (aa[rb][v] + bb)) {
-            const int ro = row0 + rb * 16 + g + 4 * v;
-            const int ao = p.amap[ro < m ? ro : m - 1];
-            const double c = dot_exact_n(p.XA + (long)ao * k, p.XB + (long)ib * k, k);
-            d2[v] = fma(-2.0, c, aa[rb][v]) + bb;
-          }
-        }
-        laplace_n_f64<4>(d2, P);  // the 4 kernel values, one power ladder
+      for (int v = 0; v < 4; ++v) racc[rb][v] = fma(d2[v], wj, racc[rb][v]);
+    } else {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) racc[rb][v] = fma(d2[v], wj, racc[rb][v]);
-      } else {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          double K = kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P);
-          racc[rb][v] = fma(K, wj, racc[rb][v]);
-        }
+      for (int v = 0; v < 4; ++v) {
+        double K = kval_f64<TYPE, true>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P, etab);
+        racc[rb][v] = fma(K, wj, racc[rb][v]);
       }
     }
   };
-  int sb0 = 0, sb1 = 0;  // the source rows of the blocks held in bf0 / bf1
-  if (j0 < j1) {
-    fetch_idx(j0, ib0, iw0);
-    if (j0 + 16 < j1) fetch_idx(j0 + 16, ib1, iw1);
-    fetch(ib0, iw0, bf0, bb0, wj0, hj0);
-    sb0 = ib0;
+
+  int ib1 = 0, iw1 = -1, ib2 = 0, iw2 = -1;
+  if (nst > 0) {
+    int ib0, iw0;
+    fetch_idx(0, ib0, iw0);
+    if (nst > 1) fetch_idx(1, ib1, iw1);
+    fetch(ib0, iw0);
+    stash(0);
   }
-  for (int jb = j0; jb < j1; jb += 32) {
-    // here bf0 holds block jb and (ib1, iw1) the map entries of block jb + 16
-    if (jb + 16 < j1) {
-      fetch(ib1, iw1, bf1, bb1, wj1, hj1);
-      sb1 = ib1;
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int buf = st & 1;
+    if (st + 1 < nst) fetch(ib1, iw1);
+    if (st + 2 < nst) fetch_idx(st + 2, ib2, iw2);
+    // a half-block pipeline: row block rb's products for block b + 1 are
+    // issued as soon as its kernel values for block b are formed, so the
+    // matrix core works on one row block while the VALU works on the other,
+    // with one accumulator per row block
+    double f[KC][4], fn[KC][4];
+    d4 acc[RB];
+    frag(buf, 0, f);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) acc[rb] = cross(f, rb);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (b + 1 < NB) frag(buf, b + 1, fn);
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        values(buf, b, rb, acc[rb]);
+        if (b + 1 < NB) {
+          // empty asm statements that take the row sums and the next fragment
+          // as operands: the products for block b + 1 cannot move above the
+          // kernel values of block b (left to itself the compiler issues every
+          // product of the stage first and runs out of registers)
+          asm volatile("" : "+v"(racc[rb][0]), "+v"(racc[rb][1]), "+v"(racc[rb][2]),
+                       "+v"(racc[rb][3]));
+#pragma unroll
+          for (int c = 0; c < KC; ++c)
+            asm volatile("" : "+v"(fn[c][0]), "+v"(fn[c][1]), "+v"(fn[c][2]), "+v"(fn[c][3]));
+          acc[rb] = cross(fn, rb);
+        }
+      }
+      if (b + 1 < NB)
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) f[c][t] = fn[c][t];
     }
-    if (jb + 32 < j1) fetch_idx(jb + 32, ib0, iw0);
-    block(bf0, bb0, wj0, hj0, sb0);
-    if (jb + 16 >= j1) break;
-    if (jb + 32 < j1) {
-      fetch(ib0, iw0, bf0, bb0, wj0, hj0);
-      sb0 = ib0;
-    }
-    if (jb + 48 < j1) fetch_idx(jb + 48, ib1, iw1);
-    block(bf1, bb1, wj1, hj1, sb1);
+    if (st + 1 < nst) stash(buf ^ 1);
+    __syncthreads();
+    ib1 = ib2;
+    iw1 = iw2;
   }
 
-  // ---- sum the 16 lanes (source columns) of each row group
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double s = racc[rb][v];
-      s += __shfl_xor(s, 8, 16);
-      s += __shfl_xor(s, 4, 16);
-      s += __shfl_xor(s, 2, 16);
-      s += __shfl_xor(s, 1, 16);
-      racc[rb][v] = s;
-    }
-  // lane p16 < 4 of group g writes row g + 4 p16 of each block
-  double mine[RB];
+  // Each 16-lane group holds 4 row sums per row block, spread over its
+  // lanes.  A transposing reduction leaves lane p16 with the whole sum of
+  // row v = p16 >> 2 (halving the values held at each of the first two
+  // steps), and lane 4 v of the group writes row g + 4 v.  (A select of
+  // racc[rb][p16] instead would be a lane-indexed read, which puts racc in
+  // scratch memory.)
+  const bool h8 = p16 & 8, h4 = p16 & 4;
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) {
-    double x = racc[rb][0];
-    x = (p16 == 1) ? racc[rb][1] : x;
-    x = (p16 == 2) ? racc[rb][2] : x;
-    x = (p16 == 3) ? racc[rb][3] : x;
-    mine[rb] = x;
-  }
-  const bool writer = p16 < 4;
-  if (A.nsplit == 1) {
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
-      int r = row0 + rb * 16 + g + 4 * p16;
-      if (writer && r < m) p.u[p.umap[r]] += mine[rb];
-    }
-    return;
-  }
-  // the splits' partials are added in split order by launch_split_reduce's
-  // kernel, launched next (no device-scope fence here: on gfx950 it would
-  // write back and invalidate this XCD's L2)
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb) {
-    int r = row0 + rb * 16 + g + 4 * p16;
-    if (writer && r < m) A.partial[(long)split * m + r] = mine[rb];
+    const double s0 = (h8 ? racc[rb][2] : racc[rb][0]) +
+                      __shfl_xor(h8 ? racc[rb][0] : racc[rb][2], 8, 16);
+    const double s1 = (h8 ? racc[rb][3] : racc[rb][1]) +
+                      __shfl_xor(h8 ? racc[rb][1] : racc[rb][3], 8, 16);
+    double t = (h4 ? s1 : s0) + __shfl_xor(h4 ? s0 : s1, 4, 16);
+    t += __shfl_xor(t, 2, 16);
+    t += __shfl_xor(t, 1, 16);
+    const int r = row0 + rb * 16 + g + 4 * (p16 >> 2);
+    if ((p16 & 3) != 0 || r >= m) continue;
+    if (A.nsplit == 1)
+      p.u[p.umap[r]] += t;
+    else
+      A.partial[(long)blockIdx.y * m + r] = t;
   }
 }
 
@@ -216,7 +293,6 @@ struct SymMArgs {
   int cols_per_split;  // a multiple of the row block
   int nsplit;
   double *partial;     // [nsplit][m] row partials
-  unsigned *counters;  // [row blocks], zero between launches
   double *colp;        // [row blocks][m] column partials of the upper sources
 };
 
@@ -232,6 +308,11 @@ __global__ __launch_bounds__(SMWG) void fused_f64_sym_mfma_kernel(SymMArgs A) {
   const int first = row0 / A.cols_per_split;  // the split holding this block's diagonal
   if (split < first) return;                  // earlier sources are other blocks' upper part
   const int wrow0 = row0 + wave * (RB * 16);
+  __shared__ double2 etab[64];
+  if constexpr (uses_exp<TYPE>) {
+    load_exp_table(etab);
+    __syncthreads();
+  }
 
   // ---- A fragments; norms, bandwidths and weights of the rows of this lane's results
   double af[RB][KC][4];
@@ -287,7 +368,7 @@ __global__ __launch_bounds__(SMWG) void fused_f64_sym_mfma_kernel(SymMArgs A) {
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const double K = kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P);
+        const double K = kval_f64<TYPE, uses_exp<TYPE>>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P, etab);
         racc[rb][v] = fma(K, wj, racc[rb][v]);
         if (upper) cs = fma(K, wr[rb][v], cs);
       }
@@ -425,13 +506,12 @@ static hipError_t go_type(const FusedArgs &A, int row_blocks, hipStream_t s) {
 }
 
 hipError_t launch_fused_f64_mfma(const Problem &p, const FusedPlan &plan, double *partial,
-                                 unsigned *counters, hipStream_t s) {
+                                 hipStream_t s) {
   FusedArgs A;
   A.p = p;
   A.cols_per_split = plan.cols_per_split;
   A.nsplit = plan.nsplit;
   A.partial = partial;
-  A.counters = counters;
   A.vec2 = 0;
   switch (p.type) {
     case GAUSSIAN: return go_type<GAUSSIAN>(A, plan.row_blocks, s);
@@ -489,13 +569,12 @@ static hipError_t sym_mfma_type(const SymMArgs &A, const SymPlan &pl, hipStream_
 }
 
 hipError_t launch_fused_f64_sym_mfma(const Problem &p, const SymPlan &pl, double *partial,
-                                     unsigned *counters, double *colp, hipStream_t s) {
+                                     double *colp, hipStream_t s) {
   SymMArgs A;
   A.p = p;
   A.cols_per_split = pl.cols_per_split;
   A.nsplit = pl.nsplit;
   A.partial = partial;
-  A.counters = counters;
   A.colp = colp;
   hipError_t e;
   switch (p.type) {

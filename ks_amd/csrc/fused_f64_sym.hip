@@ -17,8 +17,8 @@
 //     the row's share of source t, and the four rows are added with two lane
 //     swaps once per 16 sources; the four waves' values meet in LDS once per
 //     64 sources, and one value per (row block, source) goes to colp[I][j];
-//   * row sums use the split-source partials and device-scope counters of
-//     fused_f64_kernel (splits before a block's diagonal hold no work for it);
+//   * row sums use split-source partials, as fused_f64_kernel does (splits
+//     before a block's diagonal hold no work for it);
 //   * sym_colsum_kernel then adds sum_{I < J} colp[I][j] (J = j's block) to
 //     u[umap[j]] in block order.  Both steps are deterministic.
 // Per pair this costs about half the evaluations of the full sum plus one
@@ -37,7 +37,6 @@ struct SymArgs {
   int cols_per_split;  // a multiple of the row block
   int nsplit;
   double *partial;     // [nsplit][m] row partials
-  unsigned *counters;  // [row blocks], zero between launches
   double *colp;        // [row blocks][m] column partials of the upper sources
 };
 
@@ -69,6 +68,11 @@ __global__ __launch_bounds__(SWG) void fused_f64_sym_kernel(SymArgs A) {
   const int row0 = rblk * B;
   const int first = row0 / A.cols_per_split;  // the split holding this block's diagonal
   if (split < first) return;                  // earlier sources are other blocks' upper part
+  __shared__ double2 etab[64];
+  if constexpr (uses_exp<TYPE>) {
+    load_exp_table(etab);
+    __syncthreads();
+  }
 
   // ---- target rows in registers, with their own weights for the column sums
   double a[R][K], aa[R], hi[R], wr[R], acc[R];
@@ -115,7 +119,7 @@ __global__ __launch_bounds__(SWG) void fused_f64_sym_kernel(SymArgs A) {
 #pragma unroll
         for (int pp = 1; pp < K; ++pp) c = fma(a[q][pp], bv[pp], c);
       }
-      kv[q] = kval_f64<TYPE>(c, aa[q], bb, hi[q], hj, P);
+      kv[q] = kval_f64<TYPE, uses_exp<TYPE>>(c, aa[q], bb, hi[q], hj, P, etab);
       acc[q] = fma(kv[q], wj, acc[q]);
     }
   };
@@ -349,13 +353,12 @@ hipError_t launch_source_gather(const Problem &p, double *tab, Problem *q, hipSt
 }
 
 hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *partial,
-                                unsigned *counters, double *colp, hipStream_t s) {
+                                double *colp, hipStream_t s) {
   SymArgs A;
   A.p = p;
   A.cols_per_split = pl.cols_per_split;
   A.nsplit = pl.nsplit;
   A.partial = partial;
-  A.counters = counters;
   A.colp = colp;
   hipError_t e;
   switch (p.type) {

@@ -213,11 +213,92 @@ __device__ __forceinline__ double tanh_f64(double x) {
   return copysign(-e / (2.0 + e), x);
 }
 
+// exp(x) by Tang's table-driven method (P. T. P. Tang, "Table-driven
+// implementation of the exponential function in IEEE floating-point
+// arithmetic", ACM TOMS 15(2), 1989) with 64 table points:
+//   n = rint(64 x / ln 2)        (the low word of x 64/ln2 + 1.5 2^52),
+//   r = x - n ln2/64             (ln2/64 in two parts, the first with 36
+//                                 significant bits, so n times it is exact),
+//   exp(x) = 2^(n >> 6) 2^((n & 63)/64) exp(r),
+// with exp(r) - 1 as its Taylor series to r^5 (|r| <= ln2/128, truncation
+// 3.4e-17 relative) and 2^(j/64) as a head and a tail.  Within 0.76 ulp of the
+// exact value over [-745.2, 709.7] (checked against 50-digit arithmetic), 0
+// below the subnormal range, inf above 709.78, exactly 1 at 0.  About 15
+// vector instructions against about 22 in the device library's exp, whose
+// kernel values the FP64 kernels spend most of their vector time on.  The
+// table is read from LDS: a kernel copies kExpTable there first.
+constexpr double kExpC = 0x1.71547652b82fep+6;    // 64 / ln 2
+constexpr double kExpL1 = 0x1.62e42fefa0000p-7;   // ln2/64, 36 significant bits
+constexpr double kExpL2 = 0x1.cf79abc9e3b3ap-46;  // ln2/64 - kExpL1
+static __constant__ double2 kExpTable[64] = {  // 2^(j/64) = head + tail, correctly rounded head
+    {0x1.0000000000000p+0, 0x0.0p+0}, {0x1.02c9a3e778061p+0, -0x1.19083535b085dp-56},
+    {0x1.059b0d3158574p+0, 0x1.d73e2a475b465p-55}, {0x1.0874518759bc8p+0, 0x1.186be4bb284ffp-57},
+    {0x1.0b5586cf9890fp+0, 0x1.8a62e4adc610bp-54}, {0x1.0e3ec32d3d1a2p+0, 0x1.03a1727c57b53p-59},
+    {0x1.11301d0125b51p+0, -0x1.6c51039449b3ap-54}, {0x1.1429aaea92de0p+0, -0x1.32fbf9af1369ep-54},
+    {0x1.172b83c7d517bp+0, -0x1.19041b9d78a76p-55}, {0x1.1a35beb6fcb75p+0, 0x1.e5b4c7b4968e4p-55},
+    {0x1.1d4873168b9aap+0, 0x1.e016e00a2643cp-54}, {0x1.2063b88628cd6p+0, 0x1.dc775814a8495p-55},
+    {0x1.2387a6e756238p+0, 0x1.9b07eb6c70573p-54}, {0x1.26b4565e27cddp+0, 0x1.2bd339940e9d9p-55},
+    {0x1.29e9df51fdee1p+0, 0x1.612e8afad1255p-55}, {0x1.2d285a6e4030bp+0, 0x1.0024754db41d5p-54},
+    {0x1.306fe0a31b715p+0, 0x1.6f46ad23182e4p-55}, {0x1.33c08b26416ffp+0, 0x1.32721843659a6p-54},
+    {0x1.371a7373aa9cbp+0, -0x1.63aeabf42eae2p-54}, {0x1.3a7db34e59ff7p+0, -0x1.5e436d661f5e3p-56},
+    {0x1.3dea64c123422p+0, 0x1.ada0911f09ebcp-55}, {0x1.4160a21f72e2ap+0, -0x1.ef3691c309278p-58},
+    {0x1.44e086061892dp+0, 0x1.89b7a04ef80d0p-59}, {0x1.486a2b5c13cd0p+0, 0x1.3c1a3b69062f0p-56},
+    {0x1.4bfdad5362a27p+0, 0x1.d4397afec42e2p-56}, {0x1.4f9b2769d2ca7p+0, -0x1.4b309d25957e3p-54},
+    {0x1.5342b569d4f82p+0, -0x1.07abe1db13cadp-55}, {0x1.56f4736b527dap+0, 0x1.9bb2c011d93adp-54},
+    {0x1.5ab07dd485429p+0, 0x1.6324c054647adp-54}, {0x1.5e76f15ad2148p+0, 0x1.ba6f93080e65ep-54},
+    {0x1.6247eb03a5585p+0, -0x1.383c17e40b497p-54}, {0x1.6623882552225p+0, -0x1.bb60987591c34p-54},
+    {0x1.6a09e667f3bcdp+0, -0x1.bdd3413b26456p-54}, {0x1.6dfb23c651a2fp+0, -0x1.bbe3a683c88abp-57},
+    {0x1.71f75e8ec5f74p+0, -0x1.16e4786887a99p-55}, {0x1.75feb564267c9p+0, -0x1.0245957316dd3p-54},
+    {0x1.7a11473eb0187p+0, -0x1.41577ee04992fp-55}, {0x1.7e2f336cf4e62p+0, 0x1.05d02ba15797ep-56},
+    {0x1.82589994cce13p+0, -0x1.d4c1dd41532d8p-54}, {0x1.868d99b4492edp+0, -0x1.fc6f89bd4f6bap-54},
+    {0x1.8ace5422aa0dbp+0, 0x1.6e9f156864b27p-54}, {0x1.8f1ae99157736p+0, 0x1.5cc13a2e3976cp-55},
+    {0x1.93737b0cdc5e5p+0, -0x1.75fc781b57ebcp-57}, {0x1.97d829fde4e50p+0, -0x1.d185b7c1b85d1p-54},
+    {0x1.9c49182a3f090p+0, 0x1.c7c46b071f2bep-56}, {0x1.a0c667b5de565p+0, -0x1.359495d1cd533p-54},
+    {0x1.a5503b23e255dp+0, -0x1.d2f6edb8d41e1p-54}, {0x1.a9e6b5579fdbfp+0, 0x1.0fac90ef7fd31p-54},
+    {0x1.ae89f995ad3adp+0, 0x1.7a1cd345dcc81p-54}, {0x1.b33a2b84f15fbp+0, -0x1.2805e3084d708p-57},
+    {0x1.b7f76f2fb5e47p+0, -0x1.5584f7e54ac3bp-56}, {0x1.bcc1e904bc1d2p+0, 0x1.23dd07a2d9e84p-55},
+    {0x1.c199bdd85529cp+0, 0x1.11065895048ddp-55}, {0x1.c67f12e57d14bp+0, 0x1.2884dff483cadp-54},
+    {0x1.cb720dcef9069p+0, 0x1.503cbd1e949dbp-56}, {0x1.d072d4a07897cp+0, -0x1.cbc3743797a9cp-54},
+    {0x1.d5818dcfba487p+0, 0x1.2ed02d75b3707p-55}, {0x1.da9e603db3285p+0, 0x1.c2300696db532p-54},
+    {0x1.dfc97337b9b5fp+0, -0x1.1a5cd4f184b5cp-54}, {0x1.e502ee78b3ff6p+0, 0x1.39e8980a9cc8fp-55},
+    {0x1.ea4afa2a490dap+0, -0x1.e9c23179c2893p-54}, {0x1.efa1bee615a27p+0, 0x1.dc7f486a4b6b0p-54},
+    {0x1.f50765b6e4540p+0, 0x1.9d3e12dd8a18bp-54}, {0x1.fa7c1819e90d8p+0, 0x1.74853f3a5931ep-55},
+};
+
+__device__ __forceinline__ double exp_tab(double x, const double2 *__restrict__ tab) {
+  x = fmin(fmax(x, -746.0), 710.0);  // n stays far inside int; exp(-746) = 0, exp(710) = inf
+  const double s = fma(x, kExpC, 0x1.8p52);
+  const int n = (int)__double2loint(s);
+  const double nf = s - 0x1.8p52;
+  double r = fma(-nf, kExpL1, x);
+  r = fma(-nf, kExpL2, r);
+  double q = fma(r, 1.0 / 120.0, 1.0 / 24.0);
+  q = fma(q, r, 1.0 / 6.0);
+  q = fma(q, r, 0.5);
+  q *= r;
+  const double p = fma(q, r, r);  // exp(r) - 1
+  const double2 t = tab[n & 63];
+  return ldexp(t.x + fma(t.x, p, t.y), n >> 6);
+}
+
+// A kernel's LDS copy of kExpTable, filled by its first 64 lanes (the caller
+// synchronises the workgroup before the first exp_tab).
+__device__ __forceinline__ void load_exp_table(double2 *tab) {
+  if (threadIdx.x < 64) tab[threadIdx.x] = kExpTable[threadIdx.x];
+}
+
+// Whether TYPE's FP64 kernel values call exp (and so take exp_tab).
+template <int TYPE>
+constexpr bool uses_exp = TYPE == GAUSSIAN || TYPE == VAR_BANDWIDTH;
+
 // Kernel value from the rank-k sum c = a.b, the norms aa = |a|^2, bb = |b|^2 and
 // (variable bandwidth) the per-point factors hi, hj.
-template <int TYPE>
+// With ETAB, the Gaussian kernels take exp_tab over `etab` (an LDS copy of
+// kExpTable).
+template <int TYPE, bool ETAB = false>
 __device__ __forceinline__ double kval_f64(double c, double aa, double bb, double hi,
-                                           double hj, const KerParams &P) {
+                                           double hj, const KerParams &P,
+                                           const double2 *etab = nullptr) {
   if constexpr (TYPE == POLYNOMIAL) {
     return powe_f64(c * P.scal + P.cons, P);
   } else if constexpr (TYPE == POLY_SQUARE_T) {
@@ -235,10 +316,14 @@ __device__ __forceinline__ double kval_f64(double c, double aa, double bb, doubl
   } else {
     double d2 = fma(-2.0, c, aa) + bb;  // |a|^2 - 2 a.b + |b|^2, reference order
     if constexpr (TYPE == GAUSSIAN) {
-      return exp(fmax(d2, 0.0) * P.scal);
+      const double x = fmax(d2, 0.0) * P.scal;
+      if constexpr (ETAB) return exp_tab(x, etab);
+      return exp(x);
     } else if constexpr (TYPE == VAR_BANDWIDTH) {
       // exp(-0.5 d2 hi hj); hi already carries the -0.5 (exact power-of-two scaling)
-      return exp((fmax(d2, 0.0) * hi) * hj);
+      const double x = (fmax(d2, 0.0) * hi) * hj;
+      if constexpr (ETAB) return exp_tab(x, etab);
+      return exp(x);
     } else if constexpr (TYPE == LAPLACE) {
       return laplace_f64(d2, P);
     } else if constexpr (TYPE == QUARTIC) {

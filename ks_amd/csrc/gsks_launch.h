@@ -13,7 +13,6 @@ struct FusedArgs {
   int cols_per_split;
   int nsplit;
   double *partial;     // [nsplit][m] partial sums (nsplit > 1)
-  unsigned *counters;  // [row_blocks], zero between launches
   int vec2;            // point rows are 16-byte aligned (k even, aligned tables)
 };
 
@@ -32,7 +31,7 @@ int fused_kspec(int k);
 FusedPlan plan_split(int m, int n, int rows_per_wg, int target_wgs, int min_cols);
 FusedPlan plan_fused(int m, int n, int k, int target_wgs, int min_cols);
 hipError_t launch_fused_f64(const Problem &p, const FusedPlan &plan, double *partial,
-                            unsigned *counters, hipStream_t s);
+                            hipStream_t s);
 
 // Symmetric sums (fused_f64_sym.hip): sources == targets, each pair of
 // different row blocks evaluated once for both of its terms.
@@ -56,9 +55,9 @@ SymPlan plan_sym_rows(int m, int target_wgs, int rows_per_wg);
 // pl.rows_per_wg targets (J = j's block)
 hipError_t launch_sym_colsum(const Problem &p, const SymPlan &pl, const double *partial,
                              const double *colp, hipStream_t s);
-// partial: [nsplit][m]; counters: unused; colp: [row_blocks][m]
+// partial: [nsplit][m]; colp: [row_blocks][m]
 hipError_t launch_fused_f64_sym(const Problem &p, const SymPlan &pl, double *partial,
-                                unsigned *counters, double *colp, hipStream_t s);
+                                double *colp, hipStream_t s);
 // Compact tables of a symmetric problem (the mapped rows, norms, weights and
 // bandwidths in map order, and an identity map) in `tab` (sym_table_doubles(p)
 // doubles, 256-byte aligned); *q is p over those tables, writing through p's umap.
@@ -69,15 +68,11 @@ hipError_t launch_sym_gather(const Problem &p, double *tab, Problem *q, hipStrea
 size_t gather_table_doubles(long count, int k, int type);
 hipError_t launch_source_gather(const Problem &p, double *tab, Problem *q, hipStream_t s);
 
-// x[0..n) = 0 on stream s with a kernel (no memset node in a captured graph).
-hipError_t launch_zero_u32(unsigned *x, long n, hipStream_t s);
-
 // u[umap[r]] += sum over sp < nsplit of partial[sp][i] (added in split order),
 // r = prow ? prow[i] : i, for every i < m: the split-source reduction of the
 // fused kernels, in a launch of its own.  The launch boundary orders the
 // partials' stores before these loads, so the kernels that write them need no
-// device-scope fences (which write back and invalidate the XCD's L2 on gfx950)
-// and no counters.
+// device-scope fences (which write back and invalidate the XCD's L2 on gfx950).
 hipError_t launch_split_reduce(const double *partial, int nsplit, int m, double *u,
                                const int *umap, const int *prow, hipStream_t s);
 // The same for nrhs right-hand sides: partial [nsplit][m][nrhs],
@@ -94,8 +89,8 @@ int mfma_rows_per_wg();
 bool sym_mfma_eligible(const Problem &p);
 int sym_mfma_rows();
 hipError_t launch_fused_f64_sym_mfma(const Problem &p, const SymPlan &pl, double *partial,
-                                     unsigned *counters, double *colp, hipStream_t s);
+                                     double *colp, hipStream_t s);
 hipError_t launch_fused_f64_mfma(const Problem &p, const FusedPlan &plan, double *partial,
-                                 unsigned *counters, hipStream_t s);
+                                 hipStream_t s);
 
 }  // namespace gsks

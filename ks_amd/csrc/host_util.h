@@ -22,8 +22,7 @@ struct Scratch;
 // on the same stream cannot grow (and free) a buffer between its hand-out and
 // the launch that reads it (the lock is recursive: nested enqueues on one
 // thread share it).
-//  * partials(): split-source partial sums (FP64) and the per-row-block
-//    counters, which start at zero and which every kernel leaves at zero.
+//  * partials(): split-source partial sums and tables (FP64).
 //  * aux(): a byte arena, separate from the partials, for front ends that
 //    re-lay out their inputs before a summation.
 // Growing synchronises the stream and frees the old buffer -- unless a stream
@@ -38,7 +37,7 @@ class ScratchLease {
   ~ScratchLease();
   ScratchLease(const ScratchLease &) = delete;
   ScratchLease &operator=(const ScratchLease &) = delete;
-  int partials(size_t partial_elems, size_t counter_elems, double **partial, unsigned **counters);
+  int partials(size_t partial_elems, double **partial);
   int aux(size_t bytes, char **base);
 
  private:
@@ -49,16 +48,15 @@ void laplace_params(int k, double *powe, double *scal);
 int make_params(const ks_t *ker, int k, KerParams &P, bool warn);
 
 // A device buffer owned by the caller of enqueue_fused, used instead of the
-// stream's grow-only scratch: the per-row-block counters at its start (cleared
-// by a kernel on the stream before every use), then the partial sums and
-// tables.  A summation whose scratch does not fit is refused with
-// KS_ERR_INVALID_ARG before anything is enqueued.
+// stream's grow-only scratch for the partial sums and tables.  A summation
+// whose scratch does not fit is refused with KS_ERR_INVALID_ARG before
+// anything is enqueued.
 struct FixedScratch {
   char *base;
   size_t bytes;
 };
-// Bytes of a FixedScratch that partial_elems doubles and counter_elems counters take.
-size_t fixed_scratch_need(size_t partial_elems, size_t counter_elems);
+// Bytes of a FixedScratch that partial_elems doubles take.
+size_t fixed_scratch_need(size_t partial_elems);
 
 // The checks of the stream-ordered entry points (sizes, NULL arrays, a GPU,
 // kernel parameters, device pointers, bandwidths), raising the error they find;

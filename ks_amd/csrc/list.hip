@@ -88,7 +88,8 @@ __device__ __forceinline__ double dot_ref_order(const double *a, cptr<double> b,
 
 // lane = row; sources [c0, c1) of task T through wave-uniform scalar loads
 template <int TYPE, int K>
-__device__ double tile_valu(const ListArgs &L, const DevTask &T, int pos, int c0, int c1) {
+__device__ double tile_valu(const ListArgs &L, const DevTask &T, int pos, int c0, int c1,
+                           const double2 *etab) {
   const int kdim = K > 0 ? K : L.k;
   const int ai = L.amap[T.a0 + (pos < T.m ? pos : T.m - 1)];
   const double *arow = L.XA + (long)ai * kdim;
@@ -132,7 +133,7 @@ __device__ double tile_valu(const ListArgs &L, const DevTask &T, int pos, int c0
       c = arow[0] * b[0];
       for (int p = 1; p < kdim; ++p) c = fma(arow[p], b[p], c);
     }
-    acc = fma(kval_f64<TYPE>(c, aa, bb, hi, hj, P), wj, acc);
+    acc = fma(kval_f64<TYPE, uses_exp<TYPE>>(c, aa, bb, hi, hj, P, etab), wj, acc);
   }
   return acc;
 }
@@ -154,7 +155,7 @@ constexpr int stage_doubles(int K) { return (stage_rows(K) * K + 63) / 64 * 64; 
 // exact: the extra terms add +0.
 template <int TYPE, int K>
 __device__ double tile_valu_lds(const ListArgs &L, const DevTask &T, int pos, int c0, int c1,
-                                double *stage) {
+                                double *stage, const double2 *etab) {
   constexpr int CH = stage_rows(K);
   constexpr int NV = stage_doubles(K) / 64;  // staged doubles per lane
   const int k = L.k;
@@ -241,7 +242,7 @@ __device__ double tile_valu_lds(const ListArgs &L, const DevTask &T, int pos, in
 #pragma unroll
         for (int p = 1; p < K; ++p) c = fma(a[p], b[p], c);
       }
-      acc = fma(kval_f64<TYPE>(c, aa, bb, hi, hj, P), wj, acc);
+      acc = fma(kval_f64<TYPE, uses_exp<TYPE>>(c, aa, bb, hi, hj, P, etab), wj, acc);
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -255,7 +256,7 @@ __device__ double tile_valu_lds(const ListArgs &L, const DevTask &T, int pos, in
 // through the wave's LDS slot `red` (64 doubles).
 template <int TYPE, int KC>
 __device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c0, int c1,
-                            double *red) {
+                            double *red, const double2 *etab) {
   constexpr int RBP = 2;
   const int k = L.k;
   const bool exact = (k == 16 * KC);  // whole chunks: two 16-byte loads per chunk
@@ -359,7 +360,7 @@ __device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c
 #pragma unroll
           for (int v = 0; v < 4; ++v)
             racc[rb][v] =
-                fma(kval_f64<TYPE>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P), wj, racc[rb][v]);
+                fma(kval_f64<TYPE, uses_exp<TYPE>>(acc[v], aa[rb][v], bb, hi[rb][v], hj, P, etab), wj, racc[rb][v]);
         }
       }
     };
@@ -403,20 +404,20 @@ __device__ double tile_mfma(const ListArgs &L, const DevTask &T, int row0, int c
 // two, 64 -> three or four); MF = 0: row tiles.
 template <int TYPE, int K, int MF>
 __device__ double run_tile(const ListArgs &L, const DevTask &T, int row0, int c0, int c1,
-                           double *red, double *stage) {
+                           double *red, double *stage, const double2 *etab) {
   if constexpr (MF >= 1) {
     if constexpr (K == 16) {
-      return tile_mfma<TYPE, 1>(L, T, row0, c0, c1, red);
+      return tile_mfma<TYPE, 1>(L, T, row0, c0, c1, red, etab);
     } else if constexpr (K == 32) {
-      return tile_mfma<TYPE, 2>(L, T, row0, c0, c1, red);
+      return tile_mfma<TYPE, 2>(L, T, row0, c0, c1, red, etab);
     } else {
-      if (L.k <= 48) return tile_mfma<TYPE, 3>(L, T, row0, c0, c1, red);
-      return tile_mfma<TYPE, 4>(L, T, row0, c0, c1, red);
+      if (L.k <= 48) return tile_mfma<TYPE, 3>(L, T, row0, c0, c1, red, etab);
+      return tile_mfma<TYPE, 4>(L, T, row0, c0, c1, red, etab);
     }
   } else if constexpr (K > 0) {
-    return tile_valu_lds<TYPE, K>(L, T, row0 + (threadIdx.x & 63), c0, c1, stage);
+    return tile_valu_lds<TYPE, K>(L, T, row0 + (threadIdx.x & 63), c0, c1, stage, etab);
   } else {
-    return tile_valu<TYPE, K>(L, T, row0 + (threadIdx.x & 63), c0, c1);
+    return tile_valu<TYPE, K>(L, T, row0 + (threadIdx.x & 63), c0, c1, etab);
   }
 }
 
@@ -438,6 +439,9 @@ __global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void lis
   constexpr int NST = (K > 0 && MF == 0) ? stage_doubles(K > 0 ? K : 1) + kStageSide : 1;
   __shared__ double s_stage[4][NST];
   double *stage = s_stage[wave];
+  __shared__ double2 etab[64];
+  load_exp_table(etab);
+  __syncthreads();
   // The loop body has no lane-dependent branch outside the tile itself and no
   // `continue`: its wave-wide values (the tile index, the split count) come
   // from lane 0 through readfirstlane, which is only right while every lane
@@ -458,35 +462,35 @@ __global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void lis
     const int c1 = min(T.n, c0 + T.bc);
     double s;
     if constexpr (MF == 2) {
-      s = run_tile<LAPLACE, K, MF>(L, T, rb.row0, c0, c1, red, stage);
+      s = run_tile<LAPLACE, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab);
     } else switch (T.type) {
-      case GAUSSIAN: s = run_tile<GAUSSIAN, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+      case GAUSSIAN: s = run_tile<GAUSSIAN, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab); break;
       case POLYNOMIAL:
         if constexpr (MF == 0) {  // row tiles: the power fixed at compile time where it can be
           switch (poly_type(T.P)) {
             case POLY_SQUARE_T:
-              s = run_tile<POLY_SQUARE_T, K, MF>(L, T, rb.row0, c0, c1, red, stage);
+              s = run_tile<POLY_SQUARE_T, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab);
               break;
             case POLY_CUBE_T:
-              s = run_tile<POLY_CUBE_T, K, MF>(L, T, rb.row0, c0, c1, red, stage);
+              s = run_tile<POLY_CUBE_T, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab);
               break;
             case POLY_FOURTH_T:
-              s = run_tile<POLY_FOURTH_T, K, MF>(L, T, rb.row0, c0, c1, red, stage);
+              s = run_tile<POLY_FOURTH_T, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab);
               break;
-            default: s = run_tile<POLYNOMIAL, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+            default: s = run_tile<POLYNOMIAL, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab); break;
           }
         } else {
-          s = run_tile<POLYNOMIAL, K, MF>(L, T, rb.row0, c0, c1, red, stage);
+          s = run_tile<POLYNOMIAL, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab);
         }
         break;
       case LAPLACE:  // row tiles only: Laplace matrix-core tiles are the MF = 2 launch
-        s = MF == 0 ? run_tile<LAPLACE, K, 0>(L, T, rb.row0, c0, c1, red, stage) : 0.0;
+        s = MF == 0 ? run_tile<LAPLACE, K, 0>(L, T, rb.row0, c0, c1, red, stage, etab) : 0.0;
         break;
-      case VAR_BANDWIDTH: s = run_tile<VAR_BANDWIDTH, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
-      case TANH: s = run_tile<TANH, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
-      case QUARTIC: s = run_tile<QUARTIC, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
-      case MULTIQUADRATIC: s = run_tile<MULTIQUADRATIC, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
-      default: s = run_tile<EPANECHNIKOV, K, MF>(L, T, rb.row0, c0, c1, red, stage); break;
+      case VAR_BANDWIDTH: s = run_tile<VAR_BANDWIDTH, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab); break;
+      case TANH: s = run_tile<TANH, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab); break;
+      case QUARTIC: s = run_tile<QUARTIC, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab); break;
+      case MULTIQUADRATIC: s = run_tile<MULTIQUADRATIC, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab); break;
+      default: s = run_tile<EPANECHNIKOV, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab); break;
     }
     // A row block of one slice writes its totals; a split one writes the
     // tile's partials, which list_rowsum_kernel adds in slice order after the

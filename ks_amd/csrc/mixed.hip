@@ -1024,13 +1024,12 @@ static hipError_t mixed_nc(const FusedArgs &A, const MixPack &pk, float xs, int 
 
 template <int PREC>
 static hipError_t mixed_launch(const Problem &p, const FusedPlan &plan, double *partial,
-                               unsigned *counters, const MixPack &pk, int nc, hipStream_t s) {
+                               const MixPack &pk, int nc, hipStream_t s) {
   FusedArgs A;
   A.p = p;
   A.cols_per_split = plan.cols_per_split;
   A.nsplit = plan.nsplit;
   A.partial = partial;
-  A.counters = counters;
   A.vec2 = 0;
   const float xs = (float)(p.P.scal * (double)kLog2e);
   const long mp = (long)plan.row_blocks * kMixRowsPerWG, np = ((long)p.n + 15) / 16 * 16;
@@ -1141,10 +1140,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   const size_t o_sort = off; off += al(sort_bytes);
   const size_t npart = (size_t)(plan.nsplit > 1 ? plan.nsplit : 0) * p.m;
   double *partial = nullptr;
-  unsigned *counters = nullptr;
   ScratchLease lease(s);
-  int rc = lease.partials(npart + (off + 255) / 8 + 32, (size_t)plan.row_blocks, &partial,
-                          &counters);
+  int rc = lease.partials(npart + (off + 255) / 8 + 32, &partial);
   if (rc != KS_OK) return rc;
   char *base = reinterpret_cast<char *>(partial + npart);
   base = (char *)(((uintptr_t)base + 255) & ~(uintptr_t)255);
@@ -1229,8 +1226,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   if (hipGetLastError() != hipSuccess) return KS_ERR_DEVICE;
 
   hipError_t e = (precision == KS_PREC_MIXED_BF16X3)
-                     ? mixed_launch<KS_PREC_MIXED_BF16X3>(p, plan, partial, counters, pk, nc, s)
-                     : mixed_launch<KS_PREC_MIXED_FP32>(p, plan, partial, counters, pk, nc, s);
+                     ? mixed_launch<KS_PREC_MIXED_BF16X3>(p, plan, partial, pk, nc, s)
+                     : mixed_launch<KS_PREC_MIXED_FP32>(p, plan, partial, pk, nc, s);
   return e == hipSuccess ? KS_OK : KS_ERR_DEVICE;
 }
 

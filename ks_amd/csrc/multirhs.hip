@@ -35,7 +35,6 @@ struct MrhsArgs {
   int nrhs;
   int cols_per_split, nsplit;
   double *partial;     // [nsplit][m][nrhs]
-  unsigned *counters;  // [row blocks]
 };
 
 constexpr int MR_WG = 256;
@@ -44,6 +43,11 @@ template <int TYPE, int K, int RMAX>
 __global__ __launch_bounds__(MR_WG) void multirhs_kernel(MrhsArgs A) {
   const Problem &p = A.p;
   const int m = p.m, nr = A.nrhs;
+  __shared__ double2 etab[64];
+  if constexpr (uses_exp<TYPE>) {
+    load_exp_table(etab);
+    __syncthreads();
+  }
   const int kdim = K > 0 ? K : p.k;
   const int tid = threadIdx.x;
   const int i = blockIdx.x * MR_WG + tid;
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(MR_WG) void multirhs_kernel(MrhsArgs A) {
       c = arow[0] * b[0];
       for (int q = 1; q < kdim; ++q) c = fma(arow[q], b[q], c);
     }
-    const double Kv = kval_f64<TYPE>(c, aa, bb, hi, hj, P);
+    const double Kv = kval_f64<TYPE, uses_exp<TYPE>>(c, aa, bb, hi, hj, P, etab);
     cptr<double> wr = W + (long)wmap[j] * nr;
 #pragma unroll
     for (int r = 0; r < RMAX; ++r)
@@ -140,6 +144,11 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
   const int p16 = lane & 15, g = lane >> 4;
   const int rblk = blockIdx.x, split = blockIdx.y;
   const int row0 = rblk * MRM_ROWS + wave * (RB * 16);
+  __shared__ double2 etab[64];
+  if constexpr (uses_exp<TYPE>) {
+    load_exp_table(etab);
+    __syncthreads();
+  }
 
   // targets (B operand): lane (p16, g) -> target row0 + 16 rb + p16
   double tf[RB][KC][4], aa[RB], hi[RB];
@@ -217,7 +226,7 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
           c = __builtin_amdgcn_mfma_f64_16x16x4f64(b.f[cc][t], tf[rb][cc][t], c, 0, 0, 0);
       double Kv[4];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) Kv[v] = kval_f64<TYPE>(c[v], aa[rb], b.bb[v], hi[rb], b.hj[v], P);
+      for (int v = 0; v < 4; ++v) Kv[v] = kval_f64<TYPE, uses_exp<TYPE>>(c[v], aa[rb], b.bb[v], hi[rb], b.hj[v], P, etab);
 #pragma unroll
       for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -352,11 +361,9 @@ int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s) {
   FusedPlan plan = mfma ? plan_split(p.m, p.n, MRM_ROWS, 2048, 1024)
                         : plan_split(p.m, p.n, MR_WG, 2048, 128);
   double *partial = nullptr;
-  unsigned *counters = nullptr;
   ScratchLease lease(s);
   if (plan.nsplit > 1) {
-    int rc = lease.partials((size_t)plan.nsplit * p.m * nrhs, (size_t)plan.row_blocks, &partial,
-                            &counters);
+    int rc = lease.partials((size_t)plan.nsplit * p.m * nrhs, &partial);
     if (rc != KS_OK) return rc;
   }
   MrhsArgs A;
@@ -365,7 +372,6 @@ int enqueue_multirhs(const Problem &p, int nrhs, hipStream_t s) {
   A.cols_per_split = plan.cols_per_split;
   A.nsplit = plan.nsplit;
   A.partial = partial;
-  A.counters = counters;
   hipError_t e;
   if (mfma) {
     switch (p.type) {

@@ -78,7 +78,6 @@ struct MxArgs {
   int nrhs;
   int cols_per_split, nsplit;  // cols_per_split is a multiple of 32
   double *partial;             // [nsplit][m][nrhs]
-  unsigned *counters;          // [row blocks]
   const f4 *T;                 // targets, packed (FP32 centred)
   const float *Tf;             // [m_pad] fraction f_i of xs |a32|^2
   const double *Ts;            // [m_pad] 2^n_i
@@ -563,7 +562,6 @@ int enqueue_multirhs_mixed(const Problem &p, int nrhs, const double *centre, hip
   A.cols_per_split = cps;
   A.nsplit = nsplit;
   A.partial = nullptr;
-  A.counters = nullptr;
   A.T = T;
   A.Tf = Tf;
   A.Ts = Ts;
@@ -579,8 +577,7 @@ int enqueue_multirhs_mixed(const Problem &p, int nrhs, const double *centre, hip
   A.c2 = (float)(as * 2.01 * kU32);
   A.c0 = (float)(4.0 * kU32);
   if (nsplit > 1) {
-    rc = lease.partials((size_t)nsplit * m * nrhs, (size_t)plan.row_blocks, &A.partial,
-                        &A.counters);
+    rc = lease.partials((size_t)nsplit * m * nrhs, &A.partial);
     if (rc != KS_OK) return rc;
   }
   switch (KC) {

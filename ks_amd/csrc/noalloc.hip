@@ -1,5 +1,5 @@
 // dgsks_device_noalloc(): the FP64 summation of dgsks_device with its scratch
-// (split partial sums, per-row-block counters, compact source tables) in a
+// (split partial sums, column partials, compact source tables) in a
 // static device array instead of memory the library allocates, for callers
 // that cannot allocate.  No counterpart in the reference.
 #include <hip/hip_runtime.h>

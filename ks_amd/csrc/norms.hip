@@ -18,17 +18,6 @@ __global__ void sqnorm_kernel(const double *X, long nx, int k, double *X2) {
   X2[i] = t;
 }
 
-__global__ void zero_u32_kernel(unsigned *x, long n) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] = 0u;
-}
-
-hipError_t launch_zero_u32(unsigned *x, long n, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n);
-  return hipGetLastError();
-}
-
 __global__ void split_reduce_kernel(const double *__restrict__ partial, int nsplit, int m,
                                     double *u, const int *__restrict__ umap,
                                     const int *__restrict__ prow) {

@@ -145,7 +145,7 @@ def test_distinct_tables_and_bandwidths(G, oracle, torch_cuda):
 
 @pytest.mark.parametrize("k", [3, 16])
 def test_gaussian_exp_ulps(G, torch_cuda, k):
-    """The FP64 kernels' exp (the device library's) against numpy's (correctly rounded in practice)
+    """The FP64 kernels' exp (exp_tab, a table-driven exp of their own) against numpy's (correctly rounded in practice)
     over [-745.5, 0] and near its edges: one source at the origin with weight 1
     and targets whose norms are x, so u_i = exp(-x_i) exactly (d^2 = x_i; the
     coordinates are zero).  k = 3 runs the vector kernel, k = 16 the matrix-core
