@@ -8,7 +8,9 @@ import os
 import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgsks.so")
+# KS_LIBGSKS names another build of the same library (the host-sanitized one
+# that `python -m ks_amd.build --sanitize` writes), for the host-layer checks.
+LIB_PATH = os.environ.get("KS_LIBGSKS") or os.path.join(HERE, "libgsks.so")
 
 # include/ks.h enums
 KS_GAUSSIAN, KS_POLYNOMIAL, KS_LAPLACE, KS_GAUSSIAN_VAR_BANDWIDTH, KS_TANH, KS_QUARTIC, \

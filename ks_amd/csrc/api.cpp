@@ -245,7 +245,7 @@ void laplace_params(int k, double *powe, double *scal) {
 
 int make_params(const ks_t *ker, int k, KerParams &P, bool warn) {
   P = KerParams{ker->scal, ker->cons, ker->powe, 0, POW_GENERAL};
-  switch ((int)ker->type) {
+  switch (ker_type(ker)) {
     case KS_GAUSSIAN:
     case KS_TANH:
     case KS_QUARTIC:
@@ -393,7 +393,7 @@ int device_problem(const char *name, const ks_t *kernel, int m, int n, int k, do
   p.m = m;
   p.n = n;
   p.k = k;
-  p.type = (int)kernel->type;
+  p.type = ker_type(kernel);
   p.u = u;
   p.umap = umap;
   p.XA = XA;
@@ -504,13 +504,13 @@ int sgsks_device(const ks_t *kernel, int m, int n, int k, float *u, const int *u
   SingleProblem p;
   rc = make_params(kernel, k, p.P, true);
   if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return rc;
-  rc = single_precision_ok("sgsks_device", (int)kernel->type, k, precision);
+  rc = single_precision_ok("sgsks_device", ker_type(kernel), k, precision);
   if (rc != KS_OK) return rc;
-  const bool varbw = kernel->type == KS_GAUSSIAN_VAR_BANDWIDTH;
+  const bool varbw = ker_type(kernel) == KS_GAUSSIAN_VAR_BANDWIDTH;
   p.m = m;
   p.n = n;
   p.k = k;
-  p.type = (int)kernel->type;
+  p.type = ker_type(kernel);
   p.u = u;
   p.umap = umap;
   p.XA = XA;
@@ -548,7 +548,7 @@ void sgsks(ks_t *kernel, int m, int n, int k, float *u, int *umap, float *XA, fl
   KerParams P;
   int rc = make_params(kernel, k, P, true);
   if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return;
-  if (kernel->type == KS_LAPLACE) {
+  if (ker_type(kernel) == KS_LAPLACE) {
     kernel->powe = P.powe;
     kernel->scal = P.scal;
   }
@@ -560,7 +560,7 @@ void sgsks_wrapper(int m, int n, int k, float *u, int *umap, float *XA, float *X
                    double cons, double powe, double *h) {
   ks_t kernel;
   memset(&kernel, 0, sizeof(kernel));
-  kernel.type = (ks_type)type;
+  set_ker_type(&kernel, type);
   kernel.scal = scal;
   kernel.cons = cons;
   kernel.powe = powe;
@@ -584,7 +584,7 @@ void dgsks(ks_t *kernel, int m, int n, int k, double *u, int *umap, double *XA, 
   KerParams P;
   int rc = make_params(kernel, k, P, true);
   if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return;
-  if (kernel->type == KS_LAPLACE) {  // the reference's side effect, frame/dgsks.c:445-447
+  if (ker_type(kernel) == KS_LAPLACE) {  // the reference's side effect, frame/dgsks.c:445-447
     kernel->powe = P.powe;
     kernel->scal = P.scal;
   }
@@ -600,7 +600,7 @@ void dgsks_ref(ks_t *kernel, int m, int n, int k, double *u, int *umap, double *
   KerParams P;
   int rc = make_params(kernel, k, P, true);
   if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return;
-  if (kernel->type == KS_LAPLACE) {  // frame/dgsks_ref.c:146-148
+  if (ker_type(kernel) == KS_LAPLACE) {  // frame/dgsks_ref.c:146-148
     kernel->powe = P.powe;
     kernel->scal = P.scal;
   }
@@ -615,7 +615,7 @@ void dgsks_wrapper(int m, int n, int k, double *u, int *umap, double *XA, double
                    int type, double scal, double cons, double powe, double *h) {
   ks_t kernel;
   memset(&kernel, 0, sizeof(kernel));
-  kernel.type = (ks_type)type;
+  set_ker_type(&kernel, type);
   kernel.scal = scal;
   kernel.cons = cons;
   kernel.powe = powe;
@@ -629,7 +629,7 @@ void dgsks_ref_wrapper(int m, int n, int k, double *u, int *umap, double *XA, do
                        int type, double scal, double cons, double powe, double *h) {
   ks_t kernel;
   memset(&kernel, 0, sizeof(kernel));
-  kernel.type = (ks_type)type;
+  set_ker_type(&kernel, type);
   kernel.scal = scal;
   kernel.cons = cons;
   kernel.powe = powe;

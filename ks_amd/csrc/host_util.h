@@ -3,10 +3,24 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "../../include/ks.h"
 #include "gsks_kernels.h"
 
 namespace gsks {
+
+// The kernel type of a descriptor as the int the caller stored.  ks_type has
+// no fixed underlying type in C++, so loading a value outside 0..7 through the
+// enum is undefined behaviour; callers may store any int there (an illegal
+// type is an error reported at run time, as the reference reports it).
+static_assert(sizeof(ks_type) == sizeof(int), "ks_type is stored as an int");
+inline int ker_type(const ks_t *k) {
+  int t;
+  std::memcpy(&t, &k->type, sizeof(t));
+  return t;
+}
+inline void set_ker_type(ks_t *k, int t) { std::memcpy(&k->type, &t, sizeof(t)); }
 
 const char *err_string(int code);
 int raise_error(int code, bool fatal, const char *fmt, ...);

@@ -47,7 +47,7 @@ int list_run(int ntask, const ks_t *kernels, int k, double *u, long nu, const lo
     KerParams P;
     int rc = make_params(&kernels[t], k, P, true);
     if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return rc;
-    tasks[t].type = (int)kernels[t].type;
+    tasks[t].type = ker_type(&kernels[t]);
     tasks[t].P = P;
     tasks[t].a0 = a_off[t];
     tasks[t].m = (int)(a_off[t + 1] - a_off[t]);
@@ -212,7 +212,7 @@ extern "C" int dgsks_list(int ntask, ks_t *kernels, int k, double *u, long nu,
                     XB, XB2, 0, w, precision, "dgsks_list");
   if (rc == KS_OK)
     for (int t = 0; t < ntask; ++t)  // the Laplace descriptor side effect, per task
-      if (kernels[t].type == KS_LAPLACE) laplace_params(k, &kernels[t].powe, &kernels[t].scal);
+      if (ker_type(&kernels[t]) == KS_LAPLACE) laplace_params(k, &kernels[t].powe, &kernels[t].scal);
   return rc;
 }
 
@@ -249,7 +249,7 @@ extern "C" int ks_list_plan_create(ks_list_plan **plan, int ntask, const ks_t *k
     KerParams P;
     int rc = make_params(&kernels[t], k, P, true);
     if (rc != KS_OK && rc != KS_ERR_LAPLACE_K) return rc;
-    tasks[t].type = (int)kernels[t].type;
+    tasks[t].type = ker_type(&kernels[t]);
     tasks[t].P = P;
     tasks[t].a0 = a_off[t];
     tasks[t].m = (int)(a_off[t + 1] - a_off[t]);
@@ -257,7 +257,7 @@ extern "C" int ks_list_plan_create(ks_list_plan **plan, int ntask, const ks_t *k
     tasks[t].n = (int)(b_off[t + 1] - b_off[t]);
     tasks[t].hi = kernels[t].hi;
     tasks[t].hj = kernels[t].hj;
-    if (kernels[t].type == KS_GAUSSIAN_VAR_BANDWIDTH &&
+    if (ker_type(&kernels[t]) == KS_GAUSSIAN_VAR_BANDWIDTH &&
         !(is_device_ptr(kernels[t].hi) && is_device_ptr(kernels[t].hj)))
       return raise_error(KS_ERR_INVALID_ARG, false,
                          "%s(): the bandwidth vectors of a plan must be device arrays\n", name);

@@ -5,12 +5,15 @@
 // 16 x 16 blocks on the FP64 matrix cores.  Which shape is fastest depends on
 // the kernel type (the cost of one evaluation) and on k (the cost of the cross
 // term and of the loads), so the shape is chosen per (kernel type, k,
-// precision) by timing every candidate on a synthetic batch of that type --
+// precision) on each GPU architecture by timing every candidate on a synthetic
+// batch of that type --
 // 256 tasks with m, n log-uniform in [64, 8192] over a pool of 262144 points
 // uniform in [0,1)^k, i.e. enough tiles to fill the GPU several times and a pool
 // larger than L2 -- and keeping the fastest.  Results are cached for the
 // process and, when KS_TUNE_CACHE names a file, in that file (one line
-// "type k precision slice tile" per key), which later processes read first.
+// "arch type k precision slice tile" per key, arch being the device's gfx
+// name, e.g. gfx950), which later processes read first; lines of another
+// architecture, or in another format, are ignored.
 // KS_AUTOTUNE=0 switches tuning off (the default shape is then used).
 #include <hip/hip_runtime.h>
 
@@ -21,6 +24,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -35,7 +39,7 @@ TileChoice list_default_tile(int type, int k) {
 
 namespace {
 
-using Key = std::tuple<int, int, int>;
+using Key = std::tuple<std::string, int, int, int>;
 std::mutex g_mu;
 std::map<Key, TileChoice> g_tuned;
 bool g_loaded = false;
@@ -50,6 +54,29 @@ bool autotune_enabled() {
   return !(e && strcmp(e, "0") == 0);
 }
 
+// The current device's architecture without its feature suffix ("gfx950"
+// from "gfx950:sramecc+:xnack-"); "unknown" if the runtime cannot say.
+// Looked up once per device.
+std::string device_arch() {
+  static std::mutex mu;
+  static std::map<int, std::string> by_device;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return "unknown";
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = by_device.find(dev);
+  if (it != by_device.end()) return it->second;
+  hipDeviceProp_t prop;
+  std::string a = "unknown";
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+    a = prop.gcnArchName;
+    const size_t colon = a.find(':');
+    if (colon != std::string::npos) a.resize(colon);
+    if (a.empty()) a = "unknown";
+  }
+  by_device[dev] = a;
+  return a;
+}
+
 void load_cache_locked() {
   if (g_loaded) return;
   g_loaded = true;
@@ -57,9 +84,14 @@ void load_cache_locked() {
   if (!path) return;
   FILE *f = fopen(path, "r");
   if (!f) return;
+  char line[256], arch[64];
   int type, k, prec, slice, tile;
-  while (fscanf(f, "%d %d %d %d %d", &type, &k, &prec, &slice, &tile) == 5)
-    if (slice > 0 && (tile == 0 || tile == 1)) g_tuned[Key(type, k, prec)] = TileChoice{slice, tile};
+  while (fgets(line, sizeof(line), f)) {
+    if (sscanf(line, "%63s %d %d %d %d %d", arch, &type, &k, &prec, &slice, &tile) != 6) continue;
+    if (strncmp(arch, "gfx", 3) != 0) continue;
+    if (slice > 0 && (tile == 0 || tile == 1))
+      g_tuned[Key(std::string(arch), type, k, prec)] = TileChoice{slice, tile};
+  }
   fclose(f);
 }
 
@@ -68,8 +100,8 @@ void save_entry(const Key &key, const TileChoice &tc) {
   if (!path) return;
   FILE *f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%d %d %d %d %d\n", std::get<0>(key), std::get<1>(key), std::get<2>(key), tc.slice,
-          tc.tile);
+  fprintf(f, "%s %d %d %d %d %d\n", std::get<0>(key).c_str(), std::get<1>(key), std::get<2>(key),
+          std::get<3>(key), tc.slice, tc.tile);
   fclose(f);
 }
 
@@ -119,7 +151,7 @@ int make_synthetic(int type, int k, Synthetic &S) {
   std::vector<int> amap, bmap;
   ks_t kd;
   std::memset(&kd, 0, sizeof(kd));
-  kd.type = (ks_type)type;
+  set_ker_type(&kd, type);
   kd.scal = (type == KS_POLYNOMIAL || type == KS_TANH) ? 1.0 / k : -12.5;
   kd.cons = (type == KS_TANH) ? 0.1 : 1.0;
   kd.powe = 2.0;
@@ -214,7 +246,7 @@ TileChoice tune(int type, int k) {
 }  // namespace
 
 TileChoice list_tile_choice(int type, int k, int precision) {
-  const Key key(type, k, precision);
+  const Key key(device_arch(), type, k, precision);
   {
     std::lock_guard<std::mutex> lk(g_mu);
     load_cache_locked();

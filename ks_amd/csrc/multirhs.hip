@@ -500,7 +500,7 @@ extern "C" int dgsks_multirhs_device(const ks_t *kernel, int m, int n, int k, in
     if (!a || !device_ptr_cached(a))
       return raise_error(KS_ERR_MIXED_POINTERS, false,
                          "dgsks_multirhs_device(): every array must be device memory\n");
-  p.m = m; p.n = n; p.k = k; p.type = (int)kernel->type;
+  p.m = m; p.n = n; p.k = k; p.type = ker_type(kernel);
   p.u = U; p.umap = umap; p.XA = XA; p.XA2 = XA2; p.amap = amap;
   p.XB = XB; p.XB2 = XB2; p.bmap = bmap; p.w = W; p.wmap = wmap;
   p.hi = kernel->hi; p.hj = kernel->hj;

@@ -633,7 +633,7 @@ extern "C" int dgsks_multirhs_mixed_device(const ks_t *kernel, int m, int n, int
                                            const double *XB2, const int *bmap, const double *W,
                                            const int *wmap, void *stream) {
   if (!kernel || m < 0 || n < 0 || k < 0 || nrhs < 1 || nrhs > kMixedMrhsMaxRhs ||
-      k > kMixedMrhsMaxK || kernel->type != KS_GAUSSIAN)
+      k > kMixedMrhsMaxK || ker_type(kernel) != KS_GAUSSIAN)
     return raise_error(KS_ERR_INVALID_ARG, false,
                        "dgsks_multirhs_mixed_device(): Gaussian kernel, k <= %d and nrhs <= %d "
                        "expected\n", kMixedMrhsMaxK, kMixedMrhsMaxRhs);

@@ -55,7 +55,7 @@ void run(ks_t *kernel, int k, std::vector<double> &u, VV &ulist, double *XA, dou
   int rc = gsks::list_run(ntask, kernels.data(), k, u.data(), (long)u.size(), aoff.data(),
                           umap.data(), amap.data(), boff.data(), bmap.data(), wmap.data(), XA,
                           XA2, nxa, XB, XB2, nxb, w, KS_PREC_FP64, "omp_dgsks_list");
-  if (rc == KS_OK && kernel->type == KS_LAPLACE)
+  if (rc == KS_OK && gsks::ker_type(kernel) == KS_LAPLACE)
     gsks::laplace_params(k, &kernel->powe, &kernel->scal);
 }
 

@@ -75,8 +75,8 @@ void run_staged(const ks_t *kernel, int m, int n, int k, double *u, const int *u
   p.m = m;
   p.n = n;
   p.k = k;
-  p.type = (int)kernel->type;
-  const bool varbw = kernel->type == KS_GAUSSIAN_VAR_BANDWIDTH;
+  p.type = ker_type(kernel);
+  const bool varbw = ker_type(kernel) == KS_GAUSSIAN_VAR_BANDWIDTH;
   hipStream_t s = thread_stream();
   if (!s) {
     finish(KS_ERR_DEVICE, name);
@@ -231,8 +231,8 @@ void run_staged_single(const ks_t *kernel, int m, int n, int k, float *u, const 
   p.m = m;
   p.n = n;
   p.k = k;
-  p.type = (int)kernel->type;
-  const bool varbw = kernel->type == KS_GAUSSIAN_VAR_BANDWIDTH;
+  p.type = ker_type(kernel);
+  const bool varbw = ker_type(kernel) == KS_GAUSSIAN_VAR_BANDWIDTH;
   const int precision = single_default_precision(p.type, m, n, k);
   hipStream_t s = thread_stream();
   if (!s) {

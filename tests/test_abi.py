@@ -171,3 +171,60 @@ def test_unsupported_type_prints_and_returns(L, capsys):
     u = np.zeros(4)
     gsks.dgsks(2, u, idx, X, X2, idx, X, X2, idx, np.ones(4), idx, type="Cauchy")
     assert "not suppurted" in capsys.readouterr().out
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks host paths that stop before the GPU")
+def test_host_argument_checks_before_the_device(L):
+    """Argument and list checks that return before any device work: codes in
+    KS_ERRMODE_RETURN mode, and the Laplace descriptor side effect of dgsks
+    (frame/dgsks.c:445-447), which happens before the summation."""
+    from ks_amd import _lib
+    from ks_amd._lib import KsT
+    L.ks_set_error_mode(_lib.KS_ERRMODE_RETURN)
+    ptr = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    X = np.random.default_rng(1).random((8, 3))
+    X2 = (X * X).sum(1)
+    idx = np.arange(8, dtype=np.int32)
+    u, w = np.zeros(8), np.ones(8)
+    # Laplace k = 3: powe = -1/2, scal = 1/(4 pi) are written into the descriptor
+    kt = KsT()
+    kt.type = _lib.KS_LAPLACE
+    L.dgsks(ctypes.byref(kt), 8, 8, 3, ptr(u), ptr(idx), ptr(X), ptr(X2), ptr(idx), ptr(X),
+            ptr(X2), ptr(idx), ptr(w), ptr(idx))
+    assert kt.powe == -0.5 and abs(kt.scal - 1.0 / (4.0 * np.pi)) <= 1e-17
+    assert L.ks_last_error() == _lib.KS_ERR_DEVICE
+    # an illegal type and a variable bandwidth without bandwidths
+    args = [ptr(u), ptr(idx), ptr(X), ptr(X2), ptr(idx), ptr(X), ptr(X2), ptr(idx), ptr(w),
+            ptr(idx)]
+    L.dgsks_wrapper(8, 8, 3, *args, 11, -0.5, 0.0, 0.0, None)
+    assert L.ks_last_error() == _lib.KS_ERR_INVALID_TYPE
+    L.dgsks_wrapper(8, 8, 3, *args, _lib.KS_GAUSSIAN_VAR_BANDWIDTH, 0.0, 0.0, 0.0, None)
+    assert L.ks_last_error() == _lib.KS_ERR_NO_BANDWIDTH
+    assert np.all(u == 0)
+    # batched entry points: inconsistent offsets, a_off[0] != 0, bad counts
+    kts = (KsT * 2)()
+    off_bad = np.array([0, 5, 3], dtype=np.int64)
+    off_one = np.array([1, 4, 8], dtype=np.int64)
+    off = np.array([0, 4, 8], dtype=np.int64)
+    lst = [ptr(idx), ptr(idx)]
+
+    def dl(ntask, a_off, b_off):
+        return L.dgsks_list(ntask, kts, 3, ptr(u), 8, ptr(a_off), *lst, ptr(b_off), *lst,
+                            ptr(X), ptr(X2), ptr(X), ptr(X2), ptr(w), 0)
+    assert dl(2, off_bad, off) == _lib.KS_ERR_LIST
+    assert dl(2, off, off_bad) == _lib.KS_ERR_LIST
+    assert dl(2, off_one, off) == _lib.KS_ERR_LIST
+    assert dl(-1, off, off) == _lib.KS_ERR_INVALID_ARG
+    assert dl(0, off, off) == _lib.KS_OK
+    h = ctypes.c_void_p()
+    assert L.ks_list_plan_create(ctypes.byref(h), 2, kts, 3, ptr(off), *lst, ptr(off), *lst,
+                                 7) == _lib.KS_ERR_INVALID_ARG
+    assert L.ks_list_plan_create(ctypes.byref(h), 2, kts, 3, ptr(off_bad), *lst, ptr(off), *lst,
+                                 0) == _lib.KS_ERR_LIST
+    assert not h.value
+    assert L.ks_list_plan_execute(None, *([None] * 7)) == _lib.KS_ERR_INVALID_ARG
+    L.ks_list_plan_destroy(None)
+    a, b = ctypes.c_int(), ctypes.c_int()
+    assert L.ks_list_autotune(9, 3, 0, ctypes.byref(a), ctypes.byref(b)) == \
+        _lib.KS_ERR_INVALID_ARG
+    assert np.all(u == 0)

@@ -363,9 +363,10 @@ np.savez({str(out)!r}, u0=u0, **outs)
 
 
 def test_autotune_cache_file(tmp_path, torch_cuda):
-    """KS_TUNE_CACHE: a fresh process tunes and appends its choice; a second
-    process reads it back without tuning (KS_AUTOTUNE=0 would otherwise give the
-    default shape)."""
+    """KS_TUNE_CACHE: a fresh process tunes and appends its choice under the
+    device's architecture; a second process reads it back without tuning
+    (KS_AUTOTUNE=0 would otherwise give the default shape).  Lines of another
+    architecture or of the older five-field format are ignored."""
     cache = tmp_path / "tiles.txt"
     code = ("import ctypes,sys; sys.path.insert(0, %r); from ks_amd import _lib; L=_lib.lib(); "
             "a,b=ctypes.c_int(),ctypes.c_int(); "
@@ -376,12 +377,21 @@ def test_autotune_cache_file(tmp_path, torch_cuda):
                         timeout=300)
     assert r1.returncode == 0, r1.stderr
     lines = cache.read_text().split("\n")
-    assert lines[0].split()[:3] == ["4", "12", "0"]
+    assert lines[0].split()[:4] == ["gfx950", "4", "12", "0"]
     env2 = dict(env, KS_AUTOTUNE="0")
     r2 = subprocess.run([sys.executable, "-c", code], env=env2, capture_output=True, text=True,
                         timeout=300)
     assert r2.returncode == 0, r2.stderr
     assert r2.stdout.split() == r1.stdout.split()
+    # a cache of another architecture and of the five-field format gives the
+    # default shape (slice 1024), not the 256 those lines name
+    foreign = tmp_path / "foreign.txt"
+    foreign.write_text("gfx942 4 12 0 256 0\n4 12 0 256 0\n")
+    env3 = dict(os.environ, KS_TUNE_CACHE=str(foreign), KS_AUTOTUNE="0")
+    r3 = subprocess.run([sys.executable, "-c", code], env=env3, capture_output=True, text=True,
+                        timeout=300)
+    assert r3.returncode == 0, r3.stderr
+    assert r3.stdout.split()[0] == "1024"
 
 
 def test_mixed_batch_routing(L, torch_cuda, oracle):
