@@ -14,8 +14,10 @@
 //      rigorous margin for what the split and FP32 accumulation can change
 //      (2^-7 |a||b| from the BF16 roundings of both operands plus the FP32
 //      sums, using the largest |a| of the wave's rows and each source's |b|),
-//      a block whose every exponent is below kSkipT2 (K < 2^-60 for every
-//      pair) is skipped: a target loses at most n 2^-60 max|w| in total;
+//      a block whose every exponent is below kSkipT2 (K < 2^-50 for every
+//      pair) is skipped: a target loses at most n 2^-50 max|w| in total --
+//      below the far tier's own error bound for a single pair (about 1e-4 of
+//      K < 1.1e-7);
 //   3. a 16 x 16 block whose estimated exponents all lie below -kFarT
 //      (K < ~1e-7 for every pair) is "far": its kernel values are taken in FP32
 //      (v_exp_f32) from the estimate, whose error is bounded by the estimate's
@@ -59,7 +61,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr float kFarT = 16.0f;          // far: estimated x < -16 (natural-log units)
-constexpr float kSkipT2 = -60.0f;       // skipped: every exponent < -60 (log2 units)
+constexpr float kSkipT2 = -50.0f;       // skipped: every exponent < -50 (log2 units)
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr int XWG = 256;                // 4 waves
 constexpr int kPivots = 128;            // bucketing pivots (at most)
@@ -93,7 +95,7 @@ struct MixPack {
   // Which source blocks each wave's row group needs (bucketed BF16x3 Gaussian
   // with a decaying kernel; null: every block): word (g nsplit + split) ngw + gi
   // has bit b set when block 64 gi + b of slice `split` may hold a pair with
-  // K >= 2^-60 for row group g (32 sorted targets), mix_need_kernel.
+  // K >= 2^-50 for row group g (32 sorted targets), mix_need_kernel.
   const unsigned long long *need;
   int ngw;          // mask words per (row group, slice)
   uint4 *Ap;        // BF16 / FP32 target operand [mp/16][nc][2 parts][64 lanes]
@@ -101,27 +103,40 @@ struct MixPack {
   uint4 *Bp;        // BF16 / FP32 source operand [np/16][nc][2 parts][64 lanes]
   double4 *Bd;      // FP64 source operand [np/16][2 nc][64 lanes]
   float2 *ra;       // per target: {-|a|^2 / 2, -hi log2(e) / 2 (VarBW)}
-  float4 *cb;       // per source: {xs |b|^2 (Gaussian) or |b|^2, hj, w, |b| (Gaussian)}
+  float4 *cb;       // per source: {xs |b|^2, |b - p_own|, w, |b|} (Gaussian) or {|b|^2, hj, w, |b|}
   float2 *cs;       // per source, the skip test's two factors: {xs |b|^2, |b|} (Gaussian)
   double2 *rd;      // per target: {|a|^2, -hi / 2}
   double4 *cd;      // per source: {|b|^2, w, hj, 0}
   const int *prow;  // sorted target position -> target index of the call
   const int *pcol;  // sorted source position -> source index of the call
+  // The centred near tier (bucketed BF16x3 Gaussian with pruning bounds; Ex is
+  // null otherwise): the MixTabs tables and operands, and the bucket range of
+  // every 16-point target / source block (first | last << 16).
+  const float *Ex, *Qx;
+  const float4 *Ac, *Bc;
+  const float *na, *nb;
+  const unsigned *rt, *rs;
+  long mp, np;
 };
 
 // FP32 copies of the pivots: pivot i is target (i * m) / P, zero past k.  With
-// pivB, also in the B-operand order of v_mfma_f32_16x16x4f32 for
+// pivA, also as FP64 in the A-operand order of v_mfma_f64_16x16x4f64 for
 // mix_bounds_kernel: tile t (pivots 16 t ..), step st (coordinates 4 st ..),
-// lane 16 g + c holding coordinate 4 st + g of pivot 16 t + c, so that one
-// fragment load is 256 contiguous bytes.
-__global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv, float *pivB) {
+// lane 16 g + c holding coordinate 4 st + g of pivot 16 t + f64_a_row(c) -- the
+// row order that lands pivot 16 t + 4 g + v in accumulator slot v of lane group
+// g -- so that one fragment load is 512 contiguous bytes.  f64_a_row is its own
+// inverse (a 4 x 4 transpose of the row index).
+__global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv, double *pivA) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P * kp) return;
   const int i = t / kp, q = t - i * kp;
   const int r = (int)(((long)i * p.m) / P);
   const float v = q < p.k ? (float)p.XA[(long)p.amap[r] * p.k + q] : 0.0f;
   piv[t] = v;
-  if (pivB) pivB[((i >> 4) * (kp >> 2) + (q >> 2)) * 64 + (q & 3) * 16 + (i & 15)] = v;
+  if (pivA) {
+    const int c = 4 * ((i & 15) & 3) + ((i & 15) >> 2);
+    pivA[((long)(i >> 4) * (kp >> 2) + (q >> 2)) * 64 + (q & 3) * 16 + c] = (double)v;
+  }
 }
 
 // Bucket (nearest pivot) of every target, then every source: the pivot with the
@@ -180,8 +195,10 @@ __global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p, int P,
 // proven negligible before any of their operands is loaded.
 
 // pp[i][j] = |p_i - p_j|^2 in FP64 from the FP32 pivots, rounded up to FP32,
-// and pn[i] = |p_i|^2 (nearest FP32).  Workgroup i, thread j.
-__global__ void mix_pivot_gram_kernel(int P, int kp, const float *piv, float *pp, float *pn) {
+// and pn[i] = |p_i|^2 (nearest FP32); with ppd, also both in FP64 (ppd[i][j],
+// pn64[i]) for the centred near tier's tables.  Workgroup i, thread j.
+__global__ void mix_pivot_gram_kernel(int P, int kp, const float *piv, float *pp, float *pn,
+                                      double *ppd, double *pn64) {
   const int i = blockIdx.x, j = threadIdx.x;
   if (j >= P) return;
   double t = 0.0;
@@ -189,6 +206,7 @@ __global__ void mix_pivot_gram_kernel(int P, int kp, const float *piv, float *pp
     const double d = (double)piv[i * kp + q] - (double)piv[j * kp + q];  // exact
     t += d * d;
   }
+  if (ppd) ppd[i * P + j] = t;
   t *= 1.0 + 0x1p-40;  // covers the FP64 roundings of the sum
   float f = (float)t;
   if ((double)f < t) f = __uint_as_float(__float_as_uint(f) + 1u);  // next float up (f >= 0)
@@ -197,28 +215,56 @@ __global__ void mix_pivot_gram_kernel(int P, int kp, const float *piv, float *pp
     double n2 = 0.0;
     for (int q = 0; q < kp; ++q) n2 += (double)piv[i * kp + q] * (double)piv[i * kp + q];
     pn[i] = (float)n2;
+    if (pn64) pn64[i] = n2;
   }
 }
 
+// Outputs of mix_bounds_kernel.  Vt, Vs, rt, rs: the pruning bounds above.
+// With Ex non-null, also the operands of the mixed kernel's centred near tier:
+//   Ex[i][t] = xs e_i(a_t)                         (pivot i, sorted target t)
+//   Qx[i][j] = xs (e_i(b_j) - |p_i - p_own(j)|^2)  (pivot i, sorted source j)
+//            with e_i(x) = N(x) - 2 x.p_i + |p_i|^2 and N the caller's squared
+//            norm (XA2 / XB2), i.e. |x - p_i|^2 when the norms are exact;
+//   Ac, Bc   the coordinates centred on the point's own pivot, x - p_own, in
+//            FP32 and in the lane order of the v_mfma_f32_16x16x4f32 operands:
+//            block b, chunk c (4 steps of 4 coordinates), lane 16 g + p holding
+//            {x[16c + g], x[16c + 4 + g], x[16c + 8 + g], x[16c + 12 + g]} of
+//            point 16 b + p (targets and sources alike);
+//   na, nb   |x - p_own| rounded up, per sorted target / source.
+// xs = scal log2(e) (FP64); the tables are FP64 values rounded once to FP32.
+// For target t in bucket A and source j in bucket B, with a = p_A + alpha and
+// b = p_B + beta, the reference's d^2 = N(a) + N(b) - 2 a.b is
+//   e_B(a) + (N(b) - 2 p_A.beta - |p_B|^2) - 2 alpha.beta
+// and N(b) - 2 p_A.beta - |p_B|^2 = e_A(b) - |p_A - p_B|^2, so the pair's
+// exponent (log2 units) is Ex[B][t] + Qx[A][j] - 2 xs alpha.beta, where only
+// the last term, a product of two small centred vectors, is left to the kernel.
+struct MixTabs {
+  float *Vt, *Vs;
+  unsigned *rt, *rs;
+  float *Ex, *Qx;
+  float4 *Ac, *Bc;
+  float *na, *nb;
+  long mp, np;  // row strides of Ex (padded targets) and Qx (padded sources)
+};
+
 // One wave per 16-point block of the sorted targets (blocks [0, ntb)) or
-// sources ([ntb, ntb + nsb)): the 128 dot products x . p_i of its points on the
-// FP32 matrix cores (v_mfma_f32_16x16x4f32, eight 16-pivot tiles from the packed
-// pivB), then
+// sources ([ntb, ntb + nsb)): the dot products p_i . x of its 16 points with the
+// 128 pivots on the FP64 matrix cores (v_mfma_f64_16x16x4f64, pivots as the A
+// operand from pivA, the points as B; eight 16-pivot tiles), then
 //   V[blk][i] = min_x (pn_i - pn_own - 2 (x.p_i - x.p_own)) - mu(x),
-// the minimum of e_i(x) - e_own(x) less a margin mu(x) that covers every FP32
-// rounding (the coordinates, the dot products, the norms and the combination):
-//   mu(x) = (4 kp + 32) 2^-24 (|x| + max |p_i|)^2.
+// the minimum of e_i(x) - e_own(x) less a margin mu(x) far above the FP64
+// roundings of the dot products and norms:
+//   mu(x) = 2^-36 (|x| + max |p_i|)^2.
 // Targets go to Vt[T][i], sources to Vs[i][S].  Points past m / n do not count
-// (+inf); rng[blk] = first | last << 16 bucket of the block's points.
+// (+inf); rng[blk] = first | last << 16 bucket of the block's points.  With
+// o.Ex, also the centred tier's tables (MixTabs), zero for points past m / n.
 template <int KP>
-__global__ __launch_bounds__(256) void mix_bounds_kernel(const Problem p, int ntb, int nsb,
-                                                         const float *__restrict__ pivB,
-                                                         const float *__restrict__ pn,
-                                                         const unsigned *__restrict__ keys_a,
-                                                         const unsigned *__restrict__ keys_b,
-                                                         const int *__restrict__ prow,
-                                                         const int *__restrict__ pcol, float *Vt,
-                                                         float *Vs, unsigned *rt, unsigned *rs) {
+__global__ __launch_bounds__(256) void mix_bounds_kernel(
+    const Problem p, int ntb, int nsb, const double *__restrict__ pivA,
+    const float *__restrict__ piv, const float *__restrict__ pn, const double *__restrict__ pn64,
+    const double *__restrict__ ppd, const unsigned *__restrict__ keys_a,
+    const unsigned *__restrict__ keys_b, const int *__restrict__ prow,
+    const int *__restrict__ pcol, double xs, MixTabs o) {
   constexpr int NS = KP / 4;  // MFMA steps of 4 coordinates
   constexpr int NT = kPivots / 16;
   const int lane = threadIdx.x & 63;
@@ -229,83 +275,119 @@ __global__ __launch_bounds__(256) void mix_bounds_kernel(const Problem p, int nt
   const int cnt = tgt ? p.m : p.n;
   const unsigned *keys = tgt ? keys_a : keys_b;
   const int p16 = lane & 15, g = lane >> 4;
-  // the A operand: row p16 supplies coordinates 4 st + g
+  // the B operand: point p16 supplies coordinates 4 st + g
   const int pos = b0 * 16 + p16;
   const bool valid = pos < cnt;
   const double *x = nullptr;
-  if (valid) x = tgt ? p.XA + (long)p.amap[prow[pos]] * p.k : p.XB + (long)p.bmap[pcol[pos]] * p.k;
-  float a[NS];
-  float n2 = 0.0f;
+  double nrm = 0.0;  // the caller's squared norm of the point (XA2 / XB2)
+  if (valid) {
+    const int xi = tgt ? p.amap[prow[pos]] : p.bmap[pcol[pos]];
+    x = (tgt ? p.XA : p.XB) + (long)xi * p.k;
+    nrm = (tgt ? p.XA2 : p.XB2)[xi];
+  }
+  const int own = valid ? (int)keys[pos] : 0;
+  double a[NS];
+  double n2 = 0.0;
 #pragma unroll
   for (int st = 0; st < NS; ++st) {
     const int q = 4 * st + g;
-    a[st] = (valid && q < p.k) ? (float)x[q] : 0.0f;
-    n2 = fmaf(a[st], a[st], n2);
+    a[st] = (valid && q < p.k) ? x[q] : 0.0;
+    n2 = fma(a[st], a[st], n2);
   }
   n2 += __shfl_xor(n2, 16);
-  n2 += __shfl_xor(n2, 32);  // |x|^2 of row p16 on every lane of that row
-  float pnt[NT], pmx = 0.0f;
+  n2 += __shfl_xor(n2, 32);  // |x|^2 of point p16 on its four lanes
+  d4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    pnt[t] = pn[16 * t + p16];
-    pmx = fmaxf(pmx, pnt[t]);
-  }
-#pragma unroll
-  for (int o = 1; o < 16; o <<= 1) pmx = fmaxf(pmx, __shfl_xor(pmx, o));
-  const float pm = sqrtf(pmx) * 1.0001f;  // >= max |p_i| (pn rounded to nearest)
-  f4 acc[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    const float *pv = pivB + (long)t * NS * 64 + lane;
+    acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    const double *pv = pivA + (long)t * NS * 64 + lane;
 #pragma unroll
     for (int st = 0; st < NS; ++st)
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], pv[st * 64], acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv[st * 64], a[st], acc[t], 0, 0, 0);
   }
-  // acc[t][v]: row 4 g + v, pivot 16 t + p16
-  constexpr float kMu = (4.0f * KP + 32.0f) * 0x1p-24f * 1.01f;
-  float best[NT];
+  // acc[t][v]: pivot 16 t + 4 g + v, point p16.  x.p_own: picked out of the slot
+  // that holds it, then summed over the point's four lanes.
+  double dsel = 0.0;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) best[t] = INFINITY;
+  for (int t = 0; t < NT; ++t)
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int r = 4 * g + v;
-    const int rpos = b0 * 16 + r;
-    const bool rv = rpos < cnt;
-    const int own = rv ? (int)keys[rpos] : 0;
-    // x . p_own: picked out of the tile that holds it, then summed over the row's 16 lanes
-    float dsel = 0.0f;
+    for (int v = 0; v < 4; ++v) dsel = (16 * t + 4 * g + v == own) ? acc[t][v] : dsel;
+  dsel += __shfl_xor(dsel, 16);
+  dsel += __shfl_xor(dsel, 32);
+  float pmx = fmaxf(pn[lane], pn[64 + lane]);
 #pragma unroll
-    for (int t = 0; t < NT; ++t) dsel = (16 * t + p16 == own) ? acc[t][v] : dsel;
-    dsel += __shfl_xor(dsel, 1);
-    dsel += __shfl_xor(dsel, 2);
-    dsel += __shfl_xor(dsel, 4);
-    dsel += __shfl_xor(dsel, 8);
-    const float nx = sqrtf(__shfl(n2, r)) * 1.0001f;
-    const float mu = kMu * (nx + pm) * (nx + pm);
-    const float pno = pn[own];
+  for (int o2 = 1; o2 < 64; o2 <<= 1) pmx = fmaxf(pmx, __shfl_xor(pmx, o2));
+  const float pm = sqrtf(pmx) * 1.0001f;  // >= max |p_i| (pn rounded to nearest)
+  const float nx = sqrtf((float)n2) * 1.0001f;
+  const float mu = 0x1p-36f * (nx + pm) * (nx + pm);
+  const double pno = pn64[own];
+  const bool tabs = o.Ex != nullptr;
+  float *etab = tgt ? o.Ex : o.Qx;
+  const long estride = tgt ? o.mp : o.np;
+  float best[NT][4];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const float val = (pnt[t] - pno) - 2.0f * (acc[t][v] - dsel) - mu;
-      best[t] = rv ? fminf(best[t], val) : best[t];
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = 16 * t + 4 * g + v;
+      const double pni = pn64[i];
+      const double diff = (pni - pno) - 2.0 * (acc[t][v] - dsel);  // e_i(x) - e_own(x)
+      best[t][v] = valid ? (float)diff - mu : INFINITY;
+      if (tabs) {
+        // e_i(x) = |x - p_i|^2 with the caller's norm in place of |x|^2: the
+        // tables then carry the reference's d^2 = XA2 + XB2 - 2 a.b exactly
+        // (norms that are not the points' own, e.g. rounded ones, included)
+        double e = nrm + pni - 2.0 * acc[t][v];
+        if (!tgt) e -= ppd[i * kPivots + own];
+        etab[(long)i * estride + pos] = valid ? (float)(xs * e) : 0.0f;
+      }
     }
-  }
+  // block minima over the 16 points
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    best[t] = fminf(best[t], __shfl_xor(best[t], 16));
-    best[t] = fminf(best[t], __shfl_xor(best[t], 32));
-  }
-  if (g == 0) {
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      float b = best[t][v];
+      b = fminf(b, __shfl_xor(b, 1));
+      b = fminf(b, __shfl_xor(b, 2));
+      b = fminf(b, __shfl_xor(b, 4));
+      b = fminf(b, __shfl_xor(b, 8));
+      best[t][v] = b;
+    }
+  if (p16 == 0) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      if (tgt) Vt[(long)b0 * kPivots + 16 * t + p16] = best[t];
-      else Vs[(long)(16 * t + p16) * nsb + b0] = best[t];
+      if (tgt) {
+        *reinterpret_cast<float4 *>(&o.Vt[(long)b0 * kPivots + 16 * t + 4 * g]) =
+            make_float4(best[t][0], best[t][1], best[t][2], best[t][3]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) o.Vs[(long)(16 * t + 4 * g + v) * nsb + b0] = best[t][v];
+      }
     }
   }
   if (lane == 0) {
     const int last = min(cnt - b0 * 16, 16) - 1;
     const unsigned f = last >= 0 ? keys[b0 * 16] : 0u, l = last >= 0 ? keys[b0 * 16 + last] : 0u;
-    (tgt ? rt : rs)[b0] = f | (l << 16);
+    (tgt ? o.rt : o.rs)[b0] = f | (l << 16);
+  }
+  if (tabs) {
+    // the centred coordinates x - p_own (exact FP64 differences, rounded once)
+    double c2 = 0.0;
+    float cf[NS];
+#pragma unroll
+    for (int st = 0; st < NS; ++st) {
+      const double d = valid ? a[st] - (double)piv[own * KP + 4 * st + g] : 0.0;
+      c2 = fma(d, d, c2);
+      cf[st] = (float)d;
+    }
+    c2 += __shfl_xor(c2, 16);
+    c2 += __shfl_xor(c2, 32);
+    float4 *dst = (tgt ? o.Ac : o.Bc) + (long)b0 * (NS / 4) * 64 + lane;
+#pragma unroll
+    for (int c = 0; c < NS / 4; ++c)
+      dst[c * 64] = make_float4(cf[4 * c], cf[4 * c + 1], cf[4 * c + 2], cf[4 * c + 3]);
+    if (g == 0) (tgt ? o.na : o.nb)[pos] = (float)sqrt(c2) * (1.0f + 0x1p-20f);
   }
 }
 
@@ -511,7 +593,9 @@ __global__ void mixed_pack_kernel(const Problem p, int nc, float xs, long mp, lo
       const double w = p.w[p.wmap[jj]];
       const double hj = (TYPE == VAR_BANDWIDTH) ? p.hj[bj] : 0.0;
       const float nb = (float)sqrt(bb);
-      if (TYPE == GAUSSIAN) pk.cb[j] = make_float4((float)(bb * (double)xs), 0.0f, (float)w, nb);
+      // Gaussian: {xs |b|^2, |b - p_own| (centred near tier), w, |b|}
+      if (TYPE == GAUSSIAN)
+        pk.cb[j] = make_float4((float)(bb * (double)xs), pk.nb ? pk.nb[j] : 0.0f, (float)w, nb);
       else pk.cb[j] = make_float4((float)bb, (float)hj, (float)w, nb);
       pk.cs[j] = make_float2((float)(bb * (double)xs), nb);
       pk.cd[j] = make_double4(bb, w, hj, 0.0);
@@ -618,16 +702,31 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
   const int wg_rows = 4 * WROWS;
   const int row0 = rblk * wg_rows + wave * WROWS;  // sorted target position
 
-  // the wave's targets in FP64 (the A operand of near blocks), row stride
-  // 32 NC + 2 doubles so the 16 rows of one FP64 fragment load fall on
-  // different banks; zero past k and for padding rows
+  // The skip-tier kernel (BF16x3 Gaussian) keeps the wave's targets centred
+  // on their pivots in FP32 (the A operand of its centred near tier, in the
+  // MixTabs lane order); the other kernels keep them in FP64 (the A operand of
+  // their near blocks), row stride 32 NC + 2 doubles so the 16 rows of one
+  // FP64 fragment load fall on different banks; zero past k and for padding
+  // rows.
+  constexpr bool SK = kSkipTier<PREC, TYPE>;
   constexpr int KS = 32 * NC + 2;
-  __shared__ __align__(16) double s_rows[4][WROWS * KS];
+  constexpr int NC4 = 2 * NC;  // float4 chunks of a centred operand fragment
+  __shared__ __align__(16) double s_rows[4][SK ? 2 : WROWS * KS];
+  __shared__ __align__(16) float4 s_ac[4][SK ? RB * NC4 * 64 : 1];
   __shared__ double s_tot[4][WROWS];
   // FP64 row factors {|a|^2, -hi/2}, read by near blocks only
   __shared__ double2 s_rd[4][WROWS];
   double *arows = s_rows[wave];
-  {
+  if constexpr (SK) {
+    if (pk.Ex) {
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int c = 0; c < NC4; ++c)
+          s_ac[wave][(rb * NC4 + c) * 64 + lane] =
+              pk.Ac[((long)((row0 >> 4) + rb) * NC4 + c) * 64 + lane];
+    }
+  } else {
     // the wave's rows are contiguous in the sorted FP64 copy: every load is
     // independent and issued before the first store (a gather through prow
     // and amap made each iteration wait for three dependent round trips)
@@ -711,6 +810,92 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
     f.block = jb;
   };
 
+  // ---- the centred near tier (skip-tier kernel with pk.Ex)
+  // A near block pair (T in bucket A, S in bucket B, one bucket each) takes its
+  // exponent as Ex[B][t] + Qx[A][j] - 2 xs alpha.beta (MixTabs): the cross term
+  // of the centred coordinates on the FP32 matrix cores, 16 steps of
+  // v_mfma_f32_16x16x4f32 at k = 64 in two chains, and one v_exp_f32 per pair.
+  // The exponent's error (log2 units) is at most
+  //   u (|Ex| + |Qx| + |Ex + Qx| + |2 xs c| + |x2|) + (kp + 4) u |2 xs| A' |beta|
+  // with u = 2^-24, c the computed cross term and A' the largest |alpha| of the
+  // wave's rows: the five roundings of the FP32 evaluation, and the cross term's
+  // (coordinates rounded to FP32, sums of kp / 2 products per chain).  A block
+  // where any pair's bound exceeds 2^-15 -- or whose target or source block
+  // spans buckets -- is recomputed exactly in FP64 instead, so every kernel
+  // value of the tier is within ln2 2^-15 + 2^-22 (v_exp_f32) relative.
+  struct C32Frag {
+    float4 b[NC4];
+    float q[RB];  // Qx[A][j] of the lane's source for each row block's bucket A
+    int block;    // the source block it holds (or has in flight), -1 for none
+    int bucket;   // its single bucket, -1 if it spans several
+  };
+  int abk[RB];      // each row block's single bucket, -1 if it spans several
+  float aA = 0.0f;  // largest |alpha| of the wave's rows
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) abk[rb] = -1;
+  auto fetch32 = [&](C32Frag &f, int jb) {
+    const float4 *src = pk.Bc + (long)(jb >> 4) * NC4 * 64 + lane;
+#pragma unroll
+    for (int c = 0; c < NC4; ++c) f.b[c] = src[c * 64];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+      f.q[rb] = abk[rb] >= 0 ? pk.Qx[(long)abk[rb] * pk.np + jb + p16] : 0.0f;
+    const unsigned r = pk.rs[jb >> 4];
+    f.bucket = (r & 0xFFFFu) == (r >> 16) ? (int)(r & 0xFFFFu) : -1;
+    f.block = jb;
+  };
+
+  if constexpr (SK) {
+    if (pk.Ex) {
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        const unsigned r = pk.rt[(row0 >> 4) + rb];
+        abk[rb] = (r & 0xFFFFu) == (r >> 16) ? (int)(r & 0xFFFFu) : -1;
+      }
+      aA = pk.na[row0 + (lane & (WROWS - 1))];
+#pragma unroll
+      for (int o = 1; o < WROWS; o <<= 1) aA = fmaxf(aA, __shfl_xor(aA, o));
+    }
+  }
+  const float kCross = (32.0f * NC + 4.0f) * fabsf(m2xs) * aA;  // times |beta|
+  // Row block rb against the source block of cf: false (nothing added) when
+  // the bound sends the block to FP64.
+  auto near32 = [&](int rb, const C32Frag &cf, const float4 &cb) -> bool {
+    f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
+    const float4 *sa = &s_ac[wave][rb * NC4 * 64 + lane];
+#pragma unroll
+    for (int c = 0; c < NC4; c += 2) {
+      const float4 a0 = sa[c * 64], b0 = cf.b[c];
+      c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.x, b0.x, c0, 0, 0, 0);
+      c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.y, b0.y, c0, 0, 0, 0);
+      c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.z, b0.z, c0, 0, 0, 0);
+      c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.w, b0.w, c0, 0, 0, 0);
+      const float4 a1 = sa[(c + 1) * 64], b1 = cf.b[c + 1];
+      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.x, b1.x, c1, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.y, b1.y, c1, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.z, b1.z, c1, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.w, b1.w, c1, 0, 0, 0);
+    }
+    const float4 e = *reinterpret_cast<const float4 *>(pk.Ex + (long)cf.bucket * pk.mp + row0 +
+                                                        rb * 16 + 4 * g);
+    const float q = cf.q[rb];
+    const float ev[4] = {e.x, e.y, e.z, e.w};
+    float kv[4], err = 0.0f;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float cv = c0[v] + c1[v];
+      const float sm = ev[v] + q;
+      const float x2 = fmaf(m2xs, cv, sm);
+      err = fmaxf(err, fabsf(ev[v]) + fabsf(q) + fabsf(sm) + fabsf(m2xs * cv) + fabsf(x2));
+      kv[v] = __builtin_amdgcn_exp2f(fminf(x2, 0.0f));  // d^2 clamped at 0
+    }
+    const float bnd = 0x1p-24f * 1.01f * (err + kCross * cb.y);
+    if (__ballot(bnd > 0x1p-15f) != 0ull) return false;  // wave-uniform
+#pragma unroll
+    for (int v = 0; v < 4; ++v) facc[rb][v] = fmaf(kv[v], cb.z, facc[rb][v]);
+    return true;
+  };
+
   // source blocks start on multiples of 16 and a slice ends on one or at n, so
   // a block's columns past j1 are padding of the packed arrays.  The skip tier
   // loads the hi halves ahead of time and the lo halves (fetch_lo) only for
@@ -735,11 +920,60 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
     lo_block = jb;
   };
 
-  // One 16-source block against the wave's RB row blocks.
-  int nblk = 0;
+  int nblk = 0;  // source blocks since the last flush of the FP32 partials
+  // The skip-tier kernel's FP64 fallback (blocks the centred tier hands back):
+  // one 16-coordinate chunk of the FP64 source fragment in registers at a time.
+  auto near64_lean = [&](int rb, int jb) {
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    const double *ar = pk.Ad + (long)(row0 + rb * 16 + f64_a_row(p16)) * (32 * NC) + 4 * g;
+    const double4 *src = pk.Bd + (long)(jb >> 4) * KC * 64 + lane;
+#pragma unroll 1
+    for (int cc = 0; cc < KC; ++cc) {
+      const double4 bq = src[cc * 64];
+      const double2 a01 = *reinterpret_cast<const double2 *>(ar + cc * 16);
+      const double2 a23 = *reinterpret_cast<const double2 *>(ar + cc * 16 + 2);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.x, bq.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.y, bq.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.x, bq.z, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.y, bq.w, acc, 0, 0, 0);
+    }
+    const double4 cdv = pk.cd[jb + p16];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double2 rd = s_rd[wave][rb * 16 + out_row(g, v)];
+      const double K = kval_f64<TYPE>(acc[v], rd.x, cdv.x, rd.y, cdv.z, P);
+      racc[rb][v] = fma(K, cdv.y, racc[rb][v]);
+    }
+  };
+  // Near block, exactly: FP64 cross terms on the matrix cores, FP64 kernel
+  // values (the skip-tier kernel reads its FP64 rows from the sorted copy).
+  auto near64 = [&](int rb, const F64Frag &f64) {
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    const double *ar = SK ? pk.Ad + (long)(row0 + rb * 16 + f64_a_row(p16)) * (32 * NC) + 4 * g
+                          : arows + (rb * 16 + f64_a_row(p16)) * KS + 4 * g;
+#pragma unroll
+    for (int cc = 0; cc < KC; ++cc) {
+      const double2 a01 = *reinterpret_cast<const double2 *>(ar + cc * 16);
+      const double2 a23 = *reinterpret_cast<const double2 *>(ar + cc * 16 + 2);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.x, f64.bd[cc][0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.y, f64.bd[cc][1], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.x, f64.bd[cc][2], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.y, f64.bd[cc][3], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double2 rd = s_rd[wave][rb * 16 + out_row(g, v)];
+      const double K = kval_f64<TYPE>(acc[v], rd.x, f64.cdv.x, rd.y, f64.cdv.z, P);
+      racc[rb][v] = fma(K, f64.cdv.y, racc[rb][v]);
+    }
+  };
   // Far / near evaluation of row block rb against source block jb, given its
-  // full FP32 cross-term estimate c (= a.b - |a|^2/2).
-  auto evaluate = [&](int rb, int jb, const f4 &c, const float4 &cb, F64Frag &f64, bool &used64) {
+  // full FP32 cross-term estimate c (= a.b - |a|^2/2).  The skip-tier kernel
+  // keeps the centred fragment of a block across its row blocks (c32) and
+  // reads FP64 only where the centred tier hands a block back; the others keep
+  // the FP64 fragment across row blocks and source blocks (f64).
+  auto evaluate = [&](int rb, int jb, const f4 &c, const float4 &cb, F64Frag &f64, C32Frag &c32,
+                      bool &used64) {
     float x2[4];
     bool nr = false;
 #pragma unroll
@@ -755,27 +989,17 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
       // far block: FP32 kernel values from the estimate
 #pragma unroll
       for (int v = 0; v < 4; ++v) facc[rb][v] = fmaf(__builtin_amdgcn_exp2f(x2[v]), cb.z, facc[rb][v]);
+    } else if constexpr (SK) {
+      // near block: the centred FP32 tier where its bound holds, else FP64
+      if (pk.Ex && abk[rb] >= 0) {
+        if (c32.block != jb) fetch32(c32, jb);
+        if (c32.bucket >= 0 && near32(rb, c32, cb)) return;
+      }
+      near64_lean(rb, jb);
     } else {
-      // near block: exact FP64 cross terms on the matrix cores, FP64 kernel values
       if (f64.block != jb) fetch64(f64, jb);
       used64 = true;
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
-      const double *ar = arows + (rb * 16 + f64_a_row(p16)) * KS + 4 * g;
-#pragma unroll
-      for (int cc = 0; cc < KC; ++cc) {
-        const double2 a01 = *reinterpret_cast<const double2 *>(ar + cc * 16);
-        const double2 a23 = *reinterpret_cast<const double2 *>(ar + cc * 16 + 2);
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.x, f64.bd[cc][0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.y, f64.bd[cc][1], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.x, f64.bd[cc][2], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.y, f64.bd[cc][3], acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const double2 rd = s_rd[wave][rb * 16 + out_row(g, v)];
-        const double K = kval_f64<TYPE>(acc[v], rd.x, f64.cdv.x, rd.y, f64.cdv.z, P);
-        racc[rb][v] = fma(K, f64.cdv.y, racc[rb][v]);
-      }
+      near64(rb, f64);
     }
   };
   // FP32 far partials go to the FP64 row sums every kFlush source blocks
@@ -797,7 +1021,7 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
   if constexpr (kSkipTier<PREC, TYPE>) {
     // ---- which source blocks of the slice can matter (pruning bounds)
     // Bit b of mask word gi marks source block 64 gi + b of the slice as one
-    // that may hold a pair with K >= 2^-60 for the wave's rows
+    // that may hold a pair with K >= 2^-50 for the wave's rows
     // (mix_need_kernel); every other block is proven negligible and never
     // loaded.  Lane gi holds word gi; without masks every block is kept.
     const int nbs = j0 < j1 ? (j1 - j0 + 15) >> 4 : 0;  // source blocks of the slice
@@ -874,8 +1098,10 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
     // products complete the split-BF16 cross term, then far / near
     auto finish = [&](int jb, int jnext, const HB &b, const f4 *e, const bool *h) {
       bool used64 = false, usedlo = false;
-      F64Frag f64;  // live within this block only: loaded by its first near row block
+      F64Frag f64;  // unused in the skip-tier kernel (its FP64 fallback loads its own)
       f64.block = -1;
+      C32Frag c32;  // live within this block only: loaded by its first near row block
+      c32.block = -1;
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb) {
         if (!h[rb]) continue;  // wave-uniform
@@ -891,7 +1117,7 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
           c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
           c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
         }
-        evaluate(rb, jb, c, bcb, f64, used64);
+        evaluate(rb, jb, c, bcb, f64, c32, used64);
       }
       // blocks that pass come in runs once the points are bucketed: after one,
       // start the next needed block's lo loads
@@ -946,12 +1172,14 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
   } else {
     F64Frag f64p;  // persists across blocks: prefetched after a near block
     f64p.block = -1;
+    C32Frag c32u;  // unused: the centred tier belongs to the skip-tier kernel
+    c32u.block = -1;
     // One source block against the wave's RB row blocks.
     auto block = [&](int jb, int jnext, const Frag<PREC, NC> &bf, const float4 &cb) {
       bool used64 = false;
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb)
-        evaluate(rb, jb, cross_block<PREC, NC>(af[rb], bf, cinit[rb]), cb, f64p, used64);
+        evaluate(rb, jb, cross_block<PREC, NC>(af[rb], bf, cinit[rb]), cb, f64p, c32u, used64);
       // near blocks come in runs once the points are bucketed: after one,
       // start the next block's FP64 fragment loads
       if (used64 && jnext != jb) fetch64(f64p, jnext);
@@ -1048,13 +1276,13 @@ static hipError_t mixed_launch(const Problem &p, const FusedPlan &plan, double *
 }
 
 template <int KP>
-static void launch_bounds(const Problem &p, int ntb, int nsb, const float *pivB, const float *pn,
+static void launch_bounds(const Problem &p, int ntb, int nsb, const double *pivA, const float *piv,
+                          const float *pn, const double *pn64, const double *ppd,
                           const unsigned *ka, const unsigned *kb, const int *prow,
-                          const int *pcol, float *Vt, float *Vs, unsigned *rt, unsigned *rs,
-                          hipStream_t s) {
+                          const int *pcol, double xs, const MixTabs &o, hipStream_t s) {
   const unsigned wgs = (unsigned)(((long)ntb + nsb + 3) / 4);
-  hipLaunchKernelGGL((mix_bounds_kernel<KP>), dim3(wgs), dim3(256), 0, s, p, ntb, nsb, pivB, pn, ka,
-                     kb, prow, pcol, Vt, Vs, rt, rs);
+  hipLaunchKernelGGL((mix_bounds_kernel<KP>), dim3(wgs), dim3(256), 0, s, p, ntb, nsb, pivA, piv,
+                     pn, pn64, ppd, ka, kb, prow, pcol, xs, o);
 }
 
 template <int KP>
@@ -1128,9 +1356,18 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   const size_t ntb = mp / 16, nsb = np / 16;
   const size_t ngrp = mp / 32;  // row groups of one wave
   const size_t ngw = ((size_t)plan.cols_per_split / 16 + 63) / 64;
-  const size_t o_pivb = off; off += bounds ? al((size_t)P * kp * sizeof(float)) : 0;
+  const size_t o_pivb = off; off += bounds ? al((size_t)P * kp * sizeof(double)) : 0;
   const size_t o_pp = off; off += bounds ? al((size_t)P * P * sizeof(float)) : 0;
   const size_t o_pn = off; off += bounds ? al((size_t)P * sizeof(float)) : 0;
+  const size_t o_ppd = off; off += bounds ? al((size_t)P * P * sizeof(double)) : 0;
+  const size_t o_pn64 = off; off += bounds ? al((size_t)P * sizeof(double)) : 0;
+  // the centred near tier's tables and operands (MixTabs)
+  const size_t o_ex = off; off += bounds ? al((size_t)P * mp * sizeof(float)) : 0;
+  const size_t o_qx = off; off += bounds ? al((size_t)P * np * sizeof(float)) : 0;
+  const size_t o_ac = off; off += bounds ? al(mp * kp * sizeof(float)) : 0;
+  const size_t o_bc = off; off += bounds ? al(np * kp * sizeof(float)) : 0;
+  const size_t o_na = off; off += bounds ? al(mp * sizeof(float)) : 0;
+  const size_t o_nb = off; off += bounds ? al(np * sizeof(float)) : 0;
   const size_t o_vt = off; off += bounds ? al((size_t)P * ntb * sizeof(float)) : 0;
   const size_t o_vs = off; off += bounds ? al((size_t)P * nsb * sizeof(float)) : 0;
   const size_t o_rt = off; off += bounds ? al(ntb * sizeof(unsigned)) : 0;
@@ -1161,6 +1398,12 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   pk.pcol = pcol;
   pk.need = nullptr;
   pk.ngw = (int)ngw;
+  pk.Ex = pk.Qx = nullptr;
+  pk.Ac = pk.Bc = nullptr;
+  pk.na = pk.nb = nullptr;
+  pk.rt = pk.rs = nullptr;
+  pk.mp = (long)mp;
+  pk.np = (long)np;
 
   // ---- target and source orders
   if (!bucket) {
@@ -1171,9 +1414,9 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
     unsigned *ka = (unsigned *)(base + o_ka), *kb = (unsigned *)(base + o_kb);
     unsigned *kouta = (unsigned *)(base + o_kouta), *koutb = (unsigned *)(base + o_koutb);
     float *piv = (float *)(base + o_piv);
-    float *pivB = bounds ? (float *)(base + o_pivb) : nullptr;
+    double *pivA = bounds ? (double *)(base + o_pivb) : nullptr;
     hipLaunchKernelGGL(mix_pivot_kernel, dim3((unsigned)((P * kp + 255) / 256)), dim3(256), 0, s, p,
-                       P, kp, piv, pivB);
+                       P, kp, piv, pivA);
     switch (nc) {
       case 1: launch_bucket<32>(p, P, piv, ka, kb, s); break;
       case 2: launch_bucket<64>(p, P, piv, ka, kb, s); break;
@@ -1193,16 +1436,40 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
     if (bounds) {
       if (P != kPivots) return KS_ERR_INVALID_ARG;  // the bound kernels take all kPivots tiles
       float *pp = (float *)(base + o_pp), *pn = (float *)(base + o_pn);
-      float *Vt = (float *)(base + o_vt), *Vs = (float *)(base + o_vs);
-      unsigned *rt = (unsigned *)(base + o_rt), *rs = (unsigned *)(base + o_rs);
+      double *ppd = (double *)(base + o_ppd), *pn64 = (double *)(base + o_pn64);
+      MixTabs tb;
+      tb.Vt = (float *)(base + o_vt);
+      tb.Vs = (float *)(base + o_vs);
+      tb.rt = (unsigned *)(base + o_rt);
+      tb.rs = (unsigned *)(base + o_rs);
+      tb.Ex = (float *)(base + o_ex);
+      tb.Qx = (float *)(base + o_qx);
+      tb.Ac = (float4 *)(base + o_ac);
+      tb.Bc = (float4 *)(base + o_bc);
+      tb.na = (float *)(base + o_na);
+      tb.nb = (float *)(base + o_nb);
+      tb.mp = (long)mp;
+      tb.np = (long)np;
+      float *Vt = tb.Vt, *Vs = tb.Vs;
+      unsigned *rt = tb.rt, *rs = tb.rs;
       hipLaunchKernelGGL(mix_pivot_gram_kernel, dim3((unsigned)P), dim3((unsigned)P), 0, s, P, kp,
-                         (const float *)piv, pp, pn);
+                         (const float *)piv, pp, pn, ppd, pn64);
+      // exponents in log2 units: xs = scal log2(e) in FP64 for the tables
+      const double xs64 = p.P.scal * 1.4426950408889634;
       switch (nc) {
-        case 1: launch_bounds<32>(p, (int)ntb, (int)nsb, pivB, pn, kouta, koutb, prow, pcol, Vt, Vs, rt, rs, s); break;
-        case 2: launch_bounds<64>(p, (int)ntb, (int)nsb, pivB, pn, kouta, koutb, prow, pcol, Vt, Vs, rt, rs, s); break;
-        case 3: launch_bounds<96>(p, (int)ntb, (int)nsb, pivB, pn, kouta, koutb, prow, pcol, Vt, Vs, rt, rs, s); break;
-        default: launch_bounds<128>(p, (int)ntb, (int)nsb, pivB, pn, kouta, koutb, prow, pcol, Vt, Vs, rt, rs, s); break;
+        case 1: launch_bounds<32>(p, (int)ntb, (int)nsb, pivA, piv, pn, pn64, ppd, kouta, koutb, prow, pcol, xs64, tb, s); break;
+        case 2: launch_bounds<64>(p, (int)ntb, (int)nsb, pivA, piv, pn, pn64, ppd, kouta, koutb, prow, pcol, xs64, tb, s); break;
+        case 3: launch_bounds<96>(p, (int)ntb, (int)nsb, pivA, piv, pn, pn64, ppd, kouta, koutb, prow, pcol, xs64, tb, s); break;
+        default: launch_bounds<128>(p, (int)ntb, (int)nsb, pivA, piv, pn, pn64, ppd, kouta, koutb, prow, pcol, xs64, tb, s); break;
       }
+      pk.Ex = tb.Ex;
+      pk.Qx = tb.Qx;
+      pk.Ac = tb.Ac;
+      pk.Bc = tb.Bc;
+      pk.na = tb.na;
+      pk.nb = tb.nb;
+      pk.rt = tb.rt;
+      pk.rs = tb.rs;
       MixBounds mb;
       mb.Vt = Vt;
       mb.Vs = Vs;
@@ -1211,11 +1478,11 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
       mb.rs = rs;
       mb.ntb = (int)ntb;
       mb.nsb = (int)nsb;
-      // a block pair whose d^2 lower bound exceeds 60 / |xs| has every exponent
-      // xs d^2 below -60 (K < 2^-60); the factor covers xs = scal log2(e)
-      // rounded to FP32
+      // a block pair whose d^2 lower bound exceeds -kSkipT2 / |xs| has every
+      // exponent xs d^2 below kSkipT2 (K < 2^-50); the factor covers xs =
+      // scal log2(e) rounded to FP32
       const float xs = (float)(p.P.scal * (double)kLog2e);
-      mb.thr_d2 = (float)(60.0 / fabs((double)xs) * (1.0 + 0x1p-20));
+      mb.thr_d2 = (float)(-(double)kSkipT2 / fabs((double)xs) * (1.0 + 0x1p-20));
       unsigned long long *need = (unsigned long long *)(base + o_need);
       const long nwaves = (long)ngrp * plan.nsplit;
       hipLaunchKernelGGL(mix_need_kernel, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, s, mb,

@@ -89,7 +89,7 @@ inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs
 int single_default_precision(int type, int m, int n, int k) {
   // sgsks() sums in FP64: the mixed kernels are accurate to 1e-6 only where near
   // pairs carry the sums (a row made only of far pairs is accurate to about
-  // 1e-5, and the skip tier drops pairs with K < 2^-60 that a float could
+  // 1e-5, and the skip tier drops pairs with K < 2^-50 that a float could
   // still hold), which depends on the data.  Callers that know their data ask
   // for mixed precision through sgsks_device.
   (void)type;

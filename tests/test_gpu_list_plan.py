@@ -312,7 +312,7 @@ def test_mixed_slices_per_kernel_type(tmp_path, torch_cuda, oracle, gauss, lap, 
     started an iteration without lane 0 after a row block's last tile).  Every
     run must match the oracle, and repeated plan runs must agree bit for bit."""
     cache = tmp_path / "tiles.txt"
-    cache.write_text(f"0 3 0 {gauss} 0\n2 3 0 {lap} 0\n1 3 0 {poly} 0\n")
+    cache.write_text(f"gfx950 0 3 0 {gauss} 0\ngfx950 2 3 0 {lap} 0\ngfx950 1 3 0 {poly} 0\n")
     out = tmp_path / "u.npz"
     code = f"""
 import ctypes, sys
