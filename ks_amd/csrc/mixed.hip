@@ -811,20 +811,25 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
   };
 
   // ---- the centred near tier (skip-tier kernel with pk.Ex)
-  // A near block pair (T in bucket A, S in bucket B, one bucket each) takes its
-  // exponent as Ex[B][t] + Qx[A][j] - 2 xs alpha.beta (MixTabs): the cross term
-  // of the centred coordinates on the FP32 matrix cores, 16 steps of
-  // v_mfma_f32_16x16x4f32 at k = 64 in two chains, and one v_exp_f32 per pair.
-  // The exponent's error (log2 units) is at most
-  //   u (|Ex| + |Qx| + |Ex + Qx| + |2 xs c| + |x2|) + (kp + 4) u |2 xs| A' |beta|
-  // with u = 2^-24, c the computed cross term and A' the largest |alpha| of the
-  // wave's rows: the five roundings of the FP32 evaluation, and the cross term's
-  // (coordinates rounded to FP32, sums of kp / 2 products per chain).  A block
-  // where any pair's bound exceeds 2^-15 -- or whose target or source block
-  // spans buckets -- is recomputed exactly in FP64 instead, so every kernel
-  // value of the tier is within ln2 2^-15 + 2^-22 (v_exp_f32) relative.
+  // A block pair (T in bucket A, S in bucket B, one bucket each) takes its
+  // exponents as x2 = fl(Ex[B][t] + Qx[A][j]) + m2xs c (one FMA), with c the
+  // cross term alpha.beta of the centred coordinates on the FP32 matrix cores
+  // (16 steps of v_mfma_f32_16x16x4f32 at k = 64, in two chains), then one
+  // v_exp_f32 per pair.  With u = 2^-24, the exponent's error (log2 units) is
+  // at most
+  //   u (2 (|Ex| + |Qx|) + (kp + 4) |m2xs| A' |beta|),
+  // A' the largest |alpha| of the wave's rows: the roundings of Ex and Qx
+  // (half an ulp each, FP64 values), of their sum and of the FMA (|x2| <=
+  // |Ex| + |Qx| + |m2xs c|), of m2xs and of c0 + c1, and the cross term's
+  // (coordinates rounded to FP32, sums of kp / 2 products per chain), with
+  // |c| <= A' |beta| (1 + (kp + 2) u).  The bound is checked before the
+  // products; a block where any lane's bound exceeds 2^-15 -- or whose target
+  // or source block spans buckets -- takes the split-BF16 estimate and the FP64
+  // near tier instead, so every kernel value of this tier is within
+  // ln2 2^-15 + 2^-22 (v_exp_f32) relative.
   struct C32Frag {
     float4 b[NC4];
+    float4 cb;    // the lane's source record {xs |b|^2, |b - p_own|, w, |b|}
     float q[RB];  // Qx[A][j] of the lane's source for each row block's bucket A
     int block;    // the source block it holds (or has in flight), -1 for none
     int bucket;   // its single bucket, -1 if it spans several
@@ -833,6 +838,12 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
   float aA = 0.0f;  // largest |alpha| of the wave's rows
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) abk[rb] = -1;
+  // Ex[B][rows of each row block] for the last source bucket B seen (runs of
+  // source blocks share a bucket once the points are sorted), and the largest
+  // |Ex| of the lane's rows
+  float4 ecache[RB];
+  float emax[RB];
+  int ebucket = -1;
   auto fetch32 = [&](C32Frag &f, int jb) {
     const float4 *src = pk.Bc + (long)(jb >> 4) * NC4 * 64 + lane;
 #pragma unroll
@@ -840,9 +851,20 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
       f.q[rb] = abk[rb] >= 0 ? pk.Qx[(long)abk[rb] * pk.np + jb + p16] : 0.0f;
+    f.cb = pk.cb[jb + p16];
     const unsigned r = pk.rs[jb >> 4];
     f.bucket = (r & 0xFFFFu) == (r >> 16) ? (int)(r & 0xFFFFu) : -1;
     f.block = jb;
+    if (f.bucket >= 0 && f.bucket != ebucket) {
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        const float4 e = *reinterpret_cast<const float4 *>(pk.Ex + (long)f.bucket * pk.mp + row0 +
+                                                           rb * 16 + 4 * g);
+        ecache[rb] = e;
+        emax[rb] = fmaxf(fmaxf(fabsf(e.x), fabsf(e.y)), fmaxf(fabsf(e.z), fabsf(e.w)));
+      }
+      ebucket = f.bucket;
+    }
   };
 
   if constexpr (SK) {
@@ -858,9 +880,14 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
     }
   }
   const float kCross = (32.0f * NC + 4.0f) * fabsf(m2xs) * aA;  // times |beta|
-  // Row block rb against the source block of cf: false (nothing added) when
-  // the bound sends the block to FP64.
-  auto near32 = [&](int rb, const C32Frag &cf, const float4 &cb) -> bool {
+  constexpr float kBnd = 0x1p-24f * 1.01f;
+  // Row block rb against the source block of cf, every pair (near or far):
+  // false (nothing added) when the bound hands the block back.
+  auto near32 = [&](int rb, const C32Frag &cf) -> bool {
+    const float4 &cb = cf.cb;
+    const float q = cf.q[rb];
+    const float bnd = kBnd * (2.0f * (emax[rb] + fabsf(q)) + kCross * cb.y);
+    if (__ballot(bnd > 0x1p-15f) != 0ull) return false;  // wave-uniform
     f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
     const float4 *sa = &s_ac[wave][rb * NC4 * 64 + lane];
 #pragma unroll
@@ -876,23 +903,14 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
       c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.z, b1.z, c1, 0, 0, 0);
       c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.w, b1.w, c1, 0, 0, 0);
     }
-    const float4 e = *reinterpret_cast<const float4 *>(pk.Ex + (long)cf.bucket * pk.mp + row0 +
-                                                        rb * 16 + 4 * g);
-    const float q = cf.q[rb];
+    const float4 e = ecache[rb];  // cf.bucket == ebucket
     const float ev[4] = {e.x, e.y, e.z, e.w};
-    float kv[4], err = 0.0f;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const float cv = c0[v] + c1[v];
-      const float sm = ev[v] + q;
-      const float x2 = fmaf(m2xs, cv, sm);
-      err = fmaxf(err, fabsf(ev[v]) + fabsf(q) + fabsf(sm) + fabsf(m2xs * cv) + fabsf(x2));
-      kv[v] = __builtin_amdgcn_exp2f(fminf(x2, 0.0f));  // d^2 clamped at 0
+      const float x2 = fmaf(m2xs, c0[v] + c1[v], ev[v] + q);
+      // d^2 clamped at 0
+      facc[rb][v] = fmaf(__builtin_amdgcn_exp2f(fminf(x2, 0.0f)), cb.z, facc[rb][v]);
     }
-    const float bnd = 0x1p-24f * 1.01f * (err + kCross * cb.y);
-    if (__ballot(bnd > 0x1p-15f) != 0ull) return false;  // wave-uniform
-#pragma unroll
-    for (int v = 0; v < 4; ++v) facc[rb][v] = fmaf(kv[v], cb.z, facc[rb][v]);
     return true;
   };
 
@@ -990,11 +1008,7 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
 #pragma unroll
       for (int v = 0; v < 4; ++v) facc[rb][v] = fmaf(__builtin_amdgcn_exp2f(x2[v]), cb.z, facc[rb][v]);
     } else if constexpr (SK) {
-      // near block: the centred FP32 tier where its bound holds, else FP64
-      if (pk.Ex && abk[rb] >= 0) {
-        if (c32.block != jb) fetch32(c32, jb);
-        if (c32.bucket >= 0 && near32(rb, c32, cb)) return;
-      }
+      // near block the centred tier handed back (or had no tables for): FP64
       near64_lean(rb, jb);
     } else {
       if (f64.block != jb) fetch64(f64, jb);
@@ -1096,15 +1110,27 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
     };
     // the row blocks of one source block that passed: the two correction
     // products complete the split-BF16 cross term, then far / near
+    // centred fragment of the current block, or of the next needed one: after a
+    // block that took the centred tier, the next needed block's loads start
+    // (passing blocks come in runs once the points are sorted)
+    C32Frag c32;
+    c32.block = -1;
     auto finish = [&](int jb, int jnext, const HB &b, const f4 *e, const bool *h) {
-      bool used64 = false, usedlo = false;
+      bool used64 = false, usedlo = false, used32 = false;
       F64Frag f64;  // unused in the skip-tier kernel (its FP64 fallback loads its own)
       f64.block = -1;
-      C32Frag c32;  // live within this block only: loaded by its first near row block
-      c32.block = -1;
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb) {
         if (!h[rb]) continue;  // wave-uniform
+        // every pair of the block in the centred FP32 tier where it applies
+        if (pk.Ex && abk[rb] >= 0) {
+          if (c32.block != jb) fetch32(c32, jb);
+          if (c32.bucket >= 0 && near32(rb, c32)) {
+            used32 = true;
+            continue;
+          }
+        }
+        // else the split-BF16 estimate, then far (FP32) or near (FP64)
         if (lo_block != jb) fetch_lo(jb);
         usedlo = true;
         f4 c = e[rb];
@@ -1120,8 +1146,11 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
         evaluate(rb, jb, c, bcb, f64, c32, used64);
       }
       // blocks that pass come in runs once the points are bucketed: after one,
-      // start the next needed block's lo loads
-      if (usedlo && jnext >= 0 && jnext != jb) fetch_lo(jnext);
+      // start the next needed block's loads
+      if (jnext >= 0 && jnext != jb) {
+        if (used32) fetch32(c32, jnext);
+        if (usedlo) fetch_lo(jnext);
+      }
     };
     // one step over the pair (b0, b1); false once the needed blocks are done
     auto step = [&](const HB &b0, const HB &b1, int jafter) {
@@ -1152,22 +1181,19 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
       tick(two ? 2 : 1);
       return two;
     };
-    // three pairs of buffers in turn, two pairs ahead; a buffer past the last
+    // two pairs of buffers in turn, one pair ahead; a buffer past the last
     // needed block holds jb = -1 and is neither loaded nor evaluated
-    HB pa0, pa1, pb0, pb1, pc0, pc1;
+    HB pa0, pa1, pb0, pb1;
     auto fetch_pair = [&](HB &x0, HB &x1) {
       fetch_h(x0);
       fetch_h(x1);
     };
     fetch_pair(pa0, pa1);
-    fetch_pair(pb0, pb1);
     for (;;) {
-      fetch_pair(pc0, pc1);
+      fetch_pair(pb0, pb1);
       if (!step(pa0, pa1, pb0.jb)) break;
       fetch_pair(pa0, pa1);
-      if (!step(pb0, pb1, pc0.jb)) break;
-      fetch_pair(pb0, pb1);
-      if (!step(pc0, pc1, pa0.jb)) break;
+      if (!step(pb0, pb1, pa0.jb)) break;
     }
   } else {
     F64Frag f64p;  // persists across blocks: prefetched after a near block
