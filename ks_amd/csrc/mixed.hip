@@ -119,20 +119,24 @@ struct MixPack {
   long mp, np;
 };
 
-// FP32 copies of the pivots: pivot i is target (i * m) / P, zero past k.  With
-// pivA, also as FP64 in the A-operand order of v_mfma_f64_16x16x4f64 for
+// FP32 copies of the pivots: pivot i is target (i * m) / P, zero past k; also
+// in the B-operand order of v_mfma_f32_16x16x4f32 for mix_bucket_kernel (pivB:
+// tile t, step st, lane 16 g + c holding coordinate 4 st + g of pivot 16 t + c).
+// With pivA, also as FP64 in the A-operand order of v_mfma_f64_16x16x4f64 for
 // mix_bounds_kernel: tile t (pivots 16 t ..), step st (coordinates 4 st ..),
 // lane 16 g + c holding coordinate 4 st + g of pivot 16 t + f64_a_row(c) -- the
 // row order that lands pivot 16 t + 4 g + v in accumulator slot v of lane group
 // g -- so that one fragment load is 512 contiguous bytes.  f64_a_row is its own
 // inverse (a 4 x 4 transpose of the row index).
-__global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv, double *pivA) {
+__global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv, float *pivB,
+                                 double *pivA) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P * kp) return;
   const int i = t / kp, q = t - i * kp;
   const int r = (int)(((long)i * p.m) / P);
   const float v = q < p.k ? (float)p.XA[(long)p.amap[r] * p.k + q] : 0.0f;
   piv[t] = v;
+  pivB[((long)(i >> 4) * (kp >> 2) + (q >> 2)) * 64 + (q & 3) * 16 + (i & 15)] = v;
   if (pivA) {
     const int c = 4 * ((i & 15) & 3) + ((i & 15) >> 2);
     pivA[((long)(i >> 4) * (kp >> 2) + (q >> 2)) * 64 + (q & 3) * 16 + c] = (double)v;
@@ -140,43 +144,83 @@ __global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv, dou
 }
 
 // Bucket (nearest pivot) of every target, then every source: the pivot with the
-// least |p|^2 - 2 x.p in FP32 (|x - p|^2 less the point's own |x|^2, one FMA per
-// coordinate).  The buckets only decide which points share a block.
+// least |p|^2 - 2 x.p in FP32 (|x - p|^2 less the point's own |x|^2), ties to
+// the lowest pivot.  One wave per 16 points (targets, then sources, in call
+// order): their dot products with the 128 pivots on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32, the points as the A operand, the pivots as B from
+// pivB, eight 16-pivot tiles), |p|^2 from the same fragments.  The buckets only
+// decide which points share a block.
 template <int KP>
-__global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p, int P,
-                                                         const float *__restrict__ piv,
+__global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p,
+                                                         const float *__restrict__ pivB,
                                                          unsigned *key_a, unsigned *key_b) {
-  __shared__ float sp[kPivots * KP];
-  __shared__ float spn[kPivots];
-  for (int e = threadIdx.x; e < P * KP; e += 256) sp[e] = piv[e];
-  __syncthreads();
-  for (int i = threadIdx.x; i < P; i += 256) {
-    float n2 = 0.0f;
-    for (int q = 0; q < KP; ++q) n2 = fmaf(sp[i * KP + q], sp[i * KP + q], n2);
-    spn[i] = n2;
+  constexpr int NS = KP / 4;
+  constexpr int NT = kPivots / 16;
+  const int lane = threadIdx.x & 63, p16 = lane & 15, g = lane >> 4;
+  const long blk = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long tot = (long)p.m + p.n;
+  if (blk * 16 >= tot) return;  // wave-uniform
+  // the A operand: point blk 16 + p16 supplies coordinates 4 st + g
+  const long t = blk * 16 + p16;
+  const double *x = nullptr;
+  if (t < p.m) x = p.XA + (long)p.amap[t] * p.k;
+  else if (t < tot) x = p.XB + (long)p.bmap[t - p.m] * p.k;
+  float a[NS];
+#pragma unroll
+  for (int st = 0; st < NS; ++st) {
+    const int q = 4 * st + g;
+    a[st] = (x && q < p.k) ? (float)x[q] : 0.0f;
   }
-  __syncthreads();
-  const long t = (long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (long)p.m + p.n) return;
-  const double *x = t < p.m ? p.XA + (long)p.amap[t] * p.k : p.XB + (long)p.bmap[t - p.m] * p.k;
-  float xv[KP];
+  float best[4];
+  int bi[4];
 #pragma unroll
-  for (int q = 0; q < KP; ++q) xv[q] = q < p.k ? (float)x[q] : 0.0f;
-  float best = INFINITY;
-  unsigned bi = 0;
-  for (int i = 0; i < P; ++i) {
-    const float *pv = sp + i * KP;
-    float dot = 0.0f;
+  for (int v = 0; v < 4; ++v) {
+    best[v] = INFINITY;
+    bi[v] = 0;
+  }
 #pragma unroll
-    for (int q = 0; q < KP; ++q) dot = fmaf(xv[q], pv[q], dot);
-    const float d = fmaf(-2.0f, dot, spn[i]);
-    if (d < best) {
-      best = d;
-      bi = (unsigned)i;
+  for (int tt = 0; tt < NT; ++tt) {
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    float n2 = 0.0f;
+    const float *pv = pivB + (long)tt * NS * 64 + lane;
+#pragma unroll
+    for (int st = 0; st < NS; ++st) {
+      const float b = pv[st * 64];
+      n2 = fmaf(b, b, n2);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], b, acc, 0, 0, 0);
+    }
+    n2 += __shfl_xor(n2, 16);
+    n2 += __shfl_xor(n2, 32);  // |p|^2 of pivot 16 tt + p16
+    // acc[v]: point 4 g + v, pivot 16 tt + p16
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float d = fmaf(-2.0f, acc[v], n2);
+      if (d < best[v]) {
+        best[v] = d;
+        bi[v] = 16 * tt + p16;
+      }
     }
   }
-  if (t < p.m) key_a[t] = bi;
-  else key_b[t - p.m] = bi;
+  // the least over the 16 lanes of the point's lane group, ties to the lower pivot
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float ob = __shfl_xor(best[v], o);
+      const int oi = __shfl_xor(bi[v], o);
+      if (ob < best[v] || (ob == best[v] && oi < bi[v])) {
+        best[v] = ob;
+        bi[v] = oi;
+      }
+    }
+  if (p16 == 0) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const long tp = blk * 16 + 4 * g + v;
+      if (tp < p.m) key_a[tp] = (unsigned)bi[v];
+      else if (tp < tot) key_b[tp - p.m] = (unsigned)bi[v];
+    }
+  }
 }
 
 // ---- pruning bounds (skip tier)
@@ -296,22 +340,11 @@ __global__ __launch_bounds__(256) void mix_bounds_kernel(
   }
   n2 += __shfl_xor(n2, 16);
   n2 += __shfl_xor(n2, 32);  // |x|^2 of point p16 on its four lanes
-  d4 acc[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    acc[t] = d4{0.0, 0.0, 0.0, 0.0};
-    const double *pv = pivA + (long)t * NS * 64 + lane;
-#pragma unroll
-    for (int st = 0; st < NS; ++st)
-      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv[st * 64], a[st], acc[t], 0, 0, 0);
-  }
-  // acc[t][v]: pivot 16 t + 4 g + v, point p16.  x.p_own: picked out of the slot
-  // that holds it, then summed over the point's four lanes.
+  // x.p_own in FP64 from the point's own coordinates (lanes of the point's four
+  // coordinate groups summed)
   double dsel = 0.0;
 #pragma unroll
-  for (int t = 0; t < NT; ++t)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) dsel = (16 * t + 4 * g + v == own) ? acc[t][v] : dsel;
+  for (int st = 0; st < NS; ++st) dsel = fma(a[st], (double)piv[own * KP + 4 * st + g], dsel);
   dsel += __shfl_xor(dsel, 16);
   dsel += __shfl_xor(dsel, 32);
   float pmx = fmaxf(pn[lane], pn[64 + lane]);
@@ -324,45 +357,61 @@ __global__ __launch_bounds__(256) void mix_bounds_kernel(
   const bool tabs = o.Ex != nullptr;
   float *etab = tgt ? o.Ex : o.Qx;
   const long estride = tgt ? o.mp : o.np;
-  float best[NT][4];
+  // the pivot tiles in two halves (fewer live accumulators, more waves per SIMD)
+  constexpr int NH = NT / 2;
+#pragma unroll 1
+  for (int h = 0; h < 2; ++h) {
+    d4 acc[NH];
 #pragma unroll
-  for (int t = 0; t < NT; ++t)
+    for (int t = 0; t < NH; ++t) {
+      acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+      const double *pv = pivA + (long)(h * NH + t) * NS * 64 + lane;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = 16 * t + 4 * g + v;
-      const double pni = pn64[i];
-      const double diff = (pni - pno) - 2.0 * (acc[t][v] - dsel);  // e_i(x) - e_own(x)
-      best[t][v] = valid ? (float)diff - mu : INFINITY;
-      if (tabs) {
-        // e_i(x) = |x - p_i|^2 with the caller's norm in place of |x|^2: the
-        // tables then carry the reference's d^2 = XA2 + XB2 - 2 a.b exactly
-        // (norms that are not the points' own, e.g. rounded ones, included)
-        double e = nrm + pni - 2.0 * acc[t][v];
-        if (!tgt) e -= ppd[i * kPivots + own];
-        etab[(long)i * estride + pos] = valid ? (float)(xs * e) : 0.0f;
+      for (int st = 0; st < NS; ++st)
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv[st * 64], a[st], acc[t], 0, 0, 0);
+    }
+    // acc[t][v]: pivot 16 (h NH + t) + 4 g + v, point p16
+    float best[NH][4];
+#pragma unroll
+    for (int t = 0; t < NH; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * (h * NH + t) + 4 * g + v;
+        const double pni = pn64[i];
+        const double diff = (pni - pno) - 2.0 * (acc[t][v] - dsel);  // e_i(x) - e_own(x)
+        best[t][v] = valid ? (float)diff - mu : INFINITY;
+        if (tabs) {
+          // e_i(x) = |x - p_i|^2 with the caller's norm in place of |x|^2: the
+          // tables then carry the reference's d^2 = XA2 + XB2 - 2 a.b exactly
+          // (norms that are not the points' own, e.g. rounded ones, included)
+          double e = nrm + pni - 2.0 * acc[t][v];
+          if (!tgt) e -= ppd[i * kPivots + own];
+          etab[(long)i * estride + pos] = valid ? (float)(xs * e) : 0.0f;
+        }
       }
-    }
-  // block minima over the 16 points
+    // block minima over the 16 points
 #pragma unroll
-  for (int t = 0; t < NT; ++t)
+    for (int t = 0; t < NH; ++t)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      float b = best[t][v];
-      b = fminf(b, __shfl_xor(b, 1));
-      b = fminf(b, __shfl_xor(b, 2));
-      b = fminf(b, __shfl_xor(b, 4));
-      b = fminf(b, __shfl_xor(b, 8));
-      best[t][v] = b;
-    }
-  if (p16 == 0) {
+      for (int v = 0; v < 4; ++v) {
+        float b = best[t][v];
+        b = fminf(b, __shfl_xor(b, 1));
+        b = fminf(b, __shfl_xor(b, 2));
+        b = fminf(b, __shfl_xor(b, 4));
+        b = fminf(b, __shfl_xor(b, 8));
+        best[t][v] = b;
+      }
+    if (p16 == 0) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      if (tgt) {
-        *reinterpret_cast<float4 *>(&o.Vt[(long)b0 * kPivots + 16 * t + 4 * g]) =
-            make_float4(best[t][0], best[t][1], best[t][2], best[t][3]);
-      } else {
+      for (int t = 0; t < NH; ++t) {
+        const int i0 = 16 * (h * NH + t) + 4 * g;
+        if (tgt) {
+          *reinterpret_cast<float4 *>(&o.Vt[(long)b0 * kPivots + i0]) =
+              make_float4(best[t][0], best[t][1], best[t][2], best[t][3]);
+        } else {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) o.Vs[(long)(16 * t + 4 * g + v) * nsb + b0] = best[t][v];
+          for (int v = 0; v < 4; ++v) o.Vs[(long)(i0 + v) * nsb + b0] = best[t][v];
+        }
       }
     }
   }
@@ -1312,11 +1361,11 @@ static void launch_bounds(const Problem &p, int ntb, int nsb, const double *pivA
 }
 
 template <int KP>
-static void launch_bucket(const Problem &p, int P, const float *piv, unsigned *ka, unsigned *kb,
+static void launch_bucket(const Problem &p, const float *pivB, unsigned *ka, unsigned *kb,
                           hipStream_t s) {
-  const long tot = (long)p.m + p.n;
-  hipLaunchKernelGGL((mix_bucket_kernel<KP>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
-                     p, P, piv, ka, kb);
+  const long blocks = ((long)p.m + p.n + 15) / 16;
+  hipLaunchKernelGGL((mix_bucket_kernel<KP>), dim3((unsigned)((blocks + 3) / 4)), dim3(256), 0, s,
+                     p, pivB, ka, kb);
 }
 
 int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
@@ -1376,6 +1425,7 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   const size_t o_kouta = off; off += bucket ? al((size_t)p.m * sizeof(unsigned)) : 0;
   const size_t o_koutb = off; off += bucket ? al((size_t)p.n * sizeof(unsigned)) : 0;
   const size_t o_piv = off; off += bucket ? al((size_t)P * kp * sizeof(float)) : 0;
+  const size_t o_pivf = off; off += bucket ? al((size_t)P * kp * sizeof(float)) : 0;
   // pruning bounds of the skip tier (bucketed BF16x3 Gaussian with a decaying kernel)
   const bool bounds = bucket && precision == KS_PREC_MIXED_BF16X3 && p.type == GAUSSIAN &&
                       p.P.scal < 0.0;
@@ -1441,13 +1491,14 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
     unsigned *kouta = (unsigned *)(base + o_kouta), *koutb = (unsigned *)(base + o_koutb);
     float *piv = (float *)(base + o_piv);
     double *pivA = bounds ? (double *)(base + o_pivb) : nullptr;
+    float *pivB = (float *)(base + o_pivf);
     hipLaunchKernelGGL(mix_pivot_kernel, dim3((unsigned)((P * kp + 255) / 256)), dim3(256), 0, s, p,
-                       P, kp, piv, pivA);
+                       P, kp, piv, pivB, pivA);
     switch (nc) {
-      case 1: launch_bucket<32>(p, P, piv, ka, kb, s); break;
-      case 2: launch_bucket<64>(p, P, piv, ka, kb, s); break;
-      case 3: launch_bucket<96>(p, P, piv, ka, kb, s); break;
-      default: launch_bucket<128>(p, P, piv, ka, kb, s); break;
+      case 1: launch_bucket<32>(p, pivB, ka, kb, s); break;
+      case 2: launch_bucket<64>(p, pivB, ka, kb, s); break;
+      case 3: launch_bucket<96>(p, pivB, ka, kb, s); break;
+      default: launch_bucket<128>(p, pivB, ka, kb, s); break;
     }
     hipLaunchKernelGGL(mix_identity_kernel, dim3((unsigned)((mx + 255) / 256)), dim3(256), 0, s,
                        ids, (long)mx);
