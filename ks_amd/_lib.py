@@ -80,6 +80,7 @@ _SIGS = {
                                            ctypes.c_int]),
     "ks_list_plan_execute": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ks_list_plan_destroy": (None, [_vp]),
+    "ks_list_plan_symmetric_tasks": (ctypes.c_int, [_vp]),
     "ks_list_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
 }

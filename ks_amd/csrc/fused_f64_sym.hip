@@ -40,24 +40,6 @@ struct SymArgs {
   double *colp;        // [row blocks][m] column partials of the upper sources
 };
 
-// One DPP lane exchange of a double (two 32-bit moves), all lanes valid.
-template <int CTRL>
-__device__ __forceinline__ double sym_dpp(double x) {
-  int2 v = __builtin_bit_cast(int2, x);
-  v.x = __builtin_amdgcn_mov_dpp(v.x, CTRL, 0xF, 0xF, true);
-  v.y = __builtin_amdgcn_mov_dpp(v.y, CTRL, 0xF, 0xF, true);
-  return __builtin_bit_cast(double, v);
-}
-
-// Sum over the 16 lanes of each DPP row; every lane of a row gets its row's value.
-__device__ __forceinline__ double row_sum16(double x) {
-  x += sym_dpp<0xB1>(x);   // quad_perm [1,0,3,2]
-  x += sym_dpp<0x4E>(x);   // quad_perm [2,3,0,1]
-  x += sym_dpp<0x141>(x);  // row_half_mirror
-  x += sym_dpp<0x140>(x);  // row_mirror
-  return x;
-}
-
 template <int TYPE, int K, int R>
 __global__ __launch_bounds__(SWG) void fused_f64_sym_kernel(SymArgs A) {
   const Problem &p = A.p;

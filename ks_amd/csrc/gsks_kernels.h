@@ -281,6 +281,26 @@ __device__ __forceinline__ double exp_tab(double x, const double2 *__restrict__ 
   return ldexp(t.x + fma(t.x, p, t.y), n >> 6);
 }
 
+// One DPP lane exchange of a double (two 32-bit moves), all lanes valid.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  int2 v = __builtin_bit_cast(int2, x);
+  v.x = __builtin_amdgcn_mov_dpp(v.x, CTRL, 0xF, 0xF, true);
+  v.y = __builtin_amdgcn_mov_dpp(v.y, CTRL, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, v);
+}
+
+// Sum over the 16 lanes of each DPP row; every lane of a row gets its row's
+// value.  Butterflies: swap neighbours, swap pairs, mirror the 8-lane halves,
+// mirror the row (a fixed order: the same lanes give the same sum every run).
+__device__ __forceinline__ double row_sum16(double x) {
+  x += dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+  x += dpp_f64<0x141>(x);  // row_half_mirror
+  x += dpp_f64<0x140>(x);  // row_mirror
+  return x;
+}
+
 // A kernel's LDS copy of kExpTable, filled by its first 64 lanes (the caller
 // synchronises the workgroup before the first exp_tab).
 __device__ __forceinline__ void load_exp_table(double2 *tab) {

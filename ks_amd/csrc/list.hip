@@ -49,8 +49,9 @@ struct DevTask {
   int m, n;
   int bc;    // source slice per tile (the tuned tile shape)
   int tile;  // launch kind: 0 lane-per-row tile, 1 FP64 matrix-core tile (9 <= k <= 64),
-             // 2 the same for Laplace (a launch of its own)
-  int pad;
+             // 2 the same for Laplace (a launch of its own), 3 symmetric row tile
+  int sym;   // symmetric: the sources are the targets (same list, same point table)
+  int srb;   // symmetric: where its row blocks' indices start in ListArgs::symrb
   long a0, b0;
   KerParams P;
   const double *hi, *hj;
@@ -74,6 +75,12 @@ struct ListArgs {
   unsigned *head;      // tile counter (reset by list_rowsum_kernel after each run)
   double *R;           // [total_a + 64] per-(task, row) totals, then 64 sink entries
   long rsink;          // total_a: where the lanes past a row block's end write
+  // symmetric tasks: the column partials of each tile's later sources
+  // (colp + coff[tile], one per source from max(c0, row0 + 64) to c1), and each
+  // task's row blocks in row order (symrb[srb + row0 / 64] = row-block index)
+  double *colp;
+  const long *coff;
+  const int *symrb;
 };
 
 // ------------------------------------------------------------------ per-tile work
@@ -249,6 +256,131 @@ __device__ double tile_valu_lds(const ListArgs &L, const DevTask &T, int pos, in
   return acc;
 }
 
+// Symmetric row tile (lane = row, as tile_valu_lds) of a task whose sources
+// are its targets: rows row0 .. row0 + 63 against sources [c0, c1), c0 >= row0,
+// a slice of the upper triangle.  Sources of the rows' own block [row0,
+// row0 + 64) give row sums only -- every pair of the diagonal block, both
+// orders.  Each later source j gives its row sums and the column sum
+// sum_i K(a_i, b_j) w_i over the tile's rows (w_i: row i's weight as a source),
+// the term of target j from sources row0 .. row0 + 63, which K(a, b) = K(b, a)
+// makes the same pairs' values: it goes to colp[j - max(c0, row0 + 64)].  The
+// column sums are reduced within each 16-lane DPP row per source (lane 16q + c
+// keeping row q's share of source c of a 16-source group), then over the four
+// rows with two lane swaps per group: a fixed order.
+template <int TYPE, int K>
+__device__ double tile_valu_lds_sym(const ListArgs &L, const DevTask &T, int row0, int c0,
+                                    int c1, double *stage, double *colp, const double2 *etab) {
+  constexpr int CH = stage_rows(K);  // 64, 32 or 16: whole 16-source groups
+  constexpr int NV = stage_doubles(K) / 64;
+  const int k = L.k;
+  const int lane = threadIdx.x & 63;
+  const int pos = row0 + lane;
+  const bool rv = pos < T.m;
+  const int ai = L.amap[T.a0 + (rv ? pos : T.m - 1)];
+  const double *arow = L.XA + (long)ai * k;
+  double a[K];
+#pragma unroll
+  for (int p = 0; p < K; ++p) a[p] = p < k ? arow[p] : 0.0;
+  const double aa = L.XA2[ai];
+  const double hi = (TYPE == VAR_BANDWIDTH) ? -0.5 * T.hi[ai] : 0.0;
+  // the row's weight as a source (its position in the source list is its own);
+  // padding rows add nothing to a column
+  const double wi = rv ? L.w[L.wmap[T.b0 + pos]] : 0.0;
+  const KerParams P = T.P;
+  double *side = stage + stage_doubles(K);  // [CH][3]: |b|^2, w, hj
+  const int cs = max(c0, row0 + 64);        // the first later source of the tile
+  double acc = 0.0, part = 0.0;
+  for (int j0 = c0; j0 < c1; j0 += CH) {
+    const int nch = min(CH, c1 - j0);
+    double v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int e = lane + 64 * i, r = e / K, p = e % K;
+      const int rr = r < nch ? r : nch - 1;
+      const double x = L.XB[(long)L.bmap[T.b0 + j0 + rr] * k + (p < k ? p : k - 1)];
+      v[i] = (r < nch && p < k) ? x : 0.0;
+    }
+    double sb = 0.0, sw = 0.0, sh = 0.0;
+    if (lane < nch) {
+      const int bj = L.bmap[T.b0 + j0 + lane];
+      sb = L.XB2[bj];
+      sw = L.w[L.wmap[T.b0 + j0 + lane]];
+      sh = (TYPE == VAR_BANDWIDTH) ? T.hj[bj] : 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();  // the previous chunk's broadcasts are done
+#pragma unroll
+    for (int i = 0; i < NV; ++i) stage[lane + 64 * i] = v[i];
+    if (lane < CH) {
+      side[3 * lane] = sb;
+      side[3 * lane + 1] = sw;
+      side[3 * lane + 2] = sh;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // chunks start on multiples of CH from c0 (row0 or row0 + s bc, multiples of
+    // 64 apart), so a chunk lies wholly before or wholly after row0 + 64
+    const bool later = j0 >= row0 + 64;
+#pragma unroll 1
+    for (int r = 0; r < nch; ++r) {
+      const double bb = side[3 * r], wj = side[3 * r + 1], hj = side[3 * r + 2];
+      const double *b = stage + r * K;
+      double c;
+      if constexpr (TYPE == LAPLACE) {
+#pragma clang fp contract(off)
+        c = a[0] * b[0];
+#pragma unroll
+        for (int p = 1; p < K; ++p) c = c + a[p] * b[p];
+      } else {
+        c = a[0] * b[0];
+#pragma unroll
+        for (int p = 1; p < K; ++p) c = fma(a[p], b[p], c);
+      }
+      const double kv = kval_f64<TYPE, uses_exp<TYPE>>(c, aa, bb, hi, hj, P, etab);
+      acc = fma(kv, wj, acc);
+      if (later) {  // wave-uniform
+        const double rs = row_sum16(kv * wi);
+        part = (lane & 15) == (r & 15) ? rs : part;
+        if ((r & 15) == 15 || r == nch - 1) {
+          double tot = part + __shfl_xor(part, 16);
+          tot += __shfl_xor(tot, 32);
+          const int jj = j0 + (r & ~15) + (lane & 15);
+          if (lane < 16 && jj <= j0 + r) colp[jj - cs] = tot;
+          part = 0.0;
+        }
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  return acc;
+}
+
+template <int K>
+__device__ double sym_tile(const ListArgs &L, const DevTask &T, int row0, int c0, int c1,
+                           double *stage, double *colp, const double2 *etab) {
+  switch (T.type) {
+    case GAUSSIAN: return tile_valu_lds_sym<GAUSSIAN, K>(L, T, row0, c0, c1, stage, colp, etab);
+    case POLYNOMIAL:
+      switch (poly_type(T.P)) {
+        case POLY_SQUARE_T:
+          return tile_valu_lds_sym<POLY_SQUARE_T, K>(L, T, row0, c0, c1, stage, colp, etab);
+        case POLY_CUBE_T:
+          return tile_valu_lds_sym<POLY_CUBE_T, K>(L, T, row0, c0, c1, stage, colp, etab);
+        case POLY_FOURTH_T:
+          return tile_valu_lds_sym<POLY_FOURTH_T, K>(L, T, row0, c0, c1, stage, colp, etab);
+        default: return tile_valu_lds_sym<POLYNOMIAL, K>(L, T, row0, c0, c1, stage, colp, etab);
+      }
+    case LAPLACE: return tile_valu_lds_sym<LAPLACE, K>(L, T, row0, c0, c1, stage, colp, etab);
+    case VAR_BANDWIDTH:
+      return tile_valu_lds_sym<VAR_BANDWIDTH, K>(L, T, row0, c0, c1, stage, colp, etab);
+    case TANH: return tile_valu_lds_sym<TANH, K>(L, T, row0, c0, c1, stage, colp, etab);
+    case QUARTIC: return tile_valu_lds_sym<QUARTIC, K>(L, T, row0, c0, c1, stage, colp, etab);
+    case MULTIQUADRATIC:
+      return tile_valu_lds_sym<MULTIQUADRATIC, K>(L, T, row0, c0, c1, stage, colp, etab);
+    default: return tile_valu_lds_sym<EPANECHNIKOV, K>(L, T, row0, c0, c1, stage, colp, etab);
+  }
+}
+
 // 64 rows as 4 blocks of 16 on the FP64 matrix cores, in two passes of 2 row
 // blocks (the A fragments, norms and accumulators of 64 rows at once would not
 // fit in VGPRs beside the source fragments; the second pass re-reads the
@@ -422,21 +554,21 @@ __device__ double run_tile(const ListArgs &L, const DevTask &T, int row0, int c0
 }
 
 // One persistent launch over one tile kind (MF): the row tiles, the matrix-core
-// tiles and the Laplace matrix-core tiles (whose exact-recompute path needs more
-// registers) of a batch run as three launches, so that each kernel's register
-// allocation is that of its own tiles.
+// tiles, the Laplace matrix-core tiles (whose exact-recompute path needs more
+// registers) and the symmetric row tiles of a batch run as four launches, so
+// that each kernel's register allocation is that of its own tiles.
 // Row tiles at k <= 8 wait on their gathered source chunks: 4 waves per SIMD
 // (128 VGPRs, a few spilled at k = 8) hide more of it than 3 without spills
 // (list_pool k = 8: 27.2 -> 24.9 ms); 5 at k = 3 (96 VGPRs, 33 spilled) lost
 // (17.1 -> 17.6 ms).
 template <int K, int MF>
-__global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void list_kernel(
+__global__ __launch_bounds__(256, ((MF == 0 || MF == 3) && K > 0 && K <= 8) ? 4 : 1) void list_kernel(
     ListArgs L) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   __shared__ double s_red[4][64];
   double *red = s_red[wave];
-  constexpr int NST = (K > 0 && MF == 0) ? stage_doubles(K > 0 ? K : 1) + kStageSide : 1;
+  constexpr int NST = (K > 0 && (MF == 0 || MF == 3)) ? stage_doubles(K > 0 ? K : 1) + kStageSide : 1;
   __shared__ double s_stage[4][NST];
   double *stage = s_stage[wave];
   __shared__ double2 etab[64];
@@ -458,10 +590,13 @@ __global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void lis
     const int2 tl = L.tiles[t];
     const RowBlock rb = L.rbs[tl.x];
     const DevTask T = L.tasks[rb.task];
-    const int c0 = tl.y * T.bc;
+    // a symmetric task's slices start at the row block (the upper triangle)
+    const int c0 = (MF == 3 ? rb.row0 : 0) + tl.y * T.bc;
     const int c1 = min(T.n, c0 + T.bc);
     double s;
-    if constexpr (MF == 2) {
+    if constexpr (MF == 3) {
+      s = sym_tile<K>(L, T, rb.row0, c0, c1, stage, L.colp + L.coff[t], etab);
+    } else if constexpr (MF == 2) {
       s = run_tile<LAPLACE, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab);
     } else switch (T.type) {
       case GAUSSIAN: s = run_tile<GAUSSIAN, K, MF>(L, T, rb.row0, c0, c1, red, stage, etab); break;
@@ -497,7 +632,7 @@ __global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void lis
     // launch (a device-scope fence and counter here would write back and
     // invalidate this XCD's L2 on gfx950, once per tile).  Lanes past the
     // block's last row write the sink entries R[rsink + lane].
-    if (rb.nsplit == 1) {  // wave-uniform
+    if (MF != 3 && rb.nsplit == 1) {  // wave-uniform; symmetric rows also take column sums
       const int pos = rb.row0 + lane;
       L.R[pos < T.m ? T.a0 + pos : L.rsink + lane] = s;
     } else {
@@ -508,23 +643,55 @@ __global__ __launch_bounds__(256, (MF == 0 && K > 0 && K <= 8) ? 4 : 1) void lis
 
 // After a batch's persistent launches: R of every split row block is the sum
 // of its tiles' partials in slice order (one wave per row block), and the
-// launches' tile counters are reset for the next run.
+// launches' tile counters are reset for the next run.  A row r of a symmetric
+// task (row block J) also takes, before its own slices, the column partials of
+// the row blocks I < J for source r (from the slice of I that holds r), in
+// order of I.
 __global__ __launch_bounds__(256) void list_rowsum_kernel(const RowBlock *__restrict__ rbs, int nrb,
                                                           const DevTask *__restrict__ tasks,
                                                           const double *__restrict__ partial,
-                                                          double *R, unsigned *heads) {
+                                                          double *R, unsigned *heads,
+                                                          const double *__restrict__ colp,
+                                                          const long *__restrict__ coff,
+                                                          const int *__restrict__ symrb) {
   const int lane = threadIdx.x & 63;
   const int b = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (blockIdx.x == 0 && threadIdx.x < 6) heads[threadIdx.x] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < 8) heads[threadIdx.x] = 0u;
   if (b >= nrb) return;
   const RowBlock rb = rbs[b];
-  if (rb.nsplit == 1) return;
   const DevTask T = tasks[rb.task];
+  if (rb.nsplit == 1 && !T.sym) return;
   const int pos = rb.row0 + lane;
   if (pos >= T.m) return;
   double tot = 0.0;
+  if (T.sym) {
+    const int J = rb.row0 >> 6;
+    for (int I = 0; I < J; ++I) {
+      const int sI = (pos - 64 * I) / T.bc;  // the slice of block I holding source pos
+      const long tile = (long)rbs[symrb[T.srb + I]].tile_base + sI;
+      const int cs = sI == 0 ? 64 * I + 64 : 64 * I + sI * T.bc;
+      tot += colp[coff[tile] + (pos - cs)];
+    }
+  }
   for (int sp = 0; sp < rb.nsplit; ++sp) tot += partial[(long)(rb.tile_base + sp) * 64 + lane];
   R[T.a0 + pos] = tot;
+}
+
+// Whether each candidate task's target list equals its source list: one
+// workgroup per candidate, differs[c] = 1 on the first mismatch.
+struct SymCand {
+  long a0, b0;
+  int m, pad;
+};
+__global__ __launch_bounds__(256) void list_symcheck_kernel(const SymCand *__restrict__ cand,
+                                                            const int *__restrict__ amap,
+                                                            const int *__restrict__ bmap,
+                                                            int *differs) {
+  const SymCand c = cand[blockIdx.x];
+  int d = 0;
+  for (int i = threadIdx.x; i < c.m; i += 256) d |= amap[c.a0 + i] != bmap[c.b0 + i];
+  d = __syncthreads_or(d);
+  if (threadIdx.x == 0) differs[blockIdx.x] = d;
 }
 
 // The compact source pool of a plan run: position i (task offset b0 + j) gets
@@ -587,8 +754,8 @@ struct ListPlan {
   char *buf = nullptr;
   // layout part (tiles, row blocks, tasks, partials, counters, R) of the row-tile
   // launch [0], the matrix-core launch [1] and the Laplace matrix-core launch [2]
-  ListArgs A[3]{};
-  int grid[3] = {0, 0, 0};
+  ListArgs A[4]{};
+  int grid[4] = {0, 0, 0, 0};
   int nrb = 0;                                // row blocks of the persistent launches
   const int *keys = nullptr, *pos = nullptr;  // umap sorted, with positions
   std::vector<TaskDesc> mixed;                // tasks the mixed kernels run
@@ -603,12 +770,63 @@ struct ListPlan {
   double *cX = nullptr, *cX2 = nullptr, *cw = nullptr;
   int *cid = nullptr;
   const int *umap = nullptr, *amap = nullptr, *bmap = nullptr, *wmap = nullptr;
+  // symmetric tasks (list_plan_build's sym_mode): how many, and the same batch
+  // laid out without symmetry for runs whose target and source tables differ
+  int nsym = 0;
+  ListPlan *alt = nullptr;
 };
+
+// The tasks whose target list is their source list, read on the device:
+// candidates are tasks of m == n >= 128 (two row blocks or more) and, for the
+// variable bandwidth, hi == hj.  Returns the flags (1 = symmetric).
+static int find_symmetric(const std::vector<TaskDesc> &tasks, const int *amap, const int *bmap,
+                          hipStream_t s, std::vector<char> &sym) {
+  sym.assign(tasks.size(), 0);
+  std::vector<SymCand> cand;
+  std::vector<int> which;
+  for (size_t t = 0; t < tasks.size(); ++t) {
+    const TaskDesc &td = tasks[t];
+    if (td.m != td.n || td.m < 128) continue;
+    if (td.type == KS_GAUSSIAN_VAR_BANDWIDTH && td.hi != td.hj) continue;
+    cand.push_back(SymCand{td.a0, td.b0, td.m, 0});
+    which.push_back((int)t);
+  }
+  if (cand.empty()) return KS_OK;
+  SymCand *dc = nullptr;
+  int *dd = nullptr;
+  const size_t cb = cand.size() * sizeof(SymCand), db = cand.size() * sizeof(int);
+  if (hipMalloc((void **)&dc, cb + db) != hipSuccess) {
+    (void)hipGetLastError();
+    return KS_ERR_ALLOC;
+  }
+  dd = reinterpret_cast<int *>(reinterpret_cast<char *>(dc) + cb);
+  std::vector<int> differs(cand.size(), 1);
+  int rc = KS_OK;
+  if (hipMemcpyAsync(dc, cand.data(), cb, hipMemcpyHostToDevice, s) != hipSuccess) rc = KS_ERR_DEVICE;
+  if (rc == KS_OK) {
+    hipLaunchKernelGGL(list_symcheck_kernel, dim3((unsigned)cand.size()), dim3(256), 0, s, dc, amap,
+                       bmap, dd);
+    if (hipMemcpyAsync(differs.data(), dd, db, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      rc = KS_ERR_DEVICE;
+  }
+  (void)hipFree(dc);
+  if (rc != KS_OK) return rc;
+  for (size_t c = 0; c < cand.size(); ++c) sym[which[c]] = differs[c] == 0;
+  return KS_OK;
+}
 
 int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
                     const int *umap, const int *amap, const int *bmap, const int *wmap,
-                    long total_a, int precision, const TileChoice *force, hipStream_t s) {
+                    long total_a, int precision, const TileChoice *force, hipStream_t s,
+                    int sym_mode) {
   *out = nullptr;
+  // symmetric tasks take row tiles with LDS staging: k in {3, 8} or 9..64
+  std::vector<char> symf(tasks.size(), 0);
+  if (sym_mode != 0 && (k == 3 || k == 8 || (k >= 9 && k <= 64))) {
+    const int rc = find_symmetric(tasks, amap, bmap, s, symf);
+    if (rc != KS_OK) return rc;
+  }
   ListPlan *pl = new ListPlan();
   pl->k = k;
   pl->precision = precision;
@@ -628,7 +846,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   pl->compact = k < 8 && !varbw && pl->total_b > 0;
   for (size_t t = 0; t < tasks.size(); ++t) {
     const TaskDesc &td = tasks[t];
-    dt[t] = DevTask{td.type, td.m, td.n, 1024, 0, 0, td.a0, td.b0, td.P, td.hi, td.hj};
+    dt[t] = DevTask{td.type, td.m, td.n, 1024, 0, 0, 0, td.a0, td.b0, td.P, td.hi, td.hj};
     if (td.m == 0 || td.n == 0) continue;
     if (precision != KS_PREC_FP64 && (td.type == KS_GAUSSIAN || td.type == KS_GAUSSIAN_VAR_BANDWIDTH) &&
         k >= kMixedListMinK && td.m >= kMixedListMinSide && td.n >= kMixedListMinSide) {
@@ -637,6 +855,16 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
     }
     const TileChoice tc = force ? *force : list_tile_choice(td.type, k, KS_PREC_FP64);
     dt[t].bc = tc.slice;
+    if (symf[t]) {
+      // symmetric: row tiles over the upper triangle, each row block's slices
+      // starting at its own rows (slices are multiples of 64 sources)
+      dt[t].sym = 1;
+      dt[t].tile = 3;
+      pl->nsym += 1;
+      for (int r0 = 0; r0 < td.m; r0 += 64)
+        rbs.push_back(RowBlock{(int)t, r0, (td.n - r0 + dt[t].bc - 1) / dt[t].bc, 0});
+      continue;
+    }
     dt[t].tile = (tc.tile == 1 && list_mfma_eligible(td.type, k))
                      ? (td.type == KS_LAPLACE ? 2 : 1)
                      : 0;
@@ -644,14 +872,14 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
     for (int r0 = 0; r0 < td.m; r0 += 64) rbs.push_back(RowBlock{(int)t, r0, nsplit, 0});
   }
   // row blocks of row tiles first, then those of matrix-core tiles, then those of
-  // Laplace matrix-core tiles; within each kind by work (columns per tile x
-  // rows) descending; then lay tiles out
+  // Laplace matrix-core tiles, then symmetric row tiles; within each kind by
+  // work (columns per tile x rows) descending; then lay tiles out
   std::vector<int> order(rbs.size());
   std::iota(order.begin(), order.end(), 0);
   auto work = [&](const RowBlock &r) {
     const DevTask &T = dt[r.task];
     long rows = std::min(64, T.m - r.row0);
-    long cols = std::min(T.bc, T.n);
+    long cols = std::min(T.bc, T.sym ? T.n - r.row0 : T.n);
     return rows * cols;
   };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -663,11 +891,35 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   for (size_t i = 0; i < order.size(); ++i) rbs_sorted[i] = rbs[order[i]];
   rbs.swap(rbs_sorted);
   std::vector<int2> tiles;
-  long nkind[3] = {0, 0, 0};
+  long nkind[4] = {0, 0, 0, 0};
   for (size_t i = 0; i < rbs.size(); ++i) {
     rbs[i].tile_base = (int)tiles.size();
     for (int sp = 0; sp < rbs[i].nsplit; ++sp) tiles.push_back(make_int2((int)i, sp));
     nkind[dt[rbs[i].task].tile] += rbs[i].nsplit;
+  }
+  // symmetric tasks: each task's row blocks in row order, and where each tile's
+  // column partials go (its sources from max(c0, row0 + 64) to c1)
+  std::vector<int> symrb;
+  std::vector<long> coff;
+  long ncolp = 0;
+  if (pl->nsym > 0) {
+    for (size_t t = 0; t < dt.size(); ++t)
+      if (dt[t].sym) {
+        dt[t].srb = (int)symrb.size();
+        symrb.resize(symrb.size() + (dt[t].m + 63) / 64, -1);
+      }
+    coff.assign(tiles.size(), 0);
+    for (size_t i = 0; i < rbs.size(); ++i) {
+      const DevTask &T = dt[rbs[i].task];
+      if (!T.sym) continue;
+      symrb[T.srb + rbs[i].row0 / 64] = (int)i;
+      for (int sp = 0; sp < rbs[i].nsplit; ++sp) {
+        const int c0 = rbs[i].row0 + sp * T.bc, c1 = std::min(T.n, c0 + T.bc);
+        const int cs = std::max(c0, rbs[i].row0 + 64);
+        coff[rbs[i].tile_base + sp] = ncolp;
+        ncolp += std::max(0, c1 - cs);
+      }
+    }
   }
   pl->ntiles = (long)tiles.size();
   pl->nrb = (int)rbs.size();
@@ -691,7 +943,7 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   size_t off_tasks = off_rbs + al(rbs.size() * sizeof(RowBlock));
   size_t off_part = off_tasks + al(dt.size() * sizeof(DevTask));
   size_t off_head = off_part + al((size_t)pl->ntiles * 64 * sizeof(double));
-  size_t off_R = off_head + 256;  // the three launches' tile counters (two words each)
+  size_t off_R = off_head + 256;  // the four launches' tile counters (two words each)
   size_t off_iota = off_R + al((size_t)(total_a + 64) * sizeof(double));  // + 64 sink entries
   size_t off_kout = off_iota + al((size_t)total_a * sizeof(int));
   size_t off_vout = off_kout + al((size_t)total_a * sizeof(int));
@@ -701,7 +953,10 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   size_t off_cx2 = off_cx + al(cb * k * sizeof(double));
   size_t off_cw = off_cx2 + al(cb * sizeof(double));
   size_t off_cid = off_cw + al(cb * sizeof(double));
-  size_t bytes = off_cid + al(cb * sizeof(int));
+  size_t off_colp = off_cid + al(cb * sizeof(int));
+  size_t off_coff = off_colp + al((size_t)ncolp * sizeof(double));
+  size_t off_symrb = off_coff + al(coff.size() * sizeof(long));
+  size_t bytes = off_symrb + al(symrb.size() * sizeof(int));
   char *buf = nullptr;
   if (hipMalloc((void **)&buf, bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -718,9 +973,15 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
       hipMemcpyAsync(buf + off_tasks, dt.data(), dt.size() * sizeof(DevTask), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemsetAsync(buf + off_head, 0, off_iota - off_head, s) != hipSuccess)
     rc = KS_ERR_DEVICE;
-  for (int kind = 0; kind < 3; ++kind) {
+  if (rc == KS_OK && pl->nsym > 0 &&
+      (hipMemcpyAsync(buf + off_coff, coff.data(), coff.size() * sizeof(long), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipMemcpyAsync(buf + off_symrb, symrb.data(), symrb.size() * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess))
+    rc = KS_ERR_DEVICE;
+  for (int kind = 0; kind < 4; ++kind) {
     ListArgs &A = pl->A[kind];
-    A.tile0 = (int)(kind == 0 ? 0 : kind == 1 ? nkind[0] : nkind[0] + nkind[1]);
+    long t0 = 0;
+    for (int q = 0; q < kind; ++q) t0 += nkind[q];
+    A.tile0 = (int)t0;
     A.ntiles = (int)nkind[kind];
     A.k = k;
     A.tiles = (const int2 *)(buf + off_tiles);
@@ -730,6 +991,9 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
     A.head = (unsigned *)(buf + off_head) + 2 * kind;
     A.R = (double *)(buf + off_R);
     A.rsink = total_a;
+    A.colp = pl->nsym > 0 ? (double *)(buf + off_colp) : nullptr;
+    A.coff = pl->nsym > 0 ? (const long *)(buf + off_coff) : nullptr;
+    A.symrb = pl->nsym > 0 ? (const int *)(buf + off_symrb) : nullptr;
   }
   // merge order: target ids stably sorted with their positions
   int *iota = (int *)(buf + off_iota);
@@ -759,10 +1023,14 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
   int dev = 0, ncu = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  for (int kind = 0; kind < 3; ++kind) {
+  for (int kind = 0; kind < 4; ++kind) {
     long grid = std::min<long>((long)ncu * 4, (nkind[kind] + 3) / 4);
     pl->grid[kind] = (int)std::max<long>(grid, 1);
   }
+  // sym_mode 1: the target and source tables of a run are not known yet; the
+  // same batch without symmetry serves runs where they differ
+  if (rc == KS_OK && sym_mode == 1 && pl->nsym > 0)
+    rc = list_plan_build(&pl->alt, tasks, k, umap, amap, bmap, wmap, total_a, precision, force, s, 0);
   if (rc != KS_OK) {
     list_plan_free(pl);
     return rc;
@@ -778,6 +1046,9 @@ static void launch_list_gather(const ListPlan &pl, const ListArrays &L, hipStrea
 }
 
 int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
+  // a symmetric task's pairs are evaluated once only if the sources are the
+  // targets in the same table
+  if (pl.alt && !(L.XA == L.XB && L.XA2 == L.XB2)) return list_plan_exec(*pl.alt, L, s);
   if (pl.ntiles > 0) {
     const int *bm = pl.bmap, *wm = pl.wmap;
     const double *xb = L.XB, *xb2 = L.XB2, *ww = L.w;
@@ -796,7 +1067,7 @@ int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
       xb2 = pl.cX2;
       ww = pl.cw;
     }
-    for (int kind = 0; kind < 3; ++kind) {
+    for (int kind = 0; kind < 4; ++kind) {
       ListArgs A = pl.A[kind];
       if (A.ntiles == 0) continue;
       A.amap = pl.amap;
@@ -819,6 +1090,12 @@ int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
         if (kk <= 16) launch_list<16, 2>(A, g, s);
         else if (kk <= 32) launch_list<32, 2>(A, g, s);
         else launch_list<64, 2>(A, g, s);
+      } else if (kind == 3) {  // symmetric row tiles: k in {3, 8} or 9..64
+        if (kk == 3) launch_list<3, 3>(A, g, s);
+        else if (kk == 8) launch_list<8, 3>(A, g, s);
+        else if (kk <= 16) launch_list<16, 3>(A, g, s);
+        else if (kk <= 32) launch_list<32, 3>(A, g, s);
+        else launch_list<64, 3>(A, g, s);
       } else if (kk == 3) launch_list<3, 0>(A, g, s);
       else if (kk == 8) launch_list<8, 0>(A, g, s);
       else if (kk >= 9 && kk <= 16) launch_list<16, 0>(A, g, s);
@@ -828,7 +1105,8 @@ int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
     }
     const double *R = pl.A[0].R;
     hipLaunchKernelGGL(list_rowsum_kernel, dim3((unsigned)((pl.nrb + 3) / 4)), dim3(256), 0, s,
-                       pl.A[0].rbs, pl.nrb, pl.A[0].tasks, pl.A[0].partial, pl.A[0].R, pl.A[0].head);
+                       pl.A[0].rbs, pl.nrb, pl.A[0].tasks, pl.A[0].partial, pl.A[0].R, pl.A[0].head,
+                       pl.A[0].colp, pl.A[0].coff, pl.A[0].symrb);
     const long nb = (pl.total_a + 255) / 256;
     hipLaunchKernelGGL(merge_kernel, dim3((unsigned)nb), dim3(256), 0, s, pl.keys, pl.pos, R,
                        pl.total_a, L.u);
@@ -848,10 +1126,12 @@ int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
 
 void list_plan_free(ListPlan *pl) {
   if (!pl) return;
+  list_plan_free(pl->alt);
   if (pl->buf) (void)hipFree(pl->buf);
   delete pl;
 }
 
 long list_plan_tiles(const ListPlan *pl) { return pl ? pl->ntiles : 0; }
+int list_plan_symmetric_tasks(const ListPlan *pl) { return pl ? pl->nsym : 0; }
 
 }  // namespace gsks

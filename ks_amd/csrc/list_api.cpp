@@ -181,8 +181,11 @@ int list_run(int ntask, const ks_t *kernels, int k, double *u, long nu, const lo
   }
   if (rc == KS_OK) {
     ListPlan *pl = nullptr;
+    // one table for targets and sources: tasks whose lists are one set evaluate
+    // each pair once
+    const int sym_mode = (L.XA == L.XB && L.XA2 == L.XB2) ? 2 : 0;
     rc = list_plan_build(&pl, tasks, k, L.umap, L.amap, L.bmap, L.wmap, total_a, precision,
-                         nullptr, s);
+                         nullptr, s, sym_mode);
     if (rc == KS_OK) rc = list_plan_exec(*pl, L, s);
     if (pl) {
       (void)hipStreamSynchronize(s);
@@ -271,7 +274,7 @@ extern "C" int ks_list_plan_create(ks_list_plan **plan, int ntask, const ks_t *k
   ks_list_plan *h = new ks_list_plan();
   h->k = k;
   int rc = total_a > 0 ? list_plan_build(&h->pl, tasks, k, umap, amap, bmap, wmap, total_a,
-                                         precision, nullptr, s)
+                                         precision, nullptr, s, 1)
                        : KS_OK;
   if (rc != KS_OK) {
     delete h;
@@ -303,6 +306,10 @@ extern "C" int ks_list_plan_execute(const ks_list_plan *plan, double *u, const d
   int rc = list_plan_exec(*plan->pl, L, (hipStream_t)stream);
   if (rc != KS_OK) return raise_error(rc, rc == KS_ERR_DEVICE, "%s(): %s\n", name, err_string(rc));
   return KS_OK;
+}
+
+extern "C" int ks_list_plan_symmetric_tasks(const ks_list_plan *plan) {
+  return plan ? gsks::list_plan_symmetric_tasks(plan->pl) : 0;
 }
 
 extern "C" void ks_list_plan_destroy(ks_list_plan *plan) {

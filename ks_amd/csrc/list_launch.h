@@ -51,13 +51,18 @@ constexpr int kMixedListMinK = 16;
 // (tile layout, descriptors, the merge order of umap) and synchronises; running
 // it only enqueues kernels, so a run can be captured into a HIP graph.  The
 // maps are device arrays that must outlive the plan.
+// sym_mode: 0 evaluates every task in full; 2 evaluates each pair of a task
+// whose target list is its source list once (the caller's target and source
+// tables are one: XA == XB, XA2 == XB2); 1 does so when a run's tables are one
+// and keeps the full layout beside it for runs where they are not.
 struct ListPlan;
 int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k, const int *umap,
                     const int *amap, const int *bmap, const int *wmap, long total_a,
-                    int precision, const TileChoice *force, hipStream_t s);
+                    int precision, const TileChoice *force, hipStream_t s, int sym_mode);
 int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s);
 void list_plan_free(ListPlan *pl);
 long list_plan_tiles(const ListPlan *pl);
+int list_plan_symmetric_tasks(const ListPlan *pl);
 
 int list_run(int ntask, const ks_t *kernels, int k, double *u, long nu, const long *a_off,
              const int *umap, const int *amap, const long *b_off, const int *bmap,

@@ -190,7 +190,7 @@ int make_synthetic(int type, int k, Synthetic &S) {
 float time_shape(const Synthetic &S, int k, const TileChoice &tc, hipStream_t s) {
   ListPlan *pl = nullptr;
   if (list_plan_build(&pl, S.tasks, k, S.umap, S.amap, S.bmap, S.wmap, S.total_a, KS_PREC_FP64,
-                      &tc, s) != KS_OK)
+                      &tc, s, 0) != KS_OK)
     return -1.0f;
   ListArrays L;
   L.u = S.u;

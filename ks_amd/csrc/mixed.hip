@@ -706,25 +706,6 @@ __device__ __forceinline__ f4 cross_block(const Frag<PREC, NC> &a, const Frag<PR
 __device__ __forceinline__ int out_row(int g, int v) { return 4 * g + v; }
 __device__ __forceinline__ int f64_a_row(int p16) { return 4 * (p16 & 3) + (p16 >> 2); }
 
-// One DPP lane exchange of a double (two 32-bit moves), all lanes valid.
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double x) {
-  int2 v = __builtin_bit_cast(int2, x);
-  v.x = __builtin_amdgcn_mov_dpp(v.x, CTRL, 0xF, 0xF, true);
-  v.y = __builtin_amdgcn_mov_dpp(v.y, CTRL, 0xF, 0xF, true);
-  return __builtin_bit_cast(double, v);
-}
-
-// Sum over the 16 lanes of a DPP row; every lane gets the same value.
-// Butterflies: swap neighbours, swap pairs, mirror the 8-lane halves, mirror the row.
-__device__ __forceinline__ double row_sum16(double x) {
-  x += dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
-  x += dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
-  x += dpp_f64<0x141>(x);  // row_half_mirror
-  x += dpp_f64<0x140>(x);  // row_mirror
-  return x;
-}
-
 constexpr int kFlush = 8;  // far-pair FP32 partials go to FP64 every 8 source blocks
 
 

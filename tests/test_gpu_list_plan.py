@@ -492,3 +492,135 @@ def test_mixed_plan_capture(L, torch_cuda, oracle):
         assert err <= 1e-6
     finally:
         plan.close()
+
+
+def _sym_batch(k, seed, pool=12000, ntask=18):
+    """Tasks whose source list is their target list (m == n, 128 <= m <= 2600:
+    every kernel type, variable bandwidth with hi == hj), a task of the same
+    size whose sources are its targets reordered (not symmetric), and plain
+    tasks; umap and wmap are other permutations of the pool."""
+    rng = np.random.default_rng(seed)
+    X = rng.random((pool, k)) * (1.5 / np.sqrt(k))
+    X2 = np.zeros(pool)
+    for p in range(k):
+        X2 += X[:, p] * X[:, p]
+    w = rng.standard_normal(pool)
+    h = rng.uniform(1.5, 3.0, pool)  # 1/h of the variable-bandwidth form
+    kinds = [dict(type=0, scal=-1.5), dict(type=2), dict(type=1, scal=1.0 / k, cons=1.0, powe=2.0),
+             dict(type=3), dict(type=4, scal=0.3, cons=0.1), dict(type=5), dict(type=6, cons=0.5),
+             dict(type=7), dict(type=1, scal=0.5 / k, cons=0.5, powe=3.0)]
+    amaps, bmaps, kernels_py, nsym = [], [], [], 0
+    for t in range(ntask):
+        kd = dict(kinds[t % len(kinds)])
+        m = int(rng.integers(128, 2600))
+        a = rng.choice(pool, m, replace=False).astype(np.int32)
+        if t % 6 == 5:  # same set, other order: evaluated in full
+            b = rng.permutation(a).astype(np.int32)
+        elif t % 6 == 4:  # plain
+            b = rng.choice(pool, int(rng.integers(1, 3000)), replace=False).astype(np.int32)
+        else:
+            b = a.copy()
+            nsym += 1
+        if kd["type"] == 2 and k < 3:
+            kd = dict(type=0, scal=-0.7)
+        amaps.append(a)
+        bmaps.append(b)
+        kernels_py.append(kd)
+    ms = [len(a) for a in amaps]
+    ns = [len(b) for b in bmaps]
+    amap, bmap = np.concatenate(amaps), np.concatenate(bmaps)
+    pu = rng.permutation(pool).astype(np.int32)
+    pw = rng.permutation(pool).astype(np.int32)
+    return dict(k=k, X=X, X2=X2, w=w, h=h, amap=amap, bmap=bmap, umap=pu[amap], wmap=pw[bmap],
+                a_off=np.concatenate([[0], np.cumsum(ms)]).astype(np.int64),
+                b_off=np.concatenate([[0], np.cumsum(ns)]).astype(np.int64),
+                kernels_py=kernels_py, ntask=ntask, pool=pool, nsym=nsym)
+
+
+def _kernel_array(B, hptr):
+    from ks_amd import _lib
+    arr = (_lib.KsT * B["ntask"])()
+    for t, kd in enumerate(B["kernels_py"]):
+        arr[t].type = kd["type"]
+        arr[t].scal = kd.get("scal", 0.0)
+        arr[t].cons = kd.get("cons", 0.0)
+        arr[t].powe = kd.get("powe", 0.0)
+        if kd["type"] == 3:
+            arr[t].hi = arr[t].hj = hptr
+    return arr
+
+
+@pytest.mark.parametrize("k", [3, 8, 20, 40])
+def test_symmetric_list_tasks(L, torch_cuda, oracle, k):
+    """Tasks whose source list is their target list evaluate each pair of
+    different row blocks once (ks_list_plan_symmetric_tasks counts them): a plan
+    run with one table for targets and sources, its HIP graph replays, a run
+    with a separate copy of the table (the full layout the plan keeps), and the
+    one-shot dgsks_list on host arrays all match the oracle to 1e-12."""
+    torch = torch_cuda
+    B = _sym_batch(k, 900 + k)
+    kpy = [dict(kd, hi=B["h"], hj=B["h"]) if kd["type"] == 3 else kd for kd in B["kernels_py"]]
+    u0 = np.random.default_rng(2).standard_normal(B["pool"])
+    ref = u0.copy()
+    oracle.dgsks_list(kpy, k, ref, B["a_off"], B["umap"], B["amap"], B["b_off"], B["bmap"],
+                      B["wmap"], B["X"], B["X2"], B["X"], B["X2"], B["w"])
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    H = lambda a: ctypes.c_void_p(a.ctypes.data)  # noqa: E731
+    dh = dev(B["h"])
+    kern = _kernel_array(B, dh.data_ptr())
+    amap, bmap, umap, wmap = map(dev, (B["amap"], B["bmap"], B["umap"], B["wmap"]))
+    X, X2, w = dev(B["X"]), dev(B["X2"]), dev(B["w"])
+    Xc, X2c = X.clone(), X2.clone()
+    h = ctypes.c_void_p()
+    assert L.ks_list_plan_create(ctypes.byref(h), B["ntask"], kern, k, H(B["a_off"]), P(umap),
+                                 P(amap), H(B["b_off"]), P(bmap), P(wmap), 0) == 0
+    try:
+        nsym = L.ks_list_plan_symmetric_tasks(h)
+        assert nsym == B["nsym"], (nsym, B["nsym"])
+        s = torch.cuda.Stream()
+
+        def run(xb, xb2):
+            return L.ks_list_plan_execute(h, P(u), P(X), P(X2), P(xb), P(xb2), P(w),
+                                          ctypes.c_void_p(s.cuda_stream))
+        u = dev(u0.copy())
+        with torch.cuda.stream(s):
+            assert run(X, X2) == 0
+        torch.cuda.synchronize()
+        got = u.cpu().numpy()
+        err = oracle.rel_l2(got, ref)
+        print(f"symmetric list k={k} ({nsym} symmetric tasks): rel L2 = {err:.3e}")
+        assert err <= TOL
+        # replays of a captured run equal the eager run
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            assert run(X, X2) == 0
+        torch.cuda.synchronize()
+        u0d = dev(u0)
+        for r in range(3):
+            u.copy_(u0d)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert np.array_equal(u.cpu().numpy(), got), f"replay {r} differs"
+        del g
+        # another table for the sources (same values): the full layout
+        u.copy_(u0d)
+        with torch.cuda.stream(s):
+            assert run(Xc, X2c) == 0
+        torch.cuda.synchronize()
+        err_full = oracle.rel_l2(u.cpu().numpy(), ref)
+        print(f"  full layout: rel L2 = {err_full:.3e}")
+        assert err_full <= TOL
+    finally:
+        L.ks_list_plan_destroy(h)
+    # the one-shot call on host arrays, one table
+    kern_h = _kernel_array(B, B["h"].ctypes.data)
+    u1 = u0.copy()
+    rc = L.dgsks_list(B["ntask"], kern_h, k, H(u1), u1.size, H(B["a_off"]), H(B["umap"]),
+                      H(B["amap"]), H(B["b_off"]), H(B["bmap"]), H(B["wmap"]), H(B["X"]),
+                      H(B["X2"]), H(B["X"]), H(B["X2"]), H(B["w"]), 0)
+    assert rc == 0
+    err1 = oracle.rel_l2(u1, ref)
+    print(f"  dgsks_list: rel L2 = {err1:.3e}")
+    assert err1 <= TOL
