@@ -22,6 +22,8 @@ CPU baseline:
   nbody_k3:<type>        m = n = 262144, k = 3, permutation maps, four kernels
   mixture_k64_varbw      variable-bandwidth Gaussian, mixed beside FP64
   list_pool_k{3,8,32}    4096 tasks over a 1,048,576-point pool, FP64 and mixed
+  list_sym_k3            the same with every task's sources its targets: symmetric
+                         layout beside the full one
   grad_gauss16           forward + gradient step, m = n = 32768, k = 16
 """
 from __future__ import annotations
@@ -81,6 +83,13 @@ def build_workload(name):
     elif name == "uniform_gauss16":
         d = W.uniform_gauss16()
         d["label"] = "uniform_gauss16 gaussian h=0.5 m=n=8192 k=16"
+    elif name.startswith("list_task_gauss_k"):
+        # the largest Gaussian task of the batched workload alone: m = n = 8192
+        # points uniform in [0,1)^k, h = 0.2, the mixed form beside FP64
+        k = int(name.split("_k")[1])
+        d = W.uniform_gauss16(k=k, seed=4)
+        d["kernel"] = dict(type=W.GAUSSIAN, scal=W.gaussian_scal(0.2))
+        d["label"] = f"list task gaussian h=0.2 m=n=8192 k={k} uniform [0,1)^k"
     elif name.startswith("nbody_k3"):
         kind = name.split(":")[1] if ":" in name else "gaussian"
         d = W.nbody_k3()
@@ -90,10 +99,14 @@ def build_workload(name):
         d = W.grad_gauss16()
         d["label"] = ("grad_gauss16 gaussian h=0.4 m=n=32768 k=16: forward u = K w and the "
                       "gradient of g.u in dXA, dXB, dw, dh")
-    elif name.startswith("list_pool"):
+    elif name.startswith("list_pool") or name.startswith("list_sym"):
         k = int(name.split("_k")[1]) if "_k" in name else 3
-        d = W.list_pool(k=k)
-        d["label"] = (f"list_pool 4096 tasks m,n log-U[64,8192] k={k} pool=1048576 "
+        sym = name.startswith("list_sym")
+        d = W.list_pool(k=k, symmetric=sym)
+        d["symmetric"] = sym
+        d["label"] = (f"list_sym 4096 tasks, sources = targets, m = n log-U[64,8192] k={k} "
+                      "pool=1048576 kernels {gaussian h=0.2, laplace, polynomial deg 2}" if sym else
+                      f"list_pool 4096 tasks m,n log-U[64,8192] k={k} pool=1048576 "
                       "kernels {gaussian h=0.2, laplace, polynomial deg 2}")
     else:
         raise SystemExit(f"unknown workload {name}")
@@ -320,7 +333,10 @@ def run_list(args, L, name, precision, steps, warmup, with_cpu):
     descriptors), then one plan run per step with the pool resident in HBM,
     timed with HIP events on the run's stream.  The one-shot dgsks_list wall
     time (layout + upload + run + synchronisation) is reported beside it, and
-    for the mixed form its relative L2 error against the FP64 plan's result."""
+    for the mixed form its relative L2 error against the FP64 plan's result.
+    For the symmetric batch (every task's sources are its targets) the same plan
+    is also timed with a copy of the point table as the sources, which runs the
+    full layout: the difference is what evaluating each pair once saves."""
     import torch
     from ks_amd import _lib
     t_start = time.perf_counter()
@@ -364,6 +380,30 @@ def run_list(args, L, name, precision, steps, warmup, with_cpu):
     plan_ms = (time.perf_counter() - t0) * 1e3
     ms_per_step, kernel_ms = _events_timed(stream, lambda: run_plan(plan), steps, warmup)
     extra = {}
+    sym = bool(d.get("symmetric"))
+    if sym:
+        nsym = L.ks_list_plan_symmetric_tasks(plan)
+        u.zero_()
+        run_plan(plan)
+        torch.cuda.synchronize()
+        u_sym = u.clone()
+        Xc, X2c = X.clone(), X2.clone()
+
+        def run_full():
+            rc = L.ks_list_plan_execute(plan, P(u), P(X), P(X2), P(Xc), P(X2c), P(w), sp)
+            if rc != 0:
+                raise RuntimeError(f"ks_list_plan_execute failed: {_lib.error_string(rc)}")
+        full_ms, full_kms = _events_timed(stream, run_full, steps, warmup)
+        u.zero_()
+        run_full()
+        torch.cuda.synchronize()
+        extra["symmetric"] = {
+            "symmetric_tasks": nsym,
+            "full_layout_ms_per_step": full_ms, "full_layout_kernel_ms": full_kms,
+            "speedup_vs_full_layout": full_ms / ms_per_step,
+            "rel_diff_vs_full_layout": float(torch.linalg.norm(u - u_sym) /
+                                             torch.linalg.norm(u_sym))}
+        del Xc, X2c
     if prec != 0:
         # the mixed batch's result against the FP64 batch's, from u = 0
         p64 = make_plan(0)
@@ -399,7 +439,13 @@ def run_list(args, L, name, precision, steps, warmup, with_cpu):
     n_mixed = 0 if prec == 0 else int(np.sum([
         t in (0, 3) and k >= 16 and ma >= 16384 and nb >= 16384
         for t, ma, nb in zip(types, np.diff(a_off), np.diff(b_off))]))
-    per = float(np.sum([(2 * k + FLOP_C[t]) * float(ma) * float(nb) for t, ma, nb in
+    def evaluated(ma, nb):
+        # a symmetric task (m = n >= 128 at k in {3, 8} or 9..64) evaluates row
+        # block I (64 rows) against sources 64 I .. n only
+        if not (sym and ma == nb and ma >= 128 and (k in (3, 8) or 9 <= k <= 64)):
+            return float(ma) * float(nb)
+        return float(sum(min(64, ma - r0) * (nb - r0) for r0 in range(0, int(ma), 64)))
+    per = float(np.sum([(2 * k + FLOP_C[t]) * evaluated(ma, nb) for t, ma, nb in
                         zip(types, np.diff(a_off), np.diff(b_off))]))
     achieved = per / (kernel_ms / 1e3) / 1e12
     out = {
@@ -414,6 +460,8 @@ def run_list(args, L, name, precision, steps, warmup, with_cpu):
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "traffic_source": None,
                      "flop_model": "2k + c_type per evaluation, test/test_dgsks.c:254-283",
+                     "evaluated_pairs": float(np.sum([evaluated(ma, nb) for ma, nb in
+                                                      zip(np.diff(a_off), np.diff(b_off))])),
                      "kernel_ms": kernel_ms,
                      "kernel": "gsks::list_kernel (+ list_rowsum_kernel)"},
         "plan_create_ms": plan_ms,
@@ -642,7 +690,10 @@ def run_single(args, L, name, precision, steps, warmup, with_cpu, cpu_s, graph=F
     want = precision
     if want == "auto":
         # the metric compares the two forms on the clustered k = 64 workloads
-        want = "mixed_bf16x3" if (mixed_ok and name.startswith("mixture")) else "fp64"
+        # (and on the batched workload's Gaussian task shape)
+        want = ("mixed_bf16x3" if (mixed_ok and (name.startswith("mixture") or
+                                                 name.startswith("list_task")))
+                else "fp64")
     prec = {"fp64": 0, "mixed_fp32": 1, "mixed_bf16x3": 2}[want]
     extra = {}
     if prec != 0:
@@ -758,6 +809,7 @@ def suite(args, L):
                                                       warm, cpu))
         guard(f"list_pool_k{k}_mixed", lambda k=k: run_list(
             args, L, f"list_pool_k{k}", "mixed_bf16x3", steps, warm, False))
+    guard("list_sym_k3", lambda: run_list(args, L, "list_sym_k3", "fp64", steps, warm, cpu))
     guard("grad_gauss16", lambda: run_grad(args, "grad_gauss16", "auto", steps, warm, cpu))
     return subs
 
@@ -806,7 +858,7 @@ def main():
     cpu = not args.no_cpu_baseline
     name = args.workload
     if name is not None:
-        if name.startswith("list_pool"):
+        if name.startswith("list_pool") or name.startswith("list_sym"):
             out = run_list(args, L, name, args.precision, args.steps, args.warmup, cpu)
         elif name == "grad_gauss16":
             out = run_grad(args, name, args.precision, args.steps, args.warmup, cpu)

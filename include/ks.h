@@ -314,11 +314,12 @@ void ks_list_plan_destroy(ks_list_plan *plan);
  * omp_dgsks_list_symmetric shares the point table but evaluates every pair,
  * frame/omp_dgsks_list.cpp:114-166).  A task whose target list is its source
  * list (m == n >= 128, amap[a_off[t] + i] == bmap[b_off[t] + i] for every i;
- * for the variable bandwidth also hi == hj), at k in {3, 8} or 9 <= k <= 64,
- * evaluates each pair of its different 64-row blocks once for both terms in
- * runs whose target and source tables are one (XA == XB and XA2 == XB2);
- * umap and wmap may be anything.  Other runs use the full layout the plan
- * keeps beside it.  dgsks_list does the same when its XA == XB and
+ * for the variable bandwidth also hi == hj), at k = 3 or 8 (where the vector
+ * units evaluate the pairs; at larger k the matrix-core layout of every pair
+ * is faster), evaluates each pair of its different 64-row blocks once for
+ * both terms in runs whose target and source tables are one (XA == XB and
+ * XA2 == XB2); umap and wmap may be anything.  Other runs use the full layout
+ * the plan keeps beside it.  dgsks_list does the same when its XA == XB and
  * XA2 == XB2.  Returns how many tasks of the plan are symmetric. */
 int ks_list_plan_symmetric_tasks(const ks_list_plan *plan);
 

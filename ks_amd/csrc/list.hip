@@ -821,9 +821,12 @@ int list_plan_build(ListPlan **out, const std::vector<TaskDesc> &tasks, int k,
                     long total_a, int precision, const TileChoice *force, hipStream_t s,
                     int sym_mode) {
   *out = nullptr;
-  // symmetric tasks take row tiles with LDS staging: k in {3, 8} or 9..64
+  // symmetric tasks take row tiles with LDS staging, at k in {3, 8}: there they
+  // beat the full layout (list_sym k = 3: 19.4 -> 17.4 ms, k = 8: 36.2 -> 26.6
+  // ms); at k >= 9 the full layout's matrix-core tiles win (k = 32: 61 ms
+  // against 279 ms for symmetric row tiles)
   std::vector<char> symf(tasks.size(), 0);
-  if (sym_mode != 0 && (k == 3 || k == 8 || (k >= 9 && k <= 64))) {
+  if (sym_mode != 0 && (k == 3 || k == 8)) {
     const int rc = find_symmetric(tasks, amap, bmap, s, symf);
     if (rc != KS_OK) return rc;
   }
@@ -1090,12 +1093,9 @@ int list_plan_exec(const ListPlan &pl, const ListArrays &L, hipStream_t s) {
         if (kk <= 16) launch_list<16, 2>(A, g, s);
         else if (kk <= 32) launch_list<32, 2>(A, g, s);
         else launch_list<64, 2>(A, g, s);
-      } else if (kind == 3) {  // symmetric row tiles: k in {3, 8} or 9..64
+      } else if (kind == 3) {  // symmetric row tiles: k in {3, 8}
         if (kk == 3) launch_list<3, 3>(A, g, s);
-        else if (kk == 8) launch_list<8, 3>(A, g, s);
-        else if (kk <= 16) launch_list<16, 3>(A, g, s);
-        else if (kk <= 32) launch_list<32, 3>(A, g, s);
-        else launch_list<64, 3>(A, g, s);
+        else launch_list<8, 3>(A, g, s);
       } else if (kk == 3) launch_list<3, 0>(A, g, s);
       else if (kk == 8) launch_list<8, 0>(A, g, s);
       else if (kk >= 9 && kk <= 16) launch_list<16, 0>(A, g, s);

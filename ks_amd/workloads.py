@@ -9,7 +9,8 @@
                       h_i ~ U[0.2, 0.4)
     list_pool         4096 tasks over a pool of 1,048,576 points, k in {3, 8, 32},
                       m, n log-uniform in [64, 8192], kernel in
-                      {Gaussian h = 0.2, Laplace, polynomial degree 2}
+                      {Gaussian h = 0.2, Laplace, polynomial degree 2}; with
+                      symmetric=True every task's sources are its targets
     grad_gauss16      m = n = 32768, k = 16, Gaussian h = 0.4, upstream gradient
 
 Squared norms are summed left to right in FP64 like the reference's norm
@@ -78,17 +79,24 @@ LIST_KERNELS = [dict(type=GAUSSIAN, scal=gaussian_scal(0.2)), dict(type=LAPLACE)
                 dict(type=POLYNOMIAL, scal=None, cons=1.0, powe=2.0)]
 
 
-def list_pool(k=3, ntask=4096, pool=1 << 20, lo=64, hi=8192, seed=4):
-    """Tasks as CSR offsets into concatenated maps; umap = amap, wmap = bmap."""
+def list_pool(k=3, ntask=4096, pool=1 << 20, lo=64, hi=8192, seed=4, symmetric=False):
+    """Tasks as CSR offsets into concatenated maps; umap = amap, wmap = bmap.
+    symmetric: every task's source list is its target list (the near-field
+    self-interaction of a leaf: m = n, the same points in the same order)."""
     rng = np.random.default_rng(seed)
     P = rng.random((pool, k))
     w = rng.standard_normal(pool)
     ms = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), ntask))).astype(np.int64)
     ns = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), ntask))).astype(np.int64)
+    if symmetric:
+        ns = ms.copy()
     amaps, bmaps, kernels = [], [], []
     for t in range(ntask):
         amaps.append(rng.choice(pool, int(ms[t]), replace=False).astype(np.int32))
-        bmaps.append(rng.choice(pool, int(ns[t]), replace=False).astype(np.int32))
+        if symmetric:
+            bmaps.append(amaps[-1].copy())
+        else:
+            bmaps.append(rng.choice(pool, int(ns[t]), replace=False).astype(np.int32))
         kd = dict(LIST_KERNELS[int(rng.integers(0, 3))])
         if kd["type"] == POLYNOMIAL:
             kd["scal"] = 1.0 / k

@@ -553,8 +553,9 @@ def _kernel_array(B, hptr):
 @pytest.mark.parametrize("k", [3, 8, 20, 40])
 def test_symmetric_list_tasks(L, torch_cuda, oracle, k):
     """Tasks whose source list is their target list evaluate each pair of
-    different row blocks once (ks_list_plan_symmetric_tasks counts them): a plan
-    run with one table for targets and sources, its HIP graph replays, a run
+    different row blocks once at k = 3 and 8 (ks_list_plan_symmetric_tasks counts
+    them; at k = 20 and 40 none are, the full matrix-core layout being faster): a
+    plan run with one table for targets and sources, its HIP graph replays, a run
     with a separate copy of the table (the full layout the plan keeps), and the
     one-shot dgsks_list on host arrays all match the oracle to 1e-12."""
     torch = torch_cuda
@@ -577,7 +578,7 @@ def test_symmetric_list_tasks(L, torch_cuda, oracle, k):
                                  P(amap), H(B["b_off"]), P(bmap), P(wmap), 0) == 0
     try:
         nsym = L.ks_list_plan_symmetric_tasks(h)
-        assert nsym == B["nsym"], (nsym, B["nsym"])
+        assert nsym == (B["nsym"] if k in (3, 8) else 0), (nsym, B["nsym"])
         s = torch.cuda.Stream()
 
         def run(xb, xb2):
