@@ -269,6 +269,34 @@ int ks_gauss_grad_backward_device(int m, int n, int k, const double *XA, const d
                                   const double *state, double *dXA, double *dXB,
                                   double *dw, double *dh, int precision, void *stream);
 
+/* Gradient of the mapped summation (no counterpart in the reference, whose
+ * summations have no gradient).  For L = sum_x g[x] du[x] with
+ *   du[umap[i]] += sum_j K(XA[amap[i]], XB[bmap[j]]) w[wmap[j]],
+ * the kernel a KS_GAUSSIAN (K = exp(scal d2)) or KS_GAUSSIAN_VAR_BANDWIDTH
+ * (K = exp(-0.5 hi[amap[i]] hj[bmap[j]] d2)) descriptor gives, as
+ * frame/dgsks_ref.c:254-288 evaluates it, with the norms in d2 those of the
+ * points themselves, ADD the gradient into the tables at the mapped entries:
+ *   dXA [.][k] at amap[i], dXB [.][k] at bmap[j], dw at wmap[j],
+ *   *dscal (Gaussian only), dhi at amap[i] and dhj at bmap[j] (variable
+ *   bandwidth only),
+ * a repeated map entry receiving the sum of its rows' terms, in row order (the
+ * result is the same on every call).  Any output may be NULL; outputs may be
+ * one table (dXA == dXB when XA == XB, dhi == dhj when hi == hj): the additions
+ * run one after the other.  Two multi-right-hand-side summations recompute the
+ * kernel matrix (k + 1 right-hand sides each for the Gaussian, k + 2 and k + 3
+ * for the variable bandwidth), on the mapped points translated by their mean.
+ * precision: KS_PREC_FP64 (Gaussian k <= 71, variable bandwidth k <= 69) or
+ * KS_PREC_MIXED_FP32 (Gaussian, k <= 32: dgsks_multirhs_mixed_device's
+ * kernel).  Device pointers (map entries below 2^31), enqueued on `stream`;
+ * temporary buffers from the stream-ordered pool. */
+int ks_grad_device(const ks_t *kernel, int m, int n, int k,
+                   const double *g, const int *umap,
+                   const double *XA, const int *amap,
+                   const double *XB, const int *bmap,
+                   const double *w, const int *wmap,
+                   double *dXA, double *dXB, double *dw,
+                   double *dscal, double *dhi, double *dhj, int precision, void *stream);
+
 /* Batched summation: ntask independent summations over shared point pools,
  * one persistent launch.  Task t owns the ranges [a_off[t], a_off[t+1]) of
  * amap/umap and [b_off[t], b_off[t+1]) of bmap/wmap and uses kernels[t]

@@ -75,6 +75,9 @@ _SIGS = {
     "ks_gauss_grad_backward_device": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                      _vp, _vp, _vp, _vp, ctypes.c_double, _vp,
                                                      _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "ks_grad_device": (ctypes.c_int, [ctypes.POINTER(KsT), ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                      _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "ks_list_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(KsT),
                                            ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                            ctypes.c_int]),

@@ -24,6 +24,11 @@ When XA or h needs a gradient (and grad mode is on), the "fp64" and
 keep V, so the backward runs the second summation only: a training step costs
 two summations, not three.  Otherwise the forward is the plain summation and
 the backward, which then needs no V, runs the second summation only.
+
+MappedSummation covers index maps (XA[amap[i]], XB[bmap[j]], w[wmap[j]]) and
+the variable-bandwidth Gaussian with its per-point bandwidths: forward through
+dgsks_device, backward through ks_grad_device, which adds each row's gradient
+into the tables at the mapped entries.
 """
 from __future__ import annotations
 
@@ -197,3 +202,148 @@ class GaussianSummation(torch.autograd.Function):
 
 def gaussian_summation(XA, XB, w, h, precision="fp64"):
     return GaussianSummation.apply(XA, XB, w, h, precision, torch.is_grad_enabled())
+
+
+# ------------------------------------------------------------ mapped summations
+
+def _i32(t, name):
+    if not (t.is_cuda and t.dtype == torch.int32 and t.dim() == 1):
+        raise AssertionError(f"{name}: int32 CUDA vector expected")
+    return t.contiguous()
+
+
+def mapped_summation_grad(ktype, g, umap, XA, amap, XB, bmap, w, wmap, *, scal=0.0, hi=None,
+                          hj=None, precision="fp64", want=("dXA", "dXB", "dw", "dparam")):
+    """Gradient tables of L = g . du, du[umap[i]] += sum_j K(XA[amap[i]],
+    XB[bmap[j]]) w[wmap[j]] (ks_grad_device), accumulated from zero: a dict
+    with dXA, dXB, dw and dscal (KS_GAUSSIAN, a 1-element tensor) or dhi, dhj
+    (KS_GAUSSIAN_VAR_BANDWIDTH) for the names in `want` ("dparam" is dscal or
+    dhi and dhj)."""
+    L = _lib.lib()
+    var = ktype == _lib.KS_GAUSSIAN_VAR_BANDWIDTH
+    XA, XB, w, g = (t.contiguous() for t in (XA, XB, w, g))
+    umap, amap, bmap, wmap = (_i32(t, nm) for t, nm in
+                              ((umap, "umap"), (amap, "amap"), (bmap, "bmap"), (wmap, "wmap")))
+    k = XA.shape[1]
+    dev = XA.device
+    out = {}
+    if "dXA" in want:
+        out["dXA"] = torch.zeros_like(XA)
+    if "dXB" in want:
+        out["dXB"] = torch.zeros_like(XB)
+    if "dw" in want:
+        out["dw"] = torch.zeros_like(w)
+    kt = KsT()
+    kt.type = ktype
+    keep = []
+    if var:
+        hi, hj = hi.contiguous(), hj.contiguous()
+        keep += [hi, hj]
+        kt.hi, kt.hj = hi.data_ptr(), hj.data_ptr()
+        if "dparam" in want:
+            out["dhi"] = torch.zeros_like(hi)
+            out["dhj"] = torch.zeros_like(hj)
+    else:
+        kt.scal = float(scal)
+        if "dparam" in want:
+            out["dscal"] = torch.zeros(1, dtype=torch.float64, device=dev)
+    ptr = lambda nm: _p(out[nm]) if nm in out else None  # noqa: E731
+    rc = L.ks_grad_device(ctypes.byref(kt), amap.numel(), bmap.numel(), k, _p(g), _p(umap),
+                          _p(XA), _p(amap), _p(XB), _p(bmap), _p(w), _p(wmap), ptr("dXA"),
+                          ptr("dXB"), ptr("dw"), ptr("dscal"), ptr("dhi"), ptr("dhj"),
+                          _prec_code(precision), _stream(dev))
+    _lib.check(rc, "ks_grad_device")
+    return out
+
+
+def mapped_summation_forward(ktype, XA, amap, XB, bmap, w, wmap, *, scal=0.0, hi=None,
+                             hj=None, precision="fp64"):
+    """u_i = sum_j K(XA[amap[i]], XB[bmap[j]]) w[wmap[j]], one row per amap
+    entry (dgsks_device with an identity umap; no autograd)."""
+    L = _lib.lib()
+    XA, XB, w = (t.contiguous() for t in (XA, XB, w))
+    amap, bmap, wmap = (_i32(t, nm) for t, nm in ((amap, "amap"), (bmap, "bmap"), (wmap, "wmap")))
+    m, n, k = amap.numel(), bmap.numel(), XA.shape[1]
+    dev = XA.device
+    s = _stream(dev)
+    XA2 = torch.empty(XA.shape[0], dtype=torch.float64, device=dev)
+    XB2 = torch.empty(XB.shape[0], dtype=torch.float64, device=dev)
+    _lib.check(L.ks_sqnorm_device(_p(XA), XA.shape[0], k, _p(XA2), s), "ks_sqnorm_device")
+    _lib.check(L.ks_sqnorm_device(_p(XB), XB.shape[0], k, _p(XB2), s), "ks_sqnorm_device")
+    iu = torch.arange(m, dtype=torch.int32, device=dev)
+    u = torch.zeros(m, dtype=torch.float64, device=dev)
+    kt = KsT()
+    kt.type = ktype
+    keep = []
+    if ktype == _lib.KS_GAUSSIAN_VAR_BANDWIDTH:
+        hi, hj = hi.contiguous(), hj.contiguous()
+        keep += [hi, hj]
+        kt.hi, kt.hj = hi.data_ptr(), hj.data_ptr()
+    else:
+        kt.scal = float(scal)
+    rc = L.dgsks_device(ctypes.byref(kt), m, n, k, _p(u), _p(iu), _p(XA), _p(XA2), _p(amap),
+                        _p(XB), _p(XB2), _p(bmap), _p(w), _p(wmap), PRECISIONS[precision], s)
+    _lib.check(rc, "dgsks_device")
+    return u
+
+
+class MappedSummation(torch.autograd.Function):
+    """u_i = sum_j K(XA[amap[i]], XB[bmap[j]]) w[wmap[j]] for the Gaussian
+    (`param` = scal, a 1-element float64 tensor) or the variable-bandwidth
+    Gaussian (`param` = hi, `param_b` = hj, per-point bandwidth vectors).  The
+    same tensor may be passed as XA and XB, or as hi and hj: autograd then sums
+    both sides' gradients.  Backward through ks_grad_device ("fp64"; the
+    Gaussian's mixed precisions through the mixed gradient, the variable
+    bandwidth's in FP64 whatever the forward's precision)."""
+
+    @staticmethod
+    def forward(ctx, XA, XB, w, param, param_b, amap, bmap, wmap, ktype, precision):
+        _check(XA, "XA", 2)
+        _check(XB, "XB", 2)
+        _check(w, "w", 1)
+        var = ktype == _lib.KS_GAUSSIAN_VAR_BANDWIDTH
+        ctx.ktype, ctx.var = ktype, var
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        # the mixed gradient covers the Gaussian; the variable bandwidth's is FP64
+        ctx.precision = "fp64" if var else precision
+        if var:
+            u = mapped_summation_forward(ktype, XA, amap, XB, bmap, w, wmap, hi=param,
+                                         hj=param_b, precision=precision)
+        else:
+            ctx.scal = float(param.detach().reshape(-1)[0].item())
+            u = mapped_summation_forward(ktype, XA, amap, XB, bmap, w, wmap, scal=ctx.scal,
+                                         precision=precision)
+        ctx.save_for_backward(XA, XB, w, param, param_b, amap, bmap, wmap)
+        return u
+
+    @staticmethod
+    def backward(ctx, g):
+        XA, XB, w, param, param_b, amap, bmap, wmap = ctx.saved_tensors
+        need = ctx.needs_input_grad
+        want = tuple(nm for nm, f in (("dXA", need[0]), ("dXB", need[1]), ("dw", need[2]),
+                                      ("dparam", need[3] or need[4])) if f)
+        iu = torch.arange(amap.numel(), dtype=torch.int32, device=g.device)
+        bw = {"hi": param, "hj": param_b} if ctx.var else {"scal": ctx.scal}
+        G = mapped_summation_grad(ctx.ktype, g.to(torch.float64).contiguous(), iu, XA, amap, XB,
+                                  bmap, w, wmap, precision=ctx.precision, want=want, **bw)
+        if ctx.var:
+            dp, dpb = G.get("dhi"), G.get("dhj")
+        else:
+            dp = G["dscal"].reshape(param.shape) if "dscal" in G else None
+            dpb = None
+        return (G.get("dXA"), G.get("dXB"), G.get("dw"), dp, dpb, None, None, None, None, None)
+
+
+def mapped_gaussian_summation(XA, XB, w, scal, amap, bmap, wmap, precision="fp64"):
+    """Gaussian summation over index maps, differentiable in XA, XB, w and the
+    1-element scal tensor."""
+    return MappedSummation.apply(XA, XB, w, scal, None, amap, bmap, wmap, _lib.KS_GAUSSIAN,
+                                 precision)
+
+
+def mapped_var_bandwidth_summation(XA, XB, w, hi, hj, amap, bmap, wmap, precision="fp64"):
+    """Variable-bandwidth Gaussian summation over index maps, differentiable in
+    XA, XB, w and the bandwidth vectors hi, hj."""
+    return MappedSummation.apply(XA, XB, w, hi, hj, amap, bmap, wmap,
+                                 _lib.KS_GAUSSIAN_VAR_BANDWIDTH, precision)

@@ -32,7 +32,7 @@ SAN_FLAGS = ["--offload-host-only", "-Xarch_host", "-fsanitize=address", "-Xarch
              "-fno-omit-frame-pointer", "-g", "-shared-libsan"]
 SANITIZER_RUNTIME = "libclang_rt.asan-x86_64.so"
 
-SOURCES = ["fused_f64.hip", "fused_f64_sym.hip", "fused_f64_mfma.hip", "mixed.hip", "multirhs.hip", "multirhs_mixed.hip", "ref_f64.hip", "norms.hip", "list.hip", "sgsks.hip", "noalloc.hip",
+SOURCES = ["fused_f64.hip", "fused_f64_sym.hip", "fused_f64_mfma.hip", "mixed.hip", "multirhs.hip", "multirhs_mixed.hip", "grad.hip", "ref_f64.hip", "norms.hip", "list.hip", "sgsks.hip", "noalloc.hip",
            "api.cpp", "staging.cpp", "list_api.cpp", "list_tune.cpp", "omp_dgsks_list.cpp"]
 
 # Packed FP32 instructions are disabled for all device code: some of their

@@ -22,6 +22,36 @@ inline int ker_type(const ks_t *k) {
 }
 inline void set_ker_type(ks_t *k, int t) { std::memcpy(&k->type, &t, sizeof(t)); }
 
+// Carves one stream-ordered allocation (hipMallocAsync on the call's stream,
+// freed with hipFreeAsync after the last launch that reads it) into 256-byte
+// aligned arrays.  A layout function runs twice: on a null base to measure,
+// then on the allocation.
+struct Carve {
+  char *base = nullptr;
+  size_t used = 0;
+  template <class T>
+  T *take(size_t count) {
+    T *r = base ? reinterpret_cast<T *>(base + used) : nullptr;
+    used += (sizeof(T) * count + 255) & ~(size_t)255;
+    return r;
+  }
+};
+
+template <class Layout>
+int carve_alloc(Layout &&layout, char **buf, hipStream_t s) {
+  Carve c;
+  layout(c);
+  if (hipMallocAsync((void **)buf, c.used, s) != hipSuccess) {
+    (void)hipGetLastError();
+    *buf = nullptr;
+    return KS_ERR_ALLOC;
+  }
+  c.base = *buf;
+  c.used = 0;
+  layout(c);
+  return KS_OK;
+}
+
 const char *err_string(int code);
 int raise_error(int code, bool fatal, const char *fmt, ...);
 void set_last_error(int code);

@@ -533,34 +533,6 @@ int grad_check(const char *name, int m, int n, int k, double h, int precision, b
   return KS_OK;
 }
 
-// Carves one stream-ordered allocation into 256-byte aligned arrays.  A layout
-// function runs twice: on a null base to measure, then on the allocation.
-struct Carve {
-  char *base = nullptr;
-  size_t used = 0;
-  template <class T>
-  T *take(size_t count) {
-    T *r = base ? reinterpret_cast<T *>(base + used) : nullptr;
-    used += (sizeof(T) * count + 255) & ~(size_t)255;
-    return r;
-  }
-};
-
-template <class Layout>
-int carve_alloc(Layout &&layout, char **buf, hipStream_t s) {
-  Carve c;
-  layout(c);
-  if (hipMallocAsync((void **)buf, c.used, s) != hipSuccess) {
-    (void)hipGetLastError();
-    *buf = nullptr;
-    return KS_ERR_ALLOC;
-  }
-  c.base = *buf;
-  c.used = 0;
-  layout(c);
-  return KS_OK;
-}
-
 __global__ void grad_take_u(const double *state, int m, int k, double *u) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) u[i] = state[i * (k + 1) + k];

@@ -301,3 +301,82 @@ void orc_gauss_grad(int m, int n, int k, const double *XA, const double *XB,
   free(XA2);
   free(XB2);
 }
+
+/* Per-pair chain rule of orc_ks_grad (declaration in gsks_oracle.h): row
+ * terms per target i and column terms per source j in two O(m n k) passes,
+ * then a sequential scatter in row order. */
+void orc_ks_grad(const orc_kernel *ker, int m, int n, int k, const double *g, const int *umap,
+                 const double *XA, const int *amap, const double *XB, const int *bmap,
+                 const double *w, const int *wmap, double *dXA, double *dXB, double *dw,
+                 double *dscal, double *dhi, double *dhj, int nthreads) {
+  const int var = ker->type == ORC_GAUSSIAN_VAR_BANDWIDTH;
+  if (nthreads < 1) nthreads = 1;
+  double *ra = (double *)calloc((size_t)m * (k + 2) + 1, sizeof(double)); /* dA | dhi | dscal */
+  double *cb = (double *)calloc((size_t)n * (k + 2) + 1, sizeof(double)); /* dB | dhj | dw */
+  double *aa = (double *)malloc(sizeof(double) * (size_t)m);
+  double *bb = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < m; i++) orc_sqnorm(XA + (long)amap[i] * k, 1, k, aa + i);
+  for (int j = 0; j < n; j++) orc_sqnorm(XB + (long)bmap[j] * k, 1, k, bb + j);
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
+  for (int i = 0; i < m; i++) {
+    const double *a = XA + (long)amap[i] * k;
+    const double gi = g[umap[i]];
+    const double hi = var ? ker->hi[amap[i]] : 0.0;
+    double *row = ra + (long)i * (k + 2);
+    for (int j = 0; j < n; j++) {
+      const double *b = XB + (long)bmap[j] * k;
+      const double hj = var ? ker->hj[bmap[j]] : 0.0;
+      double c = 0.0, d2 = 0.0;
+      for (int p = 0; p < k; p++) {
+        c += a[p] * b[p];
+        d2 += (a[p] - b[p]) * (a[p] - b[p]);
+      }
+      const double Kw = orc_pair(ker, c, aa[i], bb[j], hi, hj) * w[wmap[j]] * gi;
+      const double cf = var ? -0.5 * hi * hj : ker->scal;
+      for (int p = 0; p < k; p++) row[p] += 2.0 * cf * Kw * (a[p] - b[p]);
+      row[k] += var ? -0.5 * hj * d2 * Kw : 0.0;
+      row[k + 1] += var ? 0.0 : d2 * Kw;
+    }
+  }
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
+  for (int j = 0; j < n; j++) {
+    const double *b = XB + (long)bmap[j] * k;
+    const double wj = w[wmap[j]];
+    const double hj = var ? ker->hj[bmap[j]] : 0.0;
+    double *col = cb + (long)j * (k + 2);
+    for (int i = 0; i < m; i++) {
+      const double *a = XA + (long)amap[i] * k;
+      const double hi = var ? ker->hi[amap[i]] : 0.0;
+      double c = 0.0, d2 = 0.0;
+      for (int p = 0; p < k; p++) {
+        c += a[p] * b[p];
+        d2 += (a[p] - b[p]) * (a[p] - b[p]);
+      }
+      const double Kg = orc_pair(ker, c, aa[i], bb[j], hi, hj) * g[umap[i]];
+      const double cf = var ? -0.5 * hi * hj : ker->scal;
+      for (int p = 0; p < k; p++) col[p] -= 2.0 * cf * Kg * wj * (a[p] - b[p]);
+      col[k] += var ? -0.5 * hi * d2 * Kg * wj : 0.0;
+      col[k + 1] += Kg;
+    }
+  }
+  double s = 0.0;
+  for (int i = 0; i < m; i++) {
+    const double *row = ra + (long)i * (k + 2);
+    if (dXA)
+      for (int p = 0; p < k; p++) dXA[(long)amap[i] * k + p] += row[p];
+    if (dhi) dhi[amap[i]] += row[k];
+    s += row[k + 1];
+  }
+  if (dscal) *dscal += s;
+  for (int j = 0; j < n; j++) {
+    const double *col = cb + (long)j * (k + 2);
+    if (dXB)
+      for (int p = 0; p < k; p++) dXB[(long)bmap[j] * k + p] += col[p];
+    if (dhj) dhj[bmap[j]] += col[k];
+    if (dw) dw[wmap[j]] += col[k + 1];
+  }
+  free(ra);
+  free(cb);
+  free(aa);
+  free(bb);
+}

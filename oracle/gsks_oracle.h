@@ -116,6 +116,26 @@ void orc_gauss_grad(int m, int n, int k, const double *XA, const double *XB,
                     int nrb, const int *rows_b, double *dXB, double *dw,
                     double *dh, int nthreads);
 
+/* Gradient of L = sum_x g[x] du[x] for the mapped summation
+ *   du[umap[i]] += sum_j K(XA[amap[i]], XB[bmap[j]]) w[wmap[j]]
+ * (frame/dgsks_ref.c:94-535's loops) with the Gaussian kernel (ORC_GAUSSIAN,
+ * K = exp(scal d2), :254-270) or the variable-bandwidth Gaussian
+ * (ORC_GAUSSIAN_VAR_BANDWIDTH, K = exp(-0.5 hi[amap[i]] hj[bmap[j]] d2),
+ * :271-288), where the norms in d2 are those of the points themselves.  The
+ * reference has no gradient; this is the chain rule applied pair by pair:
+ * with c = scal (Gaussian) or -0.5 hi hj (variable bandwidth),
+ *   dK/da = 2 c K (a - b),  dK/db = -2 c K (a - b),  dK/dscal = K d2,
+ *   dK/dhi = -0.5 hj d2 K,  dK/dhj = -0.5 hi d2 K,  d2 = sum_p (a_p - b_p)^2,
+ * K itself from orc_pair.  Every output is a table that ACCUMULATES (+=) at
+ * the mapped entries -- dXA [.][k] at amap[i], dXB at bmap[j], dw at wmap[j],
+ * dhi at amap[i], dhj at bmap[j], *dscal once -- a repeated map entry adding
+ * each of its rows' terms, in row order.  Any output may be NULL; dscal is
+ * for the Gaussian, dhi / dhj for the variable bandwidth. */
+void orc_ks_grad(const orc_kernel *ker, int m, int n, int k, const double *g, const int *umap,
+                 const double *XA, const int *amap, const double *XB, const int *bmap,
+                 const double *w, const int *wmap, double *dXA, double *dXB, double *dw,
+                 double *dscal, double *dhi, double *dhj, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

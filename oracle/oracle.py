@@ -91,6 +91,9 @@ def lib():
             L.orc_gauss_grad.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                          _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp,
                                          ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_int]
+            L.orc_ks_grad.argtypes = [ctypes.POINTER(OrcKernel), ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, _dp, _ip, _dp, _ip, _dp, _ip, _dp, _ip,
+                                      _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
             _lib = L
     return _lib
 
@@ -244,6 +247,32 @@ def gauss_grad(XA, XB, w, g, h, rows_a=None, rows_b=None, want_dh=True, nthreads
                          None if rb is None else _i(rb), _d(dXB), _d(dw),
                          ctypes.byref(dh) if want_dh else None, _nthreads(nthreads))
     return dXA, dXB, dw, (dh.value if want_dh else None)
+
+
+def ks_grad(ktype, g, umap, XA, amap, XB, bmap, w, wmap, *, scal=0.0, hi=None, hj=None,
+            nthreads=None):
+    """Gradient tables of L = g . du for the mapped Gaussian or variable-bandwidth
+    summation (orc_ks_grad): dict with dXA, dXB, dw (shaped like XA, XB, w),
+    dscal (Gaussian) or dhi, dhj (shaped like hi, hj; variable bandwidth),
+    accumulated from zero."""
+    XA, XB, w, g = map(_f64, (XA, XB, w, g))
+    umap, amap, bmap, wmap = map(_i32, (umap, amap, bmap, wmap))
+    k = XA.shape[1]
+    out = {"dXA": np.zeros_like(XA), "dXB": np.zeros_like(XB), "dw": np.zeros_like(w)}
+    var = ktype == VAR_BANDWIDTH
+    ds = ctypes.c_double(0.0)
+    if var:
+        out["dhi"] = np.zeros(np.asarray(hi).shape)
+        out["dhj"] = np.zeros(np.asarray(hj).shape)
+    kr, keep = _kernel(ktype, scal, 0.0, 0.0, hi, hj, False)
+    lib().orc_ks_grad(ctypes.byref(kr), amap.size, bmap.size, k, _d(g), _i(umap), _d(XA),
+                      _i(amap), _d(XB), _i(bmap), _d(w), _i(wmap), _d(out["dXA"]),
+                      _d(out["dXB"]), _d(out["dw"]), None if var else ctypes.byref(ds),
+                      _d(out["dhi"]) if var else None, _d(out["dhj"]) if var else None,
+                      _nthreads(nthreads))
+    if not var:
+        out["dscal"] = ds.value
+    return out
 
 
 def ref_call(fn_name, ktype, u, umap, XA, XA2, amap, XB, XB2, bmap, w, wmap, *, scal=0.0,

@@ -228,3 +228,27 @@ def test_host_argument_checks_before_the_device(L):
     assert L.ks_list_autotune(9, 3, 0, ctypes.byref(a), ctypes.byref(b)) == \
         _lib.KS_ERR_INVALID_ARG
     assert np.all(u == 0)
+    # mapped gradient: kernel types, outputs of the other kernel, precision, k
+    one = np.zeros(1)
+
+    def grad(kt, k=3, prec=0, dscal=None, dhi=None):
+        return L.ks_grad_device(ctypes.byref(kt) if kt is not None else None, 8, 8, k, ptr(w),
+                                ptr(idx), ptr(X), ptr(idx), ptr(X), ptr(idx), ptr(w), ptr(idx),
+                                None, None, None, dscal, dhi, None, prec, None)
+    gk = KsT()
+    gk.type = _lib.KS_GAUSSIAN
+    gk.scal = -1.0
+    vk = KsT()
+    vk.type = _lib.KS_GAUSSIAN_VAR_BANDWIDTH
+    assert grad(None) == _lib.KS_ERR_INVALID_ARG
+    assert grad(kt) == _lib.KS_ERR_INVALID_ARG  # Laplace: not a Gaussian kernel
+    assert grad(gk, dhi=ptr(one)) == _lib.KS_ERR_INVALID_ARG
+    assert grad(gk, prec=_lib.KS_PREC_MIXED_BF16X3) == _lib.KS_ERR_INVALID_ARG
+    assert grad(gk, k=72) == _lib.KS_ERR_INVALID_ARG
+    assert grad(gk, k=33, prec=_lib.KS_PREC_MIXED_FP32) == _lib.KS_ERR_INVALID_ARG
+    assert grad(vk) == _lib.KS_ERR_NO_BANDWIDTH
+    vk.hi = vk.hj = ptr(one).value
+    assert grad(vk, dscal=ptr(one)) == _lib.KS_ERR_INVALID_ARG
+    assert grad(vk, prec=_lib.KS_PREC_MIXED_FP32) == _lib.KS_ERR_INVALID_ARG
+    assert grad(vk, k=70) == _lib.KS_ERR_INVALID_ARG
+    assert L.ks_last_error() == _lib.KS_ERR_INVALID_ARG

@@ -374,3 +374,80 @@ def test_gauss_grad_vs_central_differences(oracle):
                                          want_dh=False)
     assert np.array_equal(dA2, dA[[3, 7]]) and np.array_equal(dB2, dB[[1, 4]])
     assert np.array_equal(dw2, dw[[1, 4]])
+
+
+@pytest.mark.parametrize("ktype", ["gaussian", "var_bandwidth"])
+def test_mapped_grad_vs_central_differences(oracle, ktype):
+    """orc_ks_grad (index maps with repeated entries, one point table and one
+    bandwidth vector on both sides, as the reference's symmetric calls use them)
+    against central differences of the oracle's own mapped forward.  A point
+    that both maps select receives dXA + dXB, a bandwidth dhi + dhj; a repeated
+    wmap entry sums its sources' terms."""
+    rng = np.random.default_rng(7)
+    var = ktype == "var_bandwidth"
+    T = oracle.VAR_BANDWIDTH if var else oracle.GAUSSIAN
+    nx, k, m, n = 30, 4, 25, 28
+    X = rng.random((nx, k))
+    hv = 1.0 + rng.random(nx) * 4.0
+    w = rng.standard_normal(nx)
+    g = rng.standard_normal(nx)
+    amap = rng.integers(0, nx, m).astype(np.int32)  # repeats on purpose
+    bmap = rng.integers(0, nx, n).astype(np.int32)
+    wmap = rng.integers(0, nx, n).astype(np.int32)
+    umap = rng.integers(0, nx, m).astype(np.int32)
+    scal0 = -1.7
+
+    def loss(X, w, hv, scal):
+        X2 = oracle.sqnorm(X)
+        out = oracle.dgsks_rows(T, X, X2, amap, X, X2, bmap, w, wmap, np.zeros(nx), umap,
+                                scal=scal, hi=hv if var else None, hj=hv if var else None)
+        return float(g[umap] @ out)
+
+    G = oracle.ks_grad(T, g, umap, X, amap, X, bmap, w, wmap, scal=scal0,
+                       hi=hv if var else None, hj=hv if var else None)
+    dX = G["dXA"] + G["dXB"]
+    eps = 1e-6
+    for x in sorted(set(amap[:4]) | set(bmap[:4])):
+        for p in (0, k - 1):
+            Xp, Xm = X.copy(), X.copy()
+            Xp[x, p] += eps
+            Xm[x, p] -= eps
+            fd = (loss(Xp, w, hv, scal0) - loss(Xm, w, hv, scal0)) / (2 * eps)
+            assert dX[x, p] == pytest.approx(fd, rel=1e-6, abs=1e-8)
+    unused = sorted(set(range(nx)) - set(amap) - set(bmap))
+    assert not np.any(dX[unused])
+    for x in sorted(set(wmap[:5])):
+        wp, wm = w.copy(), w.copy()
+        wp[x] += eps
+        wm[x] -= eps
+        fd = (loss(X, wp, hv, scal0) - loss(X, wm, hv, scal0)) / (2 * eps)
+        assert G["dw"][x] == pytest.approx(fd, rel=1e-7, abs=1e-9)
+    if var:
+        dh = G["dhi"] + G["dhj"]
+        for x in sorted(set(amap[:3]) | set(bmap[:3])):
+            hp, hm = hv.copy(), hv.copy()
+            hp[x] += eps
+            hm[x] -= eps
+            fd = (loss(X, w, hp, scal0) - loss(X, w, hm, scal0)) / (2 * eps)
+            assert dh[x] == pytest.approx(fd, rel=1e-6, abs=1e-8)
+    else:
+        fd = (loss(X, w, hv, scal0 + eps) - loss(X, w, hv, scal0 - eps)) / (2 * eps)
+        assert G["dscal"] == pytest.approx(fd, rel=1e-6)
+
+
+def test_mapped_grad_reduces_to_identity_gradient(oracle):
+    """With identity maps and scal = -1/(2h^2) the mapped gradient is the
+    fixed-bandwidth one: dXA, dXB, dw agree with orc_gauss_grad, and dh =
+    dscal * dscal/dh = dscal / h^3."""
+    rng = np.random.default_rng(12)
+    m, n, k, h = 17, 23, 5, 0.6
+    A, B = rng.random((m, k)), rng.random((n, k))
+    w, g = rng.standard_normal(n), rng.standard_normal(m)
+    ia, ib = np.arange(m, dtype=np.int32), np.arange(n, dtype=np.int32)
+    G = oracle.ks_grad(oracle.GAUSSIAN, g, ia, A, ia, B, ib, w, ib, scal=-1.0 / (2 * h * h))
+    dA, dB, dw, dh = oracle.gauss_grad(A, B, w, g, h)
+    assert np.allclose(G["dXA"], dA, rtol=1e-12, atol=1e-14)
+    assert np.allclose(G["dXB"], dB, rtol=1e-12, atol=1e-14)
+    assert np.allclose(G["dw"], dw, rtol=1e-13, atol=1e-15)
+    assert G["dscal"] / h ** 3 == pytest.approx(dh, rel=1e-12)
+
