@@ -25,6 +25,7 @@ CPU baseline:
   list_sym_k3            the same with every task's sources its targets: symmetric
                          layout beside the full one
   grad_gauss16           forward + gradient step, m = n = 32768, k = 16
+  grad_varbw16_maps      variable-bandwidth forward + gradient over index maps, 32768^2, k = 16
 """
 from __future__ import annotations
 
@@ -99,6 +100,11 @@ def build_workload(name):
         d = W.grad_gauss16()
         d["label"] = ("grad_gauss16 gaussian h=0.4 m=n=32768 k=16: forward u = K w and the "
                       "gradient of g.u in dXA, dXB, dw, dh")
+    elif name == "grad_varbw16_maps":
+        d = W.grad_varbw16_maps()
+        d["label"] = ("grad_varbw16_maps variable bandwidth m=n=32768 k=16 over a 40000-point "
+                      "table, maps with repeats: forward u and the gradient of g.u in the "
+                      "tables dX, dw, dh")
     elif name.startswith("list_pool") or name.startswith("list_sym"):
         k = int(name.split("_k")[1]) if "_k" in name else 3
         sym = name.startswith("list_sym")
@@ -552,6 +558,87 @@ def run_grad(args, name, precision, steps, warmup, with_cpu):
     return out
 
 
+def cpu_baseline_grad_maps(d, budget_s=4.0):
+    """The oracle's mapped FP64 gradient (a port, oracle/gsks_oracle.c
+    orc_ks_grad) on the first R target rows against all n sources, on one core
+    and on every thread of the host's share; rate = R x n pairs / time."""
+    from oracle import oracle as O
+    info = host_info()
+    threads = info["threads"]
+    n = d["bmap"].size
+
+    def timed(rows, nth):
+        t0 = time.perf_counter()
+        O.ks_grad(O.VAR_BANDWIDTH, d["g"], np.arange(rows, dtype=np.int32), d["X"],
+                  d["amap"][:rows], d["X"], d["bmap"], d["w"], d["wmap"], hi=d["h"], hj=d["h"],
+                  nthreads=nth)
+        return time.perf_counter() - t0
+
+    m = d["amap"].size
+    t0 = timed(4, 1)
+    r1 = int(min(m, max(4, budget_s / 4 * 4 / t0)))
+    t1 = timed(r1, 1)
+    rate1 = r1 * n / t1
+    rows = int(min(m, max(threads, budget_s * rate1 * threads / n)))
+    dt = timed(rows, threads)
+    res = {"value": rows * n / dt, "unit": "evals/s", "cores": threads, "kind": "port",
+           "sample": f"the reference has no backward; oracle FP64 mapped gradient over the first "
+                     f"{rows} target rows and all {n} sources, {threads} threads, {dt:.2f} s; "
+                     f"one core: {r1} rows",
+           "one_core_value": rate1}
+    res.update(info)
+    return res
+
+
+def run_grad_maps(args, name, steps, warmup, with_cpu):
+    """One step = the mapped variable-bandwidth forward u (dgsks_device) and
+    the gradient of L = g.u added into the tables dX, dw, dh (ks_grad_device:
+    two multi-right-hand-side passes of k + 2 and k + 3 columns, then the
+    ordered scatter into the tables).  FP64; HIP events on the stream.  Flops
+    per pair: 2k + 35 for the forward (the reference's model) and 2k + 35 +
+    2R for each pass of R right-hand sides."""
+    import torch
+    from ks_amd import _lib
+    from ks_amd.autograd import mapped_summation_forward, mapped_summation_grad
+    t_start = time.perf_counter()
+    d = build_workload(name)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    X, h, w, g = dev(d["X"]), dev(d["h"]), dev(d["w"]), dev(d["g"])
+    am, bm, wm = dev(d["amap"]), dev(d["bmap"]), dev(d["wmap"])
+    iu = torch.arange(am.numel(), dtype=torch.int32, device="cuda")
+    T = _lib.KS_GAUSSIAN_VAR_BANDWIDTH
+    k, m, n = d["k"], am.numel(), bm.numel()
+    stream = torch.cuda.current_stream()
+
+    def step():
+        u = mapped_summation_forward(T, X, am, X, bm, w, wm, hi=h, hj=h)
+        G = mapped_summation_grad(T, g, iu, X, am, X, bm, w, wm, hi=h, hj=h)
+        return u, G
+
+    wall, kms = _events_timed(stream, step, steps, warmup)
+    pairs = float(m) * float(n)
+    fpe = 3 * (2 * k + FLOP_C[3]) + 2 * (k + 2) + 2 * (k + 3)
+    achieved = pairs * fpe / (kms / 1e3) / 1e12
+    out = {"metric": METRIC, "value": pairs / (wall / 1e3), "unit": "evals/s", "n_gpus": 1,
+           "steps": steps, "warmup": warmup, "ms_per_step": wall,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded numpy, regenerated each run)",
+           "config": {"workload": d["label"], "precision": "fp64", "m": m, "n": n, "k": k,
+                      "kernel": "var_bandwidth"},
+           "roofline": {"bound": "mfma", "pipe": "fp64 (matrix and vector peaks are equal)",
+                        "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                        "traffic_source": None, "flops_per_eval": fpe,
+                        "flop_model": "3 (2k + 35) (forward and two passes, "
+                                      "test/test_dgsks.c:254-283) + 2 (k + 2) + 2 (k + 3)",
+                        "kernel_ms": kms,
+                        "kernel": "fused forward + gsks::multirhs_mfma_kernel (2 passes)"}}
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline_grad_maps(d, budget_s=args.sub_cpu_seconds)
+    out["bench_s"] = time.perf_counter() - t_start
+    return out
+
+
 def bench_single(L, kt, k, amap, bmap, X, w, stream, steps, u64):
     """The single-precision front end (sgsks_device) on float copies of the same
     inputs, norms left to the library: in the precision sgsks() picks (FP64
@@ -811,6 +898,7 @@ def suite(args, L):
             args, L, f"list_pool_k{k}", "mixed_bf16x3", steps, warm, False))
     guard("list_sym_k3", lambda: run_list(args, L, "list_sym_k3", "fp64", steps, warm, cpu))
     guard("grad_gauss16", lambda: run_grad(args, "grad_gauss16", "auto", steps, warm, cpu))
+    guard("grad_varbw16_maps", lambda: run_grad_maps(args, "grad_varbw16_maps", steps, warm, cpu))
     return subs
 
 
@@ -862,6 +950,8 @@ def main():
             out = run_list(args, L, name, args.precision, args.steps, args.warmup, cpu)
         elif name == "grad_gauss16":
             out = run_grad(args, name, args.precision, args.steps, args.warmup, cpu)
+        elif name == "grad_varbw16_maps":
+            out = run_grad_maps(args, name, args.steps, args.warmup, cpu)
         else:
             out = run_single(args, L, name, args.precision, args.steps, args.warmup, cpu,
                              args.cpu_seconds, graph=name == "uniform_gauss16",

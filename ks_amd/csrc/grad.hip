@@ -208,17 +208,22 @@ int kg_pass(const GradCall &c, const GradWork &g, const double *T, const double 
   return c.mixed ? enqueue_multirhs_mixed(p, R, g.zc, s) : enqueue_multirhs(p, R, s);
 }
 
-// Adds rows [rows][width] of src into table at map's entries.
-int kg_scatter(const GradWork &g, const int *map, int rows, int width, const double *src,
-               double *table, hipStream_t s) {
+// Sorts map's entries (stably: equal entries keep row order) into g.skey with
+// their rows in g.perm, for the scatters that follow.
+int kg_sort(const GradWork &g, const int *map, int rows, hipStream_t s) {
   size_t bytes = g.sort_bytes;
   hipError_t e = hipcub::DeviceRadixSort::SortPairs(
       g.sort_tmp, bytes, reinterpret_cast<const unsigned *>(map), g.skey, g.ident, g.perm, rows,
       0, 31, s);
-  if (e != hipSuccess) return hip_failure(e, "gradient map sort");
-  hipLaunchKernelGGL(kg_scatter_add, dim3(blocks((long)rows * width)), dim3(GW), 0, s, g.skey,
-                     g.perm, rows, width, src, table);
-  return KS_OK;
+  return e == hipSuccess ? KS_OK : hip_failure(e, "gradient map sort");
+}
+
+// Adds rows [rows][width] of src into table at the entries of the map last sorted.
+void kg_scatter(const GradWork &g, int rows, int width, const double *src, double *table,
+                hipStream_t s) {
+  if (table)
+    hipLaunchKernelGGL(kg_scatter_add, dim3(blocks((long)rows * width)), dim3(GW), 0, s, g.skey,
+                       g.perm, rows, width, src, table);
 }
 
 }  // namespace
@@ -356,11 +361,21 @@ extern "C" int ks_grad_device(const ks_t *kernel, int m, int n, int k, const dou
   if (rc == KS_OK && dscal)
     hipLaunchKernelGGL(kg_sum, dim3(1), dim3(SW), 0, s, W.terms, (long)m + n, dscal);
   // add the rows into the caller's tables
-  if (rc == KS_OK && dXA) chain(kg_scatter(W, amap, m, k, W.GA, dXA, s));
-  if (rc == KS_OK && dhi) chain(kg_scatter(W, amap, m, 1, W.GhA, dhi, s));
-  if (rc == KS_OK && dXB) chain(kg_scatter(W, bmap, n, k, W.GB, dXB, s));
-  if (rc == KS_OK && dhj) chain(kg_scatter(W, bmap, n, 1, W.GhB, dhj, s));
-  if (rc == KS_OK && dw) chain(kg_scatter(W, wmap, n, 1, W.Gw, dw, s));
+  if (rc == KS_OK && (dXA || dhi)) {
+    chain(kg_sort(W, amap, m, s));
+    kg_scatter(W, m, k, W.GA, dXA, s);
+    kg_scatter(W, m, 1, W.GhA, dhi, s);
+  }
+  if (rc == KS_OK && (dXB || dhj || (dw && wmap == bmap))) {
+    chain(kg_sort(W, bmap, n, s));
+    kg_scatter(W, n, k, W.GB, dXB, s);
+    kg_scatter(W, n, 1, W.GhB, dhj, s);
+    if (wmap == bmap) kg_scatter(W, n, 1, W.Gw, dw, s);
+  }
+  if (rc == KS_OK && dw && wmap != bmap) {
+    chain(kg_sort(W, wmap, n, s));
+    kg_scatter(W, n, 1, W.Gw, dw, s);
+  }
   if (rc == KS_OK && hipGetLastError() != hipSuccess) rc = KS_ERR_DEVICE;
   (void)hipFreeAsync(buf, s);
   if (rc != KS_OK) return raise_error(rc, false, "%s(): %s\n", name, err_string(rc));

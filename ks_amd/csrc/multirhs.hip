@@ -127,15 +127,15 @@ __global__ __launch_bounds__(MR_WG) void multirhs_kernel(MrhsArgs A) {
 // next product, chunk v of U[targets][rhs] += K[targets][sources] W[sources][rhs].
 // Per 16 x 16 block that is 4 KC cross-term MFMAs, the 4 kernel values per lane
 // on the VALU, and 4 NT MFMAs for the right-hand sides; nothing is transposed.
-// With XR, right-hand side 16 NT (the last, nrhs = 16 NT + 1) is summed on the
-// VALU instead of a further 16-wide tile: 4 FMAs per lane per block, and the 4
-// lane groups (sources 4v + g) are added by two lane shuffles at the end.
+// The last NX right-hand sides (nrhs = 16 NT + NX, NX <= 3) are summed on the
+// VALU instead of a further 16-wide tile: 4 NX FMAs per lane per block, and the
+// 4 lane groups (sources 4v + g) are added by two lane shuffles at the end.
 // A wave owns RB groups of 16 targets; splits are reduced as in the VALU form.
 constexpr int MRM_RB = 2;
 constexpr int MRM_ROWS = 4 * MRM_RB * 16;  // targets per workgroup
 
-template <int TYPE, int KC, int NT, bool EXACT, bool XR>
-__global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
+template <int TYPE, int KC, int NT, bool EXACT, int NX>
+__global__ __launch_bounds__(MR_WG, (NX >= 3 && KC == 1) ? 2 : 1) void multirhs_mfma_kernel(MrhsArgs A) {
   typedef double d4 __attribute__((ext_vector_type(4)));
   constexpr int RB = MRM_RB;
   const Problem &p = A.p;
@@ -161,10 +161,11 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
     hi[rb] = (TYPE == VAR_BANDWIDTH) ? -0.5 * p.hi[ai] : 0.0;
   }
   d4 U[RB][NT];  // lane (p16, g): targets 16 rb + g + 4v' x rhs 16 t + p16
-  double ux[RB];  // XR: right-hand side 16 NT, lane (p16, g)'s sources of target p16
+  double ux[RB][NX > 0 ? NX : 1];  // right-hand sides 16 NT + x, lane (p16, g)'s sources of target p16
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) {
-    ux[rb] = 0.0;
+#pragma unroll
+    for (int x = 0; x < NX; ++x) ux[rb][x] = 0.0;
 #pragma unroll
     for (int t = 0; t < NT; ++t) U[rb][t] = d4{0.0, 0.0, 0.0, 0.0};
   }
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
     double f[KC][4];
     double bb[4], hj[4];
     double wv[4][NT];
-    double wx[4];  // XR: weight 16 NT of the sources 4v + g
+    double wx[4][NX > 0 ? NX : 1];  // weights 16 NT + x of the sources 4v + g
   };
   // a block's map entries load two blocks ahead of the data they address, so
   // no block waits on two dependent round trips
@@ -212,7 +213,8 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
         const int r = 16 * t + p16;
         b.wv[v][t] = (wrow >= 0 && r < nr) ? p.w[wrow + r] : 0.0;
       }
-      if constexpr (XR) b.wx[v] = wrow >= 0 ? p.w[wrow + 16 * NT] : 0.0;
+#pragma unroll
+      for (int x = 0; x < NX; ++x) b.wx[v][x] = wrow >= 0 ? p.w[wrow + 16 * NT + x] : 0.0;
     }
   };
   auto block = [&](const SB &b) {
@@ -232,10 +234,10 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
 #pragma unroll
         for (int v = 0; v < 4; ++v)
           U[rb][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Kv[v], b.wv[v][t], U[rb][t], 0, 0, 0);
-      if constexpr (XR) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) ux[rb] = fma(Kv[v], b.wx[v], ux[rb]);
-      }
+      for (int x = 0; x < NX; ++x)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ux[rb][x] = fma(Kv[v], b.wx[v][x], ux[rb][x]);
     }
   };
   // b0 / b1: data of blocks jb / jb + 16; i0 / i1: map entries of the block
@@ -263,13 +265,13 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
 
   // ---- results: lane (p16, g) holds target 16 rb + g + 4v, rhs 16 t + p16;
   // the extra right-hand side sums its 4 lane groups into lanes g = 0
-  if constexpr (XR) {
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
-      ux[rb] += __shfl_xor(ux[rb], 16);
-      ux[rb] += __shfl_xor(ux[rb], 32);
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int x = 0; x < NX; ++x) {
+      ux[rb][x] += __shfl_xor(ux[rb][x], 16);
+      ux[rb][x] += __shfl_xor(ux[rb][x], 32);
     }
-  }
   if (A.nsplit == 1) {
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
@@ -280,9 +282,12 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
           const int i = row0 + 16 * rb + g + 4 * v, r = 16 * t + p16;
           if (i < m && r < nr) p.u[(long)p.umap[i] * nr + r] += U[rb][t][v];
         }
-      if constexpr (XR) {
+      if (NX > 0) {
         const int i = row0 + 16 * rb + p16;
-        if (g == 0 && i < m) p.u[(long)p.umap[i] * nr + 16 * NT] += ux[rb];
+        if (g == 0 && i < m) {
+#pragma unroll
+          for (int x = 0; x < NX; ++x) p.u[(long)p.umap[i] * nr + 16 * NT + x] += ux[rb][x];
+        }
       }
     }
     return;
@@ -296,9 +301,12 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
         const int i = row0 + 16 * rb + g + 4 * v, r = 16 * t + p16;
         if (i < m && r < nr) A.partial[((long)split * m + i) * nr + r] = U[rb][t][v];
       }
-    if constexpr (XR) {
+    if (NX > 0) {
       const int i = row0 + 16 * rb + p16;
-      if (g == 0 && i < m) A.partial[((long)split * m + i) * nr + 16 * NT] = ux[rb];
+      if (g == 0 && i < m) {
+#pragma unroll
+        for (int x = 0; x < NX; ++x) A.partial[((long)split * m + i) * nr + 16 * NT + x] = ux[rb][x];
+      }
     }
   }
   // the splits' partials are added in split order by a reduction kernel
@@ -306,34 +314,39 @@ __global__ __launch_bounds__(MR_WG) void multirhs_mfma_kernel(MrhsArgs A) {
   // and invalidate this XCD's L2)
 }
 
-template <int TYPE, int KC, int NT, bool XR>
+template <int TYPE, int KC, int NT, int NX>
 static hipError_t mrm_go(const MrhsArgs &A, int rblocks, hipStream_t s) {
   dim3 grid(rblocks, A.nsplit);
   const bool exact = A.p.k == 16 * KC && (uintptr_t)A.p.XA % 16 == 0 &&
                      (uintptr_t)A.p.XB % 16 == 0;
   if (exact)
-    hipLaunchKernelGGL((multirhs_mfma_kernel<TYPE, KC, NT, true, XR>), grid, dim3(MR_WG), 0, s, A);
+    hipLaunchKernelGGL((multirhs_mfma_kernel<TYPE, KC, NT, true, NX>), grid, dim3(MR_WG), 0, s, A);
   else
-    hipLaunchKernelGGL((multirhs_mfma_kernel<TYPE, KC, NT, false, XR>), grid, dim3(MR_WG), 0, s, A);
+    hipLaunchKernelGGL((multirhs_mfma_kernel<TYPE, KC, NT, false, NX>), grid, dim3(MR_WG), 0, s, A);
   return hipGetLastError();
 }
 
-template <int TYPE, int NT, bool XR>
+template <int TYPE, int NT, int NX>
 static hipError_t mrm_kc(const MrhsArgs &A, int rblocks, hipStream_t s) {
   switch ((A.p.k + 15) / 16) {
-    case 1: return mrm_go<TYPE, 1, NT, XR>(A, rblocks, s);
-    case 2: return mrm_go<TYPE, 2, NT, XR>(A, rblocks, s);
-    case 3: return mrm_go<TYPE, 3, NT, XR>(A, rblocks, s);
-    default: return mrm_go<TYPE, 4, NT, XR>(A, rblocks, s);
+    case 1: return mrm_go<TYPE, 1, NT, NX>(A, rblocks, s);
+    case 2: return mrm_go<TYPE, 2, NT, NX>(A, rblocks, s);
+    case 3: return mrm_go<TYPE, 3, NT, NX>(A, rblocks, s);
+    default: return mrm_go<TYPE, 4, NT, NX>(A, rblocks, s);
   }
 }
 
-// 16-wide tiles on the matrix cores; a 17th right-hand side on the VALU
+// 16-wide tiles on the matrix cores; a 17th right-hand side on the VALU, and
+// for the Gaussian kernels (whose gradients take 17 to 19) up to three
 template <int TYPE>
 static hipError_t mrm_nt(const MrhsArgs &A, int rblocks, hipStream_t s) {
-  if (A.nrhs <= 16) return mrm_kc<TYPE, 1, false>(A, rblocks, s);
-  if (A.nrhs == 17) return mrm_kc<TYPE, 1, true>(A, rblocks, s);
-  return mrm_kc<TYPE, 2, false>(A, rblocks, s);
+  if (A.nrhs <= 16) return mrm_kc<TYPE, 1, 0>(A, rblocks, s);
+  if (A.nrhs == 17) return mrm_kc<TYPE, 1, 1>(A, rblocks, s);
+  if constexpr (TYPE == GAUSSIAN || TYPE == VAR_BANDWIDTH) {
+    if (A.nrhs == 18) return mrm_kc<TYPE, 1, 2>(A, rblocks, s);
+    if (A.nrhs == 19) return mrm_kc<TYPE, 1, 3>(A, rblocks, s);
+  }
+  return mrm_kc<TYPE, 2, 0>(A, rblocks, s);
 }
 
 template <int TYPE, int RMAX>

@@ -107,6 +107,18 @@ def list_pool(k=3, ntask=4096, pool=1 << 20, lo=64, hi=8192, seed=4, symmetric=F
                 amap=np.concatenate(amaps), bmap=np.concatenate(bmaps))
 
 
+def grad_varbw16_maps(m=32768, n=32768, k=16, nx=40000, seed=9):
+    """Variable-bandwidth gradient over index maps: one table of nx points in
+    [0,1)^k and one bandwidth vector on both sides (hi = hj, per-point values
+    in [0.6, 1.8)), maps drawn with repeats."""
+    rng = np.random.default_rng(seed)
+    return dict(X=rng.random((nx, k)), h=(0.5 + rng.random(nx)) * 1.2,
+                w=rng.standard_normal(nx), g=rng.standard_normal(m),
+                amap=rng.integers(0, nx, m).astype(np.int32),
+                bmap=rng.integers(0, nx, n).astype(np.int32),
+                wmap=rng.integers(0, nx, n).astype(np.int32), k=k)
+
+
 def grad_gauss16(m=32768, n=32768, k=16, h=0.4, seed=5):
     rng = np.random.default_rng(seed)
     X = rng.random((max(m, n), k))

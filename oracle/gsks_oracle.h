@@ -18,6 +18,10 @@
  * golden fixtures under tests/golden/ that were written from it, an
  * independent numpy restatement, mpmath brute force on tiny inputs, and closed
  * forms (multiquadric, integer polynomial, scal = 0) -- see tests/test_oracle.py.
+ * The gradients (orc_gauss_grad, orc_ks_grad; the reference has none) are
+ * pinned by central differences of this oracle's own forward, with repeated
+ * map entries and one table on both sides, and orc_ks_grad with identity maps
+ * against orc_gauss_grad (tests/test_oracle.py).
  */
 #ifndef GSKS_ORACLE_H
 #define GSKS_ORACLE_H

@@ -213,13 +213,16 @@ def test_multirhs_matches_single_rhs(torch_cuda, oracle):
 
 @pytest.mark.parametrize("t,k,R", [(0, 16, 7), (1, 40, 16), (3, 64, 23), (4, 32, 32), (5, 20, 3),
                                    (6, 16, 17), (3, 33, 17), (0, 64, 17), (7, 64, 9),
-                                   (0, 16, 40)])
+                                   (0, 16, 40), (0, 20, 18), (3, 16, 18), (3, 48, 19),
+                                   (0, 16, 19), (1, 16, 19)])
 def test_multirhs_matrix_core_form(torch_cuda, oracle, t, k, R):
     """The matrix-core multi-RHS form (9 <= k <= 64, not Laplace, R <= 32: the
     sources are the A operand of the cross term, so the kernel values land in
     the A-operand layout of the right-hand-side product) against the oracle,
     column by column, with random maps and ragged sizes; R = 17 adds the last
-    right-hand side on the VALU, R = 40 runs the vector form."""
+    right-hand side on the VALU (R = 18 and 19 the last two and three for the
+    Gaussian kernels; a polynomial at 19 takes two tiles), R = 40 runs the
+    vector form."""
     torch = torch_cuda
     from ks_amd import _lib
     L = _lib.lib()
