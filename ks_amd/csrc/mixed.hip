@@ -117,6 +117,8 @@ struct MixPack {
   const float *na, *nb;
   const unsigned *rt, *rs;
   long mp, np;
+  // per slice, its row blocks by descending needed-block count (null: in order)
+  const int *order;
 };
 
 // FP32 copies of the pivots: pivot i is target (i * m) / P, zero past k; also
@@ -451,6 +453,23 @@ __global__ __launch_bounds__(256) void mix_bounds_kernel(
 // registers (lane l holds entries l and 64 + l) and are read with lane
 // shuffles, so a group's only memory loads are the blocks' bucket ranges and
 // their Vs entries, independent of each other.
+// Sort keys of the workgroups of each slice: slice in the high bits, then the
+// complement of the workgroup's needed-block count, so an ascending sort puts
+// every slice's heaviest row blocks first.
+__global__ __launch_bounds__(256) void mix_order_kernel(const unsigned long long *need, int ngw,
+                                                       int nsplit, int row_blocks,
+                                                       unsigned *key, int *val) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= nsplit * row_blocks) return;
+  const int split = t / row_blocks, rb = t - split * row_blocks;
+  unsigned work = 0;
+  for (int w = 0; w < 4; ++w)
+    for (int gi = 0; gi < ngw; ++gi)
+      work += __popcll(need[((long)(rb * 4 + w) * nsplit + split) * ngw + gi]);
+  key[t] = ((unsigned)split << 20) | (0xFFFFFu - (work < 0xFFFFFu ? work : 0xFFFFFu));
+  val[t] = rb;
+}
+
 __global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ngrp, int nsplit,
                                                        int cps, int ngw, int n,
                                                        unsigned long long *need) {
@@ -724,10 +743,13 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
   // runs workgroups x, x + 8, ...; give it the source slices x, x + 8, ... and
   // let it sweep every row block of one slice before the next, so that the
   // workgroups sharing its L2 read the same slice (nsplit is a multiple of 8).
+  // Within a slice the row blocks go heaviest first (pk.order, by their
+  // needed-block counts): the few row blocks whose clusters lie in the slice
+  // hold most of its work, and started last they would set the launch's end.
   const int row_blocks = (m + 4 * RB * 16 - 1) / (4 * RB * 16);
   const int xcd = (int)(blockIdx.x % kXcds), loc = (int)(blockIdx.x / kXcds);
-  const int rblk = loc % row_blocks;
   const int split = xcd + kXcds * (loc / row_blocks);
+  const int rblk = pk.order ? pk.order[split * row_blocks + loc % row_blocks] : loc % row_blocks;
   constexpr int WROWS = RB * 16;
   const int wg_rows = 4 * WROWS;
   const int row0 = rblk * wg_rows + wave * WROWS;  // sorted target position
@@ -1431,6 +1453,9 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   const size_t o_rs = off; off += bounds ? al(nsb * sizeof(unsigned)) : 0;
   const size_t o_need = off;
   off += bounds ? al(ngrp * plan.nsplit * ngw * sizeof(unsigned long long)) : 0;
+  // dispatch order of the row blocks of each slice: keys and values, twice
+  const size_t nord = bounds ? (size_t)plan.nsplit * plan.row_blocks : 0;
+  const size_t o_ord = off; off += bounds ? al(4 * nord * sizeof(int)) : 0;
   const size_t o_sort = off; off += al(sort_bytes);
   const size_t npart = (size_t)(plan.nsplit > 1 ? plan.nsplit : 0) * p.m;
   double *partial = nullptr;
@@ -1461,6 +1486,7 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   pk.rt = pk.rs = nullptr;
   pk.mp = (long)mp;
   pk.np = (long)np;
+  pk.order = nullptr;
 
   // ---- target and source orders
   if (!bucket) {
@@ -1546,6 +1572,17 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
       hipLaunchKernelGGL(mix_need_kernel, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, s, mb,
                          (int)ngrp, plan.nsplit, plan.cols_per_split, (int)ngw, p.n, need);
       pk.need = need;
+      // each slice's row blocks by descending needed-block count (stable: equal
+      // counts keep their order), so the heaviest start first
+      unsigned *ok0 = (unsigned *)(base + o_ord), *ok1 = ok0 + nord;
+      int *ov0 = (int *)(ok1 + nord), *ov1 = ov0 + nord;
+      hipLaunchKernelGGL(mix_order_kernel, dim3((unsigned)((nord + 255) / 256)), dim3(256), 0, s,
+                         need, (int)ngw, plan.nsplit, plan.row_blocks, ok0, ov0);
+      size_t obytes = sort_bytes;
+      if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, obytes, ok0, ok1, ov0, ov1, (int)nord, 0,
+                                             32, s) != hipSuccess)
+        return KS_ERR_DEVICE;
+      pk.order = ov1;
     }
   }
   if (hipGetLastError() != hipSuccess) return KS_ERR_DEVICE;
