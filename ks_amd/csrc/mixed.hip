@@ -81,11 +81,12 @@ constexpr bool kSkipTier = PREC == KS_PREC_MIXED_BF16X3 && TYPE == GAUSSIAN;
 // ------------------------------------------------------------------ pre-pass
 
 // Pruning bounds of the skip tier (mix_bounds_kernel): Vt[T][i] and Vs[i][S]
-// per pivot i and 16-point target block T / source block S, the bucket range
-// (first | last << 16) of every block, and the pivot distances
-// pp[A][B] = |p_A - p_B|^2 rounded up.
+// per pivot i and 16-point target block T / source block S over the block's
+// points in its first bucket, Vt1 and Vs1 over its other points (+inf in a
+// block of one bucket), the bucket range (first | last << 16) of every block,
+// and the pivot distances pp[A][B] = |p_A - p_B|^2 rounded up.
 struct MixBounds {
-  const float *Vt, *Vs, *pp;
+  const float *Vt, *Vs, *Vt1, *Vs1, *pp;
   const unsigned *rt, *rs;
   int ntb, nsb;  // target and source blocks in the tables
   float thr_d2;  // a block pair whose d^2 lower bound exceeds this is negligible
@@ -285,7 +286,7 @@ __global__ void mix_pivot_gram_kernel(int P, int kp, const float *piv, float *pp
 // exponent (log2 units) is Ex[B][t] + Qx[A][j] - 2 xs alpha.beta, where only
 // the last term, a product of two small centred vectors, is left to the kernel.
 struct MixTabs {
-  float *Vt, *Vs;
+  float *Vt, *Vs, *Vt1, *Vs1;
   unsigned *rt, *rs;
   float *Ex, *Qx;
   float4 *Ac, *Bc;
@@ -332,6 +333,10 @@ __global__ __launch_bounds__(256) void mix_bounds_kernel(
     nrm = (tgt ? p.XA2 : p.XB2)[xi];
   }
   const int own = valid ? (int)keys[pos] : 0;
+  // the block's first bucket: its points there make part 0 of the bounds, the
+  // others part 1
+  const int fbk = b0 * 16 < cnt ? (int)keys[b0 * 16] : 0;
+  const bool part0 = own == fbk;
   double a[NS];
   double n2 = 0.0;
 #pragma unroll
@@ -373,7 +378,7 @@ __global__ __launch_bounds__(256) void mix_bounds_kernel(
         acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv[st * 64], a[st], acc[t], 0, 0, 0);
     }
     // acc[t][v]: pivot 16 (h NH + t) + 4 g + v, point p16
-    float best[NH][4];
+    float best[NH][4], best1[NH][4];
 #pragma unroll
     for (int t = 0; t < NH; ++t)
 #pragma unroll
@@ -381,7 +386,9 @@ __global__ __launch_bounds__(256) void mix_bounds_kernel(
         const int i = 16 * (h * NH + t) + 4 * g + v;
         const double pni = pn64[i];
         const double diff = (pni - pno) - 2.0 * (acc[t][v] - dsel);  // e_i(x) - e_own(x)
-        best[t][v] = valid ? (float)diff - mu : INFINITY;
+        const float bv = valid ? (float)diff - mu : INFINITY;
+        best[t][v] = part0 ? bv : INFINITY;
+        best1[t][v] = part0 ? INFINITY : bv;
         if (tabs) {
           // e_i(x) = |x - p_i|^2 with the caller's norm in place of |x|^2: the
           // tables then carry the reference's d^2 = XA2 + XB2 - 2 a.b exactly
@@ -391,17 +398,22 @@ __global__ __launch_bounds__(256) void mix_bounds_kernel(
           etab[(long)i * estride + pos] = valid ? (float)(xs * e) : 0.0f;
         }
       }
-    // block minima over the 16 points
+    // block minima over the 16 points, per part
 #pragma unroll
     for (int t = 0; t < NH; ++t)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        float b = best[t][v];
+        float b = best[t][v], c = best1[t][v];
         b = fminf(b, __shfl_xor(b, 1));
+        c = fminf(c, __shfl_xor(c, 1));
         b = fminf(b, __shfl_xor(b, 2));
+        c = fminf(c, __shfl_xor(c, 2));
         b = fminf(b, __shfl_xor(b, 4));
+        c = fminf(c, __shfl_xor(c, 4));
         b = fminf(b, __shfl_xor(b, 8));
+        c = fminf(c, __shfl_xor(c, 8));
         best[t][v] = b;
+        best1[t][v] = c;
       }
     if (p16 == 0) {
 #pragma unroll
@@ -410,9 +422,14 @@ __global__ __launch_bounds__(256) void mix_bounds_kernel(
         if (tgt) {
           *reinterpret_cast<float4 *>(&o.Vt[(long)b0 * kPivots + i0]) =
               make_float4(best[t][0], best[t][1], best[t][2], best[t][3]);
+          *reinterpret_cast<float4 *>(&o.Vt1[(long)b0 * kPivots + i0]) =
+              make_float4(best1[t][0], best1[t][1], best1[t][2], best1[t][3]);
         } else {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) o.Vs[(long)(i0 + v) * nsb + b0] = best[t][v];
+          for (int v = 0; v < 4; ++v) {
+            o.Vs[(long)(i0 + v) * nsb + b0] = best[t][v];
+            o.Vs1[(long)(i0 + v) * nsb + b0] = best1[t][v];
+          }
         }
       }
     }
@@ -480,22 +497,28 @@ __global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ng
   const int j0 = split * cps, j1 = min(n, j0 + cps);
   const int nbs = j0 < j1 ? (j1 - j0 + 15) >> 4 : 0;
   unsigned long long *out = need + w * ngw;
+  // Each block is taken as two parts: its points in its first bucket (bucket
+  // range [f, f], bounds V) and the others ([f + 1, l], bounds V1).  For parts
+  // with bucket ranges RA (targets) and RB (sources), every pair obeys
+  //   d^2 >= min_{B in RB} V[T][B] + min_{A in RA} V[S][A] - max_{A, B} pp[A][B]
+  // (the pivot-hyperplane bound of each pair's own pivots); a block pair is
+  // kept when the bound of any of its part pairs fails to prove the cutoff.
   int a0[2], a1[2];
-  float vt_lo[2], vt_hi[2];
-  bool simple = true;  // both row blocks lie in one bucket each
+  float vt_lo[2][2], vt_hi[2][2];  // rows V[T] and V1[T] of the two row blocks
+  float pp_lo[2], pp_hi[2];        // pp rows of their first buckets
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb) {
     const int T = 2 * g + rb;
     const unsigned rtv = b.rt[T];
     a0[rb] = (int)(rtv & 0xFFFFu);
     a1[rb] = (int)(rtv >> 16);
-    simple = simple && a0[rb] == a1[rb];
-    vt_lo[rb] = b.Vt[(long)T * kPivots + lane];
-    vt_hi[rb] = b.Vt[(long)T * kPivots + 64 + lane];
+    vt_lo[rb][0] = b.Vt[(long)T * kPivots + lane];
+    vt_hi[rb][0] = b.Vt[(long)T * kPivots + 64 + lane];
+    vt_lo[rb][1] = b.Vt1[(long)T * kPivots + lane];
+    vt_hi[rb][1] = b.Vt1[(long)T * kPivots + 64 + lane];
+    pp_lo[rb] = b.pp[a0[rb] * kPivots + lane];
+    pp_hi[rb] = b.pp[a0[rb] * kPivots + 64 + lane];
   }
-  // pp rows of the two row blocks' buckets (the common case)
-  const float pp0_lo = b.pp[a0[0] * kPivots + lane], pp0_hi = b.pp[a0[0] * kPivots + 64 + lane];
-  const float pp1_lo = b.pp[a0[1] * kPivots + lane], pp1_hi = b.pp[a0[1] * kPivots + 64 + lane];
   auto pick = [&](float lo, float hi, int B) {  // entry B of a row held as (lo, hi)
     const float x = __shfl(lo, B & 63), y = __shfl(hi, B & 63);
     return B < 64 ? x : y;
@@ -506,41 +529,40 @@ __global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ng
     const int S = (j0 >> 4) + (in ? sb : 0);
     const unsigned rsv = b.rs[S];
     const int b0 = (int)(rsv & 0xFFFFu), b1 = (int)(rsv >> 16);
-    const float vs0 = b.Vs[(long)a0[0] * b.nsb + S], vs1 = b.Vs[(long)a0[1] * b.nsb + S];
+    const int bn = b0 + 1 < kPivots ? b0 + 1 : kPivots - 1;  // first bucket of part 1
     bool nd = false;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
-      bool keep = true;
-      float tmin = INFINITY, smin = INFINITY, ppmax = 0.0f;
-      if (simple) {
-        // one bucket per row block: shuffles read the rows (every lane takes part)
-        const float pl = rb == 0 ? pp0_lo : pp1_lo, ph = rb == 0 ? pp0_hi : pp1_hi;
-        float t0 = pick(vt_lo[rb], vt_hi[rb], b0), t1 = pick(vt_lo[rb], vt_hi[rb], b1);
-        float q0 = pick(pl, ph, b0), q1 = pick(pl, ph, b1);
-        tmin = fminf(t0, t1);
-        ppmax = fmaxf(q0, q1);
-        smin = rb == 0 ? vs0 : vs1;
-        if (b1 - b0 > 1) {  // buckets strictly inside the block's range: memory
-          for (int B = b0 + 1; B < b1; ++B) {
-            tmin = fminf(tmin, b.Vt[(long)(2 * g + rb) * kPivots + B]);
-            ppmax = fmaxf(ppmax, b.pp[a0[rb] * kPivots + B]);
+      const int T = 2 * g + rb;
+      bool keep = b1 - b0 >= 4 || a1[rb] - a0[rb] >= 4;
+      const float q_b0 = pick(pp_lo[rb], pp_hi[rb], b0), q_bn = pick(pp_lo[rb], pp_hi[rb], bn);
+      const int npa = a1[rb] > a0[rb] ? 2 : 1;  // wave-uniform
+      for (int pa = 0; pa < npa; ++pa) {
+        const float *vt = pa ? b.Vt1 : b.Vt;
+        const int alo = pa ? a0[rb] + 1 : a0[rb], ahi = pa ? a1[rb] : a0[rb];
+        // shuffles first: every lane takes part
+        const float t_b0 = pick(vt_lo[rb][pa], vt_hi[rb][pa], b0);
+        const float t_bn = pick(vt_lo[rb][pa], vt_hi[rb][pa], bn);
+        for (int pb = 0; pb < 2; ++pb) {
+          if (pb == 1 && b1 == b0) break;  // a source block of one bucket
+          const float *vs = pb ? b.Vs1 : b.Vs;
+          const int blo = pb ? b0 + 1 : b0, bhi = pb ? b1 : b0;
+          float tmin = pb ? t_bn : t_b0;
+          for (int B = blo + 1; B <= bhi; ++B) tmin = fminf(tmin, vt[(long)T * kPivots + B]);
+          float smin = INFINITY, ppmax = 0.0f;
+          for (int A = alo; A <= ahi; ++A) {
+            smin = fminf(smin, vs[(long)A * b.nsb + S]);
+            for (int B = blo; B <= bhi; ++B) {
+              const float q = (A == a0[rb] && B == blo) ? (pb ? q_bn : q_b0)
+                                                         : b.pp[A * kPivots + B];
+              ppmax = fmaxf(ppmax, q);
+            }
           }
+          if (tmin == INFINITY || smin == INFINITY) continue;  // a part with no points
+          const float L = (tmin + smin) - ppmax;
+          const float err = 0x1p-22f * (fabsf(tmin) + fabsf(smin) + ppmax);
+          keep = keep || !(L - err > b.thr_d2);  // NaN keeps the block
         }
-      } else {
-        for (int B = b0; B <= b1; ++B) tmin = fminf(tmin, b.Vt[(long)(2 * g + rb) * kPivots + B]);
-        for (int A = a0[rb]; A <= a1[rb]; ++A) {
-          smin = fminf(smin, b.Vs[(long)A * b.nsb + S]);
-          for (int B = b0; B <= b1; ++B) ppmax = fmaxf(ppmax, b.pp[A * kPivots + B]);
-        }
-      }
-      if (b1 - b0 >= 4 || a1[rb] - a0[rb] >= 4) {
-        keep = true;
-      } else if (tmin == INFINITY || smin == INFINITY) {
-        keep = false;  // a block of padding points only: no pairs
-      } else {
-        const float L = (tmin + smin) - ppmax;
-        const float err = 0x1p-22f * (fabsf(tmin) + fabsf(smin) + ppmax);
-        keep = !(L - err > b.thr_d2);  // NaN keeps the block
       }
       nd = nd || keep;
     }
@@ -870,7 +892,7 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
   // v_exp_f32 per pair.  With u = 2^-24, the exponent's error (log2 units) is
   // at most
   //   u (2 (|Ex| + |Qx|) + (kp + 4) |m2xs| A' |beta|),
-  // A' the largest |alpha| of the wave's rows: the roundings of Ex and Qx
+  // A' the largest |alpha| of the lane's four rows: the roundings of Ex and Qx
   // (half an ulp each, FP64 values), of their sum and of the FMA (|x2| <=
   // |Ex| + |Qx| + |m2xs c|), of m2xs and of c0 + c1, and the cross term's
   // (coordinates rounded to FP32, sums of kp / 2 products per chain), with
@@ -887,7 +909,9 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
     int bucket;   // its single bucket, -1 if it spans several
   };
   int abk[RB];      // each row block's single bucket, -1 if it spans several
-  float aA = 0.0f;  // largest |alpha| of the wave's rows
+  float aA[RB];  // largest |alpha| of the lane's four rows in each row block
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) aA[rb] = 0.0f;
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) abk[rb] = -1;
   // Ex[B][rows of each row block] for the last source bucket B seen (runs of
@@ -926,19 +950,21 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
         const unsigned r = pk.rt[(row0 >> 4) + rb];
         abk[rb] = (r & 0xFFFFu) == (r >> 16) ? (int)(r & 0xFFFFu) : -1;
       }
-      aA = pk.na[row0 + (lane & (WROWS - 1))];
 #pragma unroll
-      for (int o = 1; o < WROWS; o <<= 1) aA = fmaxf(aA, __shfl_xor(aA, o));
+      for (int rb = 0; rb < RB; ++rb) {
+        const float4 na4 = *reinterpret_cast<const float4 *>(pk.na + row0 + rb * 16 + 4 * g);
+        aA[rb] = fmaxf(fmaxf(na4.x, na4.y), fmaxf(na4.z, na4.w));
+      }
     }
   }
-  const float kCross = (32.0f * NC + 4.0f) * fabsf(m2xs) * aA;  // times |beta|
+  const float kCross = (32.0f * NC + 4.0f) * fabsf(m2xs);  // times |alpha| |beta|
   constexpr float kBnd = 0x1p-24f * 1.01f;
   // Row block rb against the source block of cf, every pair (near or far):
   // false (nothing added) when the bound hands the block back.
   auto near32 = [&](int rb, const C32Frag &cf) -> bool {
     const float4 &cb = cf.cb;
     const float q = cf.q[rb];
-    const float bnd = kBnd * (2.0f * (emax[rb] + fabsf(q)) + kCross * cb.y);
+    const float bnd = kBnd * (2.0f * (emax[rb] + fabsf(q)) + kCross * aA[rb] * cb.y);
     if (__ballot(bnd > 0x1p-15f) != 0ull) return false;  // wave-uniform
     f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
     const float4 *sa = &s_ac[wave][rb * NC4 * 64 + lane];
@@ -992,12 +1018,14 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
 
   int nblk = 0;  // source blocks since the last flush of the FP32 partials
   // The skip-tier kernel's FP64 fallback (blocks the centred tier hands back):
-  // one 16-coordinate chunk of the FP64 source fragment in registers at a time.
+  // two 16-coordinate chunks of the FP64 source fragment in registers at a
+  // time (their loads in flight together; one at a time waited on four round
+  // trips per block).
   auto near64_lean = [&](int rb, int jb) {
     d4 acc = {0.0, 0.0, 0.0, 0.0};
     const double *ar = pk.Ad + (long)(row0 + rb * 16 + f64_a_row(p16)) * (32 * NC) + 4 * g;
     const double4 *src = pk.Bd + (long)(jb >> 4) * KC * 64 + lane;
-#pragma unroll 1
+#pragma unroll 2
     for (int cc = 0; cc < KC; ++cc) {
       const double4 bq = src[cc * 64];
       const double2 a01 = *reinterpret_cast<const double2 *>(ar + cc * 16);
@@ -1449,6 +1477,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   const size_t o_nb = off; off += bounds ? al(np * sizeof(float)) : 0;
   const size_t o_vt = off; off += bounds ? al((size_t)P * ntb * sizeof(float)) : 0;
   const size_t o_vs = off; off += bounds ? al((size_t)P * nsb * sizeof(float)) : 0;
+  const size_t o_vt1 = off; off += bounds ? al((size_t)P * ntb * sizeof(float)) : 0;
+  const size_t o_vs1 = off; off += bounds ? al((size_t)P * nsb * sizeof(float)) : 0;
   const size_t o_rt = off; off += bounds ? al(ntb * sizeof(unsigned)) : 0;
   const size_t o_rs = off; off += bounds ? al(nsb * sizeof(unsigned)) : 0;
   const size_t o_need = off;
@@ -1524,6 +1554,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
       MixTabs tb;
       tb.Vt = (float *)(base + o_vt);
       tb.Vs = (float *)(base + o_vs);
+      tb.Vt1 = (float *)(base + o_vt1);
+      tb.Vs1 = (float *)(base + o_vs1);
       tb.rt = (unsigned *)(base + o_rt);
       tb.rs = (unsigned *)(base + o_rs);
       tb.Ex = (float *)(base + o_ex);
@@ -1557,6 +1589,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
       MixBounds mb;
       mb.Vt = Vt;
       mb.Vs = Vs;
+      mb.Vt1 = tb.Vt1;
+      mb.Vs1 = tb.Vs1;
       mb.pp = pp;
       mb.rt = rt;
       mb.rs = rs;
