@@ -530,6 +530,21 @@ __global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ng
     const unsigned rsv = b.rs[S];
     const int b0 = (int)(rsv & 0xFFFFu), b1 = (int)(rsv >> 16);
     const int bn = b0 + 1 < kPivots ? b0 + 1 : kPivots - 1;  // first bucket of part 1
+    // the source block's two bound columns at the row blocks' first buckets,
+    // every load issued before the first use
+    float vs0[2], vs1[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      vs0[rb] = b.Vs[(long)a0[rb] * b.nsb + S];
+      vs1[rb] = b1 > b0 ? b.Vs1[(long)a0[rb] * b.nsb + S] : INFINITY;
+    }
+    // one part pair: false where its bound proves every pair negligible
+    auto part_keeps = [&](float tmin, float smin, float ppmax) {
+      if (tmin == INFINITY || smin == INFINITY) return false;  // a part with no points
+      const float L = (tmin + smin) - ppmax;
+      const float err = 0x1p-22f * (fabsf(tmin) + fabsf(smin) + ppmax);
+      return !(L - err > b.thr_d2);  // NaN keeps the block
+    };
     bool nd = false;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
@@ -539,29 +554,37 @@ __global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ng
       const int npa = a1[rb] > a0[rb] ? 2 : 1;  // wave-uniform
       for (int pa = 0; pa < npa; ++pa) {
         const float *vt = pa ? b.Vt1 : b.Vt;
-        const int alo = pa ? a0[rb] + 1 : a0[rb], ahi = pa ? a1[rb] : a0[rb];
         // shuffles first: every lane takes part
         const float t_b0 = pick(vt_lo[rb][pa], vt_hi[rb][pa], b0);
         const float t_bn = pick(vt_lo[rb][pa], vt_hi[rb][pa], bn);
-        for (int pb = 0; pb < 2; ++pb) {
-          if (pb == 1 && b1 == b0) break;  // a source block of one bucket
-          const float *vs = pb ? b.Vs1 : b.Vs;
-          const int blo = pb ? b0 + 1 : b0, bhi = pb ? b1 : b0;
-          float tmin = pb ? t_bn : t_b0;
-          for (int B = blo + 1; B <= bhi; ++B) tmin = fminf(tmin, vt[(long)T * kPivots + B]);
-          float smin = INFINITY, ppmax = 0.0f;
-          for (int A = alo; A <= ahi; ++A) {
-            smin = fminf(smin, vs[(long)A * b.nsb + S]);
-            for (int B = blo; B <= bhi; ++B) {
-              const float q = (A == a0[rb] && B == blo) ? (pb ? q_bn : q_b0)
-                                                         : b.pp[A * kPivots + B];
-              ppmax = fmaxf(ppmax, q);
+        if (pa == 0) {
+          // the row block's first bucket against the source's first bucket, and
+          // against its other buckets (one more bucket: registers; more: memory)
+          keep = keep || part_keeps(t_b0, vs0[rb], q_b0);
+          if (b1 == b0 + 1) {
+            keep = keep || part_keeps(t_bn, vs1[rb], q_bn);
+          } else if (b1 > b0 + 1) {
+            float tmin = t_bn, ppmax = q_bn;
+            for (int B = b0 + 2; B <= b1; ++B) {
+              tmin = fminf(tmin, vt[(long)T * kPivots + B]);
+              ppmax = fmaxf(ppmax, b.pp[a0[rb] * kPivots + B]);
             }
+            keep = keep || part_keeps(tmin, vs1[rb], ppmax);
           }
-          if (tmin == INFINITY || smin == INFINITY) continue;  // a part with no points
-          const float L = (tmin + smin) - ppmax;
-          const float err = 0x1p-22f * (fabsf(tmin) + fabsf(smin) + ppmax);
-          keep = keep || !(L - err > b.thr_d2);  // NaN keeps the block
+        } else {
+          // the row block's other buckets (a block over buckets: rare), from memory
+          for (int pb = 0; pb < 2; ++pb) {
+            if (pb == 1 && b1 == b0) break;  // a source block of one bucket
+            const float *vs = pb ? b.Vs1 : b.Vs;
+            const int blo = pb ? b0 + 1 : b0, bhi = pb ? b1 : b0;
+            float tmin = pb ? t_bn : t_b0, smin = INFINITY, ppmax = 0.0f;
+            for (int B = blo + 1; B <= bhi; ++B) tmin = fminf(tmin, vt[(long)T * kPivots + B]);
+            for (int A = a0[rb] + 1; A <= a1[rb]; ++A) {
+              smin = fminf(smin, vs[(long)A * b.nsb + S]);
+              for (int B = blo; B <= bhi; ++B) ppmax = fmaxf(ppmax, b.pp[A * kPivots + B]);
+            }
+            keep = keep || part_keeps(tmin, smin, ppmax);
+          }
         }
       }
       nd = nd || keep;
