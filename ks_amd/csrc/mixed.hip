@@ -63,20 +63,31 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 constexpr float kFarT = 16.0f;          // far: estimated x < -16 (natural-log units)
 constexpr float kSkipT2 = -50.0f;       // skipped: every exponent < -50 (log2 units)
 constexpr float kLog2e = 1.4426950408889634f;
-constexpr int XWG = 256;                // 4 waves
 constexpr int kPivots = 128;            // bucketing pivots (at most)
 constexpr int kSortMin = 16384;         // bucket only when m and n are both at least this
 constexpr int kXcds = 8;                // L2 domains; workgroups go to them round robin
 constexpr long kSliceHiBytes = 2l << 20;  // BF16 hi halves of one source slice: 2 MiB
                                         // (its sorts cost ~15 launches per call)
 constexpr int kMixRB = 2;               // 16-row blocks per wave
-constexpr int kMixRowsPerWG = 4 * kMixRB * 16;
 
 // The skip tier runs for the BF16x3 Gaussian only: with a bandwidth per point,
 // one small hi hj in a 16 x 16 block keeps it from being skipped, and the
 // two-stage test then only adds work.
 template <int PREC, int TYPE>
 constexpr bool kSkipTier = PREC == KS_PREC_MIXED_BF16X3 && TYPE == GAUSSIAN;
+
+// Waves per workgroup of the main kernel.  The skip-tier kernel takes one: its
+// waves' work differs widely (their rows' clusters in the slice or not), and a
+// workgroup holds its CU slot until its slowest wave ends (four waves per
+// workgroup: 2.85 ms against 2.35 at mixture_k64).  The others take four: their
+// waves walk the same source blocks at the same pace and share them through L1
+// (one wave per workgroup made the variable bandwidth 5% slower).
+constexpr int mix_waves(int precision, int type) {
+  return precision == KS_PREC_MIXED_BF16X3 && type == GAUSSIAN ? 1 : 4;
+}
+constexpr int mix_rows_per_wg(int precision, int type) {
+  return mix_waves(precision, type) * kMixRB * 16;
+}
 
 // ------------------------------------------------------------------ pre-pass
 
@@ -474,15 +485,15 @@ __global__ __launch_bounds__(256) void mix_bounds_kernel(
 // complement of the workgroup's needed-block count, so an ascending sort puts
 // every slice's heaviest row blocks first.
 __global__ __launch_bounds__(256) void mix_order_kernel(const unsigned long long *need, int ngw,
-                                                       int nsplit, int row_blocks,
+                                                       int nsplit, int row_blocks, int waves,
                                                        unsigned *key, int *val) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= nsplit * row_blocks) return;
   const int split = t / row_blocks, rb = t - split * row_blocks;
   unsigned work = 0;
-  for (int w = 0; w < 4; ++w)
+  for (int w = 0; w < waves; ++w)
     for (int gi = 0; gi < ngw; ++gi)
-      work += __popcll(need[((long)(rb * 4 + w) * nsplit + split) * ngw + gi]);
+      work += __popcll(need[((long)(rb * waves + w) * nsplit + split) * ngw + gi]);
   key[t] = ((unsigned)split << 20) | (0xFFFFFu - (work < 0xFFFFFu ? work : 0xFFFFFu));
   val[t] = rb;
 }
@@ -774,7 +785,7 @@ constexpr int kFlush = 8;  // far-pair FP32 partials go to FP64 every 8 source b
 
 
 template <int PREC, int TYPE, int NC>
-__global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A, MixPack pk,
+__global__ __launch_bounds__(64 * mix_waves(PREC, TYPE), NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A, MixPack pk,
                                                                   float xs) {
   constexpr int RB = kMixRB;
   constexpr int KC = 2 * NC;  // 16-coordinate chunks of the FP64 operand
@@ -791,12 +802,14 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
   // Within a slice the row blocks go heaviest first (pk.order, by their
   // needed-block counts): the few row blocks whose clusters lie in the slice
   // hold most of its work, and started last they would set the launch's end.
-  const int row_blocks = (m + 4 * RB * 16 - 1) / (4 * RB * 16);
+  constexpr int WAVES = mix_waves(PREC, TYPE);
+  constexpr int RPW = mix_rows_per_wg(PREC, TYPE);
+  const int row_blocks = (m + RPW - 1) / RPW;
   const int xcd = (int)(blockIdx.x % kXcds), loc = (int)(blockIdx.x / kXcds);
   const int split = xcd + kXcds * (loc / row_blocks);
   const int rblk = pk.order ? pk.order[split * row_blocks + loc % row_blocks] : loc % row_blocks;
   constexpr int WROWS = RB * 16;
-  const int wg_rows = 4 * WROWS;
+  const int wg_rows = WAVES * WROWS;
   const int row0 = rblk * wg_rows + wave * WROWS;  // sorted target position
 
   // The skip-tier kernel (BF16x3 Gaussian) keeps the wave's targets centred
@@ -808,11 +821,11 @@ __global__ __launch_bounds__(XWG, NC <= 2 ? 2 : 1) void mixed_kernel(FusedArgs A
   constexpr bool SK = kSkipTier<PREC, TYPE>;
   constexpr int KS = 32 * NC + 2;
   constexpr int NC4 = 2 * NC;  // float4 chunks of a centred operand fragment
-  __shared__ __align__(16) double s_rows[4][SK ? 2 : WROWS * KS];
-  __shared__ __align__(16) float4 s_ac[4][SK ? RB * NC4 * 64 : 1];
-  __shared__ double s_tot[4][WROWS];
+  __shared__ __align__(16) double s_rows[WAVES][SK ? 2 : WROWS * KS];
+  __shared__ __align__(16) float4 s_ac[WAVES][SK ? RB * NC4 * 64 : 1];
+  __shared__ double s_tot[WAVES][WROWS];
   // FP64 row factors {|a|^2, -hi/2}, read by near blocks only
-  __shared__ double2 s_rd[4][WROWS];
+  __shared__ double2 s_rd[WAVES][WROWS];
   double *arows = s_rows[wave];
   if constexpr (SK) {
     if (pk.Ex) {
@@ -1362,7 +1375,8 @@ template <int PREC, int TYPE, int NC>
 static hipError_t mixed_go(const FusedArgs &A, const MixPack &pk, float xs, int row_blocks,
                            hipStream_t s) {
   dim3 grid((unsigned)row_blocks * (unsigned)A.nsplit);
-  hipLaunchKernelGGL((mixed_kernel<PREC, TYPE, NC>), grid, dim3(XWG), 0, s, A, pk, xs);
+  hipLaunchKernelGGL((mixed_kernel<PREC, TYPE, NC>), grid, dim3(64 * mix_waves(PREC, TYPE)), 0, s,
+                     A, pk, xs);
   if (A.nsplit > 1)
     return launch_split_reduce(A.partial, A.nsplit, A.p.m, A.p.u, A.p.umap, pk.prow, s);
   return hipGetLastError();
@@ -1389,7 +1403,8 @@ static hipError_t mixed_launch(const Problem &p, const FusedPlan &plan, double *
   A.partial = partial;
   A.vec2 = 0;
   const float xs = (float)(p.P.scal * (double)kLog2e);
-  const long mp = (long)plan.row_blocks * kMixRowsPerWG, np = ((long)p.n + 15) / 16 * 16;
+  const long mp = (long)plan.row_blocks * mix_rows_per_wg(PREC, p.type),
+             np = ((long)p.n + 15) / 16 * 16;
   dim3 pg((unsigned)((mp + np + 255) / 256));
   const int kp = 32 * nc;
   hipLaunchKernelGGL(mix_rows_kernel, dim3((unsigned)((mp * (kp / 2) + 255) / 256)), dim3(256), 0,
@@ -1433,7 +1448,7 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   if (nc > 4)
     return raise_error(KS_ERR_INVALID_ARG, false,
                        "dgsks_device(): mixed precision supports k <= 128\n");
-  FusedPlan plan = plan_split(p.m, p.n, kMixRowsPerWG, 2048, 128);
+  FusedPlan plan = plan_split(p.m, p.n, mix_rows_per_wg(precision, p.type), 2048, 128);
   // Source slices: a multiple of the 8 XCDs (the kernel gives XCD x the slices
   // x, x + 8, ...), each with at most kSliceHiBytes of packed hi halves so that
   // one slice stays in an XCD's L2, and at least as many as plan_split wants
@@ -1453,7 +1468,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
 
   // scratch: partial sums | packed operands and factors (rows padded to whole
   // workgroups, sources to 16-point blocks) | orders | bucket keys | sort space
-  const size_t mp = (size_t)plan.row_blocks * kMixRowsPerWG, np = ((size_t)p.n + 15) / 16 * 16;
+  const size_t mp = (size_t)plan.row_blocks * mix_rows_per_wg(precision, p.type),
+               np = ((size_t)p.n + 15) / 16 * 16;
   const size_t mx = (size_t)(p.m > p.n ? p.m : p.n);
   size_t sort_bytes = 0;
   if (bucket)
@@ -1634,7 +1650,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
       unsigned *ok0 = (unsigned *)(base + o_ord), *ok1 = ok0 + nord;
       int *ov0 = (int *)(ok1 + nord), *ov1 = ov0 + nord;
       hipLaunchKernelGGL(mix_order_kernel, dim3((unsigned)((nord + 255) / 256)), dim3(256), 0, s,
-                         need, (int)ngw, plan.nsplit, plan.row_blocks, ok0, ov0);
+                         need, (int)ngw, plan.nsplit, plan.row_blocks,
+                         mix_waves(precision, p.type), ok0, ov0);
       size_t obytes = sort_bytes;
       if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, obytes, ok0, ok1, ov0, ov1, (int)nord, 0,
                                              32, s) != hipSuccess)
