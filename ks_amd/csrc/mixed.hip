@@ -125,7 +125,7 @@ struct MixPack {
   // null otherwise): the MixTabs tables and operands, and the bucket range of
   // every 16-point target / source block (first | last << 16).
   const float *Ex, *Qx;
-  const float4 *Ac, *Bc;
+  const uint4 *Ac, *Bc;
   const float *na, *nb;
   const unsigned *rt, *rs;
   long mp, np;
@@ -283,11 +283,13 @@ __global__ void mix_pivot_gram_kernel(int P, int kp, const float *piv, float *pp
 //   Qx[i][j] = xs (e_i(b_j) - |p_i - p_own(j)|^2)  (pivot i, sorted source j)
 //            with e_i(x) = N(x) - 2 x.p_i + |p_i|^2 and N the caller's squared
 //            norm (XA2 / XB2), i.e. |x - p_i|^2 when the norms are exact;
-//   Ac, Bc   the coordinates centred on the point's own pivot, x - p_own, in
-//            FP32 and in the lane order of the v_mfma_f32_16x16x4f32 operands:
-//            block b, chunk c (4 steps of 4 coordinates), lane 16 g + p holding
-//            {x[16c + g], x[16c + 4 + g], x[16c + 8 + g], x[16c + 12 + g]} of
-//            point 16 b + p (targets and sources alike);
+//   Ac, Bc   the coordinates centred on the point's own pivot, x - p_own,
+//            each split from the FP64 difference into three BF16 parts
+//            (h + m + l within 2^-24 of it), in the lane order of the
+//            v_mfma_f32_16x16x32_bf16 operands: block b, chunk cc (32
+//            coordinates), part, lane 16 g + p holding coordinates
+//            32 cc + 8 g .. 32 cc + 8 g + 7 of point 16 b + p (targets and
+//            sources alike);
 //   na, nb   |x - p_own| rounded up, per sorted target / source.
 // xs = scal log2(e) (FP64); the tables are FP64 values rounded once to FP32.
 // For target t in bucket A and source j in bucket B, with a = p_A + alpha and
@@ -300,7 +302,7 @@ struct MixTabs {
   float *Vt, *Vs, *Vt1, *Vs1;
   unsigned *rt, *rs;
   float *Ex, *Qx;
-  float4 *Ac, *Bc;
+  uint4 *Ac, *Bc;
   float *na, *nb;
   long mp, np;  // row strides of Ex (padded targets) and Qx (padded sources)
 };
@@ -453,19 +455,36 @@ __global__ __launch_bounds__(256) void mix_bounds_kernel(
   if (tabs) {
     // the centred coordinates x - p_own (exact FP64 differences, rounded once)
     double c2 = 0.0;
-    float cf[NS];
 #pragma unroll
     for (int st = 0; st < NS; ++st) {
       const double d = valid ? a[st] - (double)piv[own * KP + 4 * st + g] : 0.0;
       c2 = fma(d, d, c2);
-      cf[st] = (float)d;
     }
     c2 += __shfl_xor(c2, 16);
     c2 += __shfl_xor(c2, 32);
-    float4 *dst = (tgt ? o.Ac : o.Bc) + (long)b0 * (NS / 4) * 64 + lane;
+    // the same differences, split into three BF16 parts (h + m + l within
+    // 2^-24 of each), in the lane order of the v_mfma_f32_16x16x32_bf16
+    // operands: lane 16 g + p holds coordinates 32 cc + 8 g .. + 7 of point p
+    uint4 *dst = (tgt ? o.Ac : o.Bc) + (long)b0 * (KP / 32) * 3 * 64 + lane;
 #pragma unroll
-    for (int c = 0; c < NS / 4; ++c)
-      dst[c * 64] = make_float4(cf[4 * c], cf[4 * c + 1], cf[4 * c + 2], cf[4 * c + 3]);
+    for (int cc = 0; cc < KP / 32; ++cc) {
+      bf8 ph, pm, pl;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int q = 32 * cc + 8 * g + t;
+        const double d = (valid && q < p.k) ? x[q] - (double)piv[own * KP + q] : 0.0;
+        const __bf16 h = (__bf16)(float)d;
+        const double r1 = d - (double)(float)h;
+        const __bf16 mm = (__bf16)(float)r1;
+        const double r2 = r1 - (double)(float)mm;
+        ph[t] = h;
+        pm[t] = mm;
+        pl[t] = (__bf16)(float)r2;
+      }
+      dst[(cc * 3 + 0) * 64] = *reinterpret_cast<uint4 *>(&ph);
+      dst[(cc * 3 + 1) * 64] = *reinterpret_cast<uint4 *>(&pm);
+      dst[(cc * 3 + 2) * 64] = *reinterpret_cast<uint4 *>(&pl);
+    }
     if (g == 0) (tgt ? o.na : o.nb)[pos] = (float)sqrt(c2) * (1.0f + 0x1p-20f);
   }
 }
@@ -822,7 +841,7 @@ __global__ __launch_bounds__(64 * mix_waves(PREC, TYPE), NC <= 2 ? 2 : 1) void m
   constexpr int KS = 32 * NC + 2;
   constexpr int NC4 = 2 * NC;  // float4 chunks of a centred operand fragment
   __shared__ __align__(16) double s_rows[WAVES][SK ? 2 : WROWS * KS];
-  __shared__ __align__(16) float4 s_ac[WAVES][SK ? RB * NC4 * 64 : 1];
+  __shared__ __align__(16) uint4 s_ac[WAVES][SK ? RB * NC * 3 * 64 : 1];
   __shared__ double s_tot[WAVES][WROWS];
   // FP64 row factors {|a|^2, -hi/2}, read by near blocks only
   __shared__ double2 s_rd[WAVES][WROWS];
@@ -832,9 +851,9 @@ __global__ __launch_bounds__(64 * mix_waves(PREC, TYPE), NC <= 2 ? 2 : 1) void m
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-        for (int c = 0; c < NC4; ++c)
-          s_ac[wave][(rb * NC4 + c) * 64 + lane] =
-              pk.Ac[((long)((row0 >> 4) + rb) * NC4 + c) * 64 + lane];
+        for (int c = 0; c < NC * 3; ++c)
+          s_ac[wave][(rb * NC * 3 + c) * 64 + lane] =
+              pk.Ac[((long)((row0 >> 4) + rb) * NC * 3 + c) * 64 + lane];
     }
   } else {
     // the wave's rows are contiguous in the sorted FP64 copy: every load is
@@ -923,22 +942,26 @@ __global__ __launch_bounds__(64 * mix_waves(PREC, TYPE), NC <= 2 ? 2 : 1) void m
   // ---- the centred near tier (skip-tier kernel with pk.Ex)
   // A block pair (T in bucket A, S in bucket B, one bucket each) takes its
   // exponents as x2 = fl(Ex[B][t] + Qx[A][j]) + m2xs c (one FMA), with c the
-  // cross term alpha.beta of the centred coordinates on the FP32 matrix cores
-  // (16 steps of v_mfma_f32_16x16x4f32 at k = 64, in two chains), then one
+  // cross term alpha.beta of the centred coordinates on the BF16 matrix cores:
+  // with alpha and beta in three BF16 parts each, the products h.h in one FP32
+  // accumulator (2 v_mfma_f32_16x16x32_bf16 at k = 64) and h.m, m.h, h.l, m.m,
+  // l.h in another (10), so the large terms' sums see no small ones; then one
   // v_exp_f32 per pair.  With u = 2^-24, the exponent's error (log2 units) is
   // at most
-  //   u (2 (|Ex| + |Qx|) + (kp + 4) |m2xs| A' |beta|),
+  //   u (2 (|Ex| + |Qx|) + (kp + 10) |m2xs| A' |beta|),
   // A' the largest |alpha| of the lane's four rows: the roundings of Ex and Qx
   // (half an ulp each, FP64 values), of their sum and of the FMA (|x2| <=
-  // |Ex| + |Qx| + |m2xs c|), of m2xs and of c0 + c1, and the cross term's
-  // (coordinates rounded to FP32, sums of kp / 2 products per chain), with
-  // |c| <= A' |beta| (1 + (kp + 2) u).  The bound is checked before the
+  // |Ex| + |Qx| + |m2xs c|), of m2xs and of c0 + c1, and the cross term's: the
+  // splits (2u), the dropped products m.l, l.m, l.l (2u), the FP32 sums of the
+  // kp exact products h.h (kp u) and of the 5 kp small ones (together at most
+  // 2^-7 A' |beta|: 5 kp 2^-7 u <= 5u for kp <= 128), with
+  // |c| <= A' |beta| (1 + (kp + 10) u).  The bound is checked before the
   // products; a block where any lane's bound exceeds 2^-15 -- or whose target
   // or source block spans buckets -- takes the split-BF16 estimate and the FP64
   // near tier instead, so every kernel value of this tier is within
   // ln2 2^-15 + 2^-22 (v_exp_f32) relative.
   struct C32Frag {
-    float4 b[NC4];
+    uint4 b[NC * 3];  // the three BF16 parts of each 32-coordinate chunk
     float4 cb;    // the lane's source record {xs |b|^2, |b - p_own|, w, |b|}
     float q[RB];  // Qx[A][j] of the lane's source for each row block's bucket A
     int block;    // the source block it holds (or has in flight), -1 for none
@@ -957,9 +980,9 @@ __global__ __launch_bounds__(64 * mix_waves(PREC, TYPE), NC <= 2 ? 2 : 1) void m
   float emax[RB];
   int ebucket = -1;
   auto fetch32 = [&](C32Frag &f, int jb) {
-    const float4 *src = pk.Bc + (long)(jb >> 4) * NC4 * 64 + lane;
+    const uint4 *src = pk.Bc + (long)(jb >> 4) * NC * 3 * 64 + lane;
 #pragma unroll
-    for (int c = 0; c < NC4; ++c) f.b[c] = src[c * 64];
+    for (int c = 0; c < NC * 3; ++c) f.b[c] = src[c * 64];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
       f.q[rb] = abk[rb] >= 0 ? pk.Qx[(long)abk[rb] * pk.np + jb + p16] : 0.0f;
@@ -993,7 +1016,7 @@ __global__ __launch_bounds__(64 * mix_waves(PREC, TYPE), NC <= 2 ? 2 : 1) void m
       }
     }
   }
-  const float kCross = (32.0f * NC + 4.0f) * fabsf(m2xs);  // times |alpha| |beta|
+  const float kCross = (32.0f * NC + 10.0f) * fabsf(m2xs);  // times |alpha| |beta|
   constexpr float kBnd = 0x1p-24f * 1.01f;
   // Row block rb against the source block of cf, every pair (near or far):
   // false (nothing added) when the bound hands the block back.
@@ -1002,20 +1025,22 @@ __global__ __launch_bounds__(64 * mix_waves(PREC, TYPE), NC <= 2 ? 2 : 1) void m
     const float q = cf.q[rb];
     const float bnd = kBnd * (2.0f * (emax[rb] + fabsf(q)) + kCross * aA[rb] * cb.y);
     if (__ballot(bnd > 0x1p-15f) != 0ull) return false;  // wave-uniform
+    // alpha.beta from the BF16 parts: h.h in one accumulator, the five next
+    // products (h.m, m.h, h.l, m.m, l.h; at most 2^-7 |alpha||beta| together)
+    // in another, so the large terms' FP32 sums see no small ones
     f4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
-    const float4 *sa = &s_ac[wave][rb * NC4 * 64 + lane];
+    const uint4 *sa = &s_ac[wave][rb * NC * 3 * 64 + lane];
 #pragma unroll
-    for (int c = 0; c < NC4; c += 2) {
-      const float4 a0 = sa[c * 64], b0 = cf.b[c];
-      c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.x, b0.x, c0, 0, 0, 0);
-      c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.y, b0.y, c0, 0, 0, 0);
-      c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.z, b0.z, c0, 0, 0, 0);
-      c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.w, b0.w, c0, 0, 0, 0);
-      const float4 a1 = sa[(c + 1) * 64], b1 = cf.b[c + 1];
-      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.x, b1.x, c1, 0, 0, 0);
-      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.y, b1.y, c1, 0, 0, 0);
-      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.z, b1.z, c1, 0, 0, 0);
-      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.w, b1.w, c1, 0, 0, 0);
+    for (int cc = 0; cc < NC; ++cc) {
+      const uint4 ah = sa[(cc * 3) * 64], am = sa[(cc * 3 + 1) * 64], al = sa[(cc * 3 + 2) * 64];
+      const uint4 &bh = cf.b[cc * 3], &bm = cf.b[cc * 3 + 1], &bl = cf.b[cc * 3 + 2];
+      auto bfv = [](const uint4 &u) { return *reinterpret_cast<const bf8 *>(&u); };
+      c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfv(ah), bfv(bh), c0, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfv(ah), bfv(bm), c1, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfv(am), bfv(bh), c1, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfv(ah), bfv(bl), c1, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfv(am), bfv(bm), c1, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfv(al), bfv(bh), c1, 0, 0, 0);
     }
     const float4 e = ecache[rb];  // cf.bucket == ebucket
     const float ev[4] = {e.x, e.y, e.z, e.w};
@@ -1510,8 +1535,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   // the centred near tier's tables and operands (MixTabs)
   const size_t o_ex = off; off += bounds ? al((size_t)P * mp * sizeof(float)) : 0;
   const size_t o_qx = off; off += bounds ? al((size_t)P * np * sizeof(float)) : 0;
-  const size_t o_ac = off; off += bounds ? al(mp * kp * sizeof(float)) : 0;
-  const size_t o_bc = off; off += bounds ? al(np * kp * sizeof(float)) : 0;
+  const size_t o_ac = off; off += bounds ? al(mp * kp * 3 * sizeof(__bf16)) : 0;
+  const size_t o_bc = off; off += bounds ? al(np * kp * 3 * sizeof(__bf16)) : 0;
   const size_t o_na = off; off += bounds ? al(mp * sizeof(float)) : 0;
   const size_t o_nb = off; off += bounds ? al(np * sizeof(float)) : 0;
   const size_t o_vt = off; off += bounds ? al((size_t)P * ntb * sizeof(float)) : 0;
@@ -1599,8 +1624,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
       tb.rs = (unsigned *)(base + o_rs);
       tb.Ex = (float *)(base + o_ex);
       tb.Qx = (float *)(base + o_qx);
-      tb.Ac = (float4 *)(base + o_ac);
-      tb.Bc = (float4 *)(base + o_bc);
+      tb.Ac = (uint4 *)(base + o_ac);
+      tb.Bc = (uint4 *)(base + o_bc);
       tb.na = (float *)(base + o_na);
       tb.nb = (float *)(base + o_nb);
       tb.mp = (long)mp;
