@@ -626,12 +626,107 @@ __global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ng
   }
 }
 
-__global__ void mix_identity_kernel(int *v, long n) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] = (int)i;
+// ---- stable counting sort of the bucket keys (keys < kPivots = 2^kKeyBits),
+// the targets' and the sources' in the same three launches: per tile of
+// kSortTile keys a histogram, per segment the tiles' first positions of every
+// key, and per tile one wave placing its keys in order.
+constexpr int kSortTile = 1024;
+constexpr int kKeyBits = 7;
+static_assert(kPivots <= (1 << kKeyBits), "bucket keys take kKeyBits bits");
+
+__global__ __launch_bounds__(256) void mix_hist_kernel(const unsigned *__restrict__ ka, long m,
+                                                       const unsigned *__restrict__ kb, long n,
+                                                       int tiles_a, unsigned *hist) {
+  __shared__ unsigned h[kPivots];
+  const int tile = blockIdx.x;
+  const bool seg_a = tile < tiles_a;
+  const unsigned *k = seg_a ? ka : kb;
+  const long cnt = seg_a ? m : n;
+  const long t0 = (long)(seg_a ? tile : tile - tiles_a) * kSortTile;
+  for (int i = threadIdx.x; i < kPivots; i += 256) h[i] = 0u;
+  __syncthreads();
+  for (int i = threadIdx.x; i < kSortTile; i += 256)
+    if (t0 + i < cnt) atomicAdd(&h[k[t0 + i] & (kPivots - 1)], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kPivots; i += 256) hist[(long)tile * kPivots + i] = h[i];
 }
 
-// Identity target and source orders in one launch.
+// hist[tile][key] becomes the position of the tile's first key `key` in its
+// segment's sorted order: the keys below it in the segment, then the same key
+// in earlier tiles.  Workgroup 0: targets, 1: sources; thread (part, key) sums
+// and then rewrites an eighth of the segment's tiles for its key.
+__global__ __launch_bounds__(1024) void mix_scan_kernel(unsigned *hist, int tiles_a,
+                                                        int tiles_b) {
+  constexpr int kParts = 1024 / kPivots;
+  __shared__ unsigned part[kParts][kPivots];
+  __shared__ unsigned start[kPivots];
+  const int t0 = blockIdx.x ? tiles_a : 0, t1 = blockIdx.x ? tiles_a + tiles_b : tiles_a;
+  const int per = (t1 - t0 + kParts - 1) / kParts;
+  const int key = threadIdx.x % kPivots, pp = threadIdx.x / kPivots;
+  const int a = t0 + pp * per, b = min(t1, a + per);
+  unsigned sum = 0u;
+  for (int t = a; t < b; ++t) sum += hist[(long)t * kPivots + key];
+  part[pp][key] = sum;
+  __syncthreads();
+  if (pp == 0) {  // the key's total, then (below) the keys before it
+    unsigned tot = 0u;
+    for (int q = 0; q < kParts; ++q) tot += part[q][key];
+    start[key] = tot;
+  }
+  __syncthreads();
+  unsigned run = 0u;
+  for (int i = 0; i < key; ++i) run += start[i];
+  for (int q = 0; q < pp; ++q) run += part[q][key];
+  __syncthreads();  // every thread has read start[] before the tiles change
+  for (int t = a; t < b; ++t) {
+    const unsigned c = hist[(long)t * kPivots + key];
+    hist[(long)t * kPivots + key] = run;
+    run += c;
+  }
+}
+
+// One wave per tile, 64 keys at a time in order: a key's position is its
+// tile's running position for that key plus its rank among the equal keys of
+// the round (from kKeyBits ballots), so equal keys keep their order.  Writes
+// the sorted keys and the call index of each.
+__global__ __launch_bounds__(64) void mix_place_kernel(const unsigned *__restrict__ ka, long m,
+                                                       const unsigned *__restrict__ kb, long n,
+                                                       int tiles_a, const unsigned *__restrict__ off,
+                                                       unsigned *outa, int *vala, unsigned *outb,
+                                                       int *valb) {
+  __shared__ unsigned base[kPivots];
+  const int tile = blockIdx.x, lane = threadIdx.x;
+  const bool seg_a = tile < tiles_a;
+  const unsigned *k = seg_a ? ka : kb;
+  const long cnt = seg_a ? m : n;
+  unsigned *out = seg_a ? outa : outb;
+  int *val = seg_a ? vala : valb;
+  const long t0 = (long)(seg_a ? tile : tile - tiles_a) * kSortTile;
+  for (int i = lane; i < kPivots; i += 64) base[i] = off[(long)tile * kPivots + i];
+  __syncthreads();
+  const unsigned long long before = (1ull << lane) - 1ull;
+  for (int r = 0; r < kSortTile / 64; ++r) {
+    const long e = t0 + r * 64 + lane;
+    const bool v = e < cnt;
+    const unsigned key = v ? (k[e] & (kPivots - 1)) : 0u;
+    unsigned long long same = __ballot(v);
+#pragma unroll
+    for (int bit = 0; bit < kKeyBits; ++bit) {
+      const bool set = (key >> bit) & 1u;
+      const unsigned long long bb = __ballot(set);
+      same &= set ? bb : ~bb;
+    }
+    if (v) {
+      const unsigned pos = base[key] + (unsigned)__popcll(same & before);
+      out[pos] = key;
+      val[pos] = (int)e;
+    }
+    // the last lane of each group of equal keys advances that key's position
+    // (one wave: its LDS operations complete in order)
+    if (v && (same >> lane) == 1ull) base[key] += (unsigned)__popcll(same);
+  }
+}
+
 __global__ void mix_identity2_kernel(int *a, long m, int *b, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) a[i] = (int)i;
@@ -1498,11 +1593,13 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   const size_t mp = (size_t)plan.row_blocks * mix_rows_per_wg(precision, p.type),
                np = ((size_t)p.n + 15) / 16 * 16;
   const size_t mx = (size_t)(p.m > p.n ? p.m : p.n);
+  // sort space for the dispatch order (radix sort of nsplit row_blocks 32-bit keys)
   size_t sort_bytes = 0;
   if (bucket)
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const unsigned *)nullptr,
                                              (unsigned *)nullptr, (const int *)nullptr,
-                                             (int *)nullptr, (int)mx, 0, 8, s);
+                                             (int *)nullptr, plan.nsplit * plan.row_blocks, 0,
+                                             32, s);
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t off = 0;
   const size_t o_Ap = off; off += al(mp * nc * 128);
@@ -1516,7 +1613,7 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   const size_t o_cd = off; off += al(np * sizeof(double4));
   const size_t o_prow = off; off += al((size_t)p.m * sizeof(int));
   const size_t o_pcol = off; off += al((size_t)p.n * sizeof(int));
-  const size_t o_ids = off; off += al(mx * sizeof(int));
+
   const size_t o_ka = off; off += bucket ? al((size_t)p.m * sizeof(unsigned)) : 0;
   const size_t o_kb = off; off += bucket ? al((size_t)p.n * sizeof(unsigned)) : 0;
   const size_t o_kouta = off; off += bucket ? al((size_t)p.m * sizeof(unsigned)) : 0;
@@ -1552,6 +1649,9 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   // dispatch order of the row blocks of each slice: keys and values, twice
   const size_t nord = bounds ? (size_t)plan.nsplit * plan.row_blocks : 0;
   const size_t o_ord = off; off += bounds ? al(4 * nord * sizeof(int)) : 0;
+  const int tiles_a = (int)((p.m + kSortTile - 1) / kSortTile);
+  const int tiles_b = (int)((p.n + kSortTile - 1) / kSortTile);
+  const size_t o_hist = off; off += bucket ? al((size_t)(tiles_a + tiles_b) * kPivots * 4) : 0;
   const size_t o_sort = off; off += al(sort_bytes);
   const size_t npart = (size_t)(plan.nsplit > 1 ? plan.nsplit : 0) * p.m;
   double *partial = nullptr;
@@ -1589,7 +1689,6 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
     hipLaunchKernelGGL(mix_identity2_kernel, dim3((unsigned)((mx + 255) / 256)), dim3(256), 0, s,
                        prow, (long)p.m, pcol, (long)p.n);
   } else {
-    int *ids = (int *)(base + o_ids);
     unsigned *ka = (unsigned *)(base + o_ka), *kb = (unsigned *)(base + o_kb);
     unsigned *kouta = (unsigned *)(base + o_kouta), *koutb = (unsigned *)(base + o_koutb);
     float *piv = (float *)(base + o_piv);
@@ -1603,16 +1702,15 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
       case 3: launch_bucket<96>(p, pivB, ka, kb, s); break;
       default: launch_bucket<128>(p, pivB, ka, kb, s); break;
     }
-    hipLaunchKernelGGL(mix_identity_kernel, dim3((unsigned)((mx + 255) / 256)), dim3(256), 0, s,
-                       ids, (long)mx);
-    size_t tb = sort_bytes;
-    if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, tb, (const unsigned *)ka, kouta,
-                                           (const int *)ids, prow, p.m, 0, 8, s) != hipSuccess)
-      return KS_ERR_DEVICE;
-    tb = sort_bytes;
-    if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, tb, (const unsigned *)kb, koutb,
-                                           (const int *)ids, pcol, p.n, 0, 8, s) != hipSuccess)
-      return KS_ERR_DEVICE;
+    // targets and sources ordered by bucket, stably (a counting sort: keys < 128)
+    unsigned *hist = (unsigned *)(base + o_hist);
+    hipLaunchKernelGGL(mix_hist_kernel, dim3((unsigned)(tiles_a + tiles_b)), dim3(256), 0, s,
+                       (const unsigned *)ka, (long)p.m, (const unsigned *)kb, (long)p.n, tiles_a,
+                       hist);
+    hipLaunchKernelGGL(mix_scan_kernel, dim3(2), dim3(1024), 0, s, hist, tiles_a, tiles_b);
+    hipLaunchKernelGGL(mix_place_kernel, dim3((unsigned)(tiles_a + tiles_b)), dim3(64), 0, s,
+                       (const unsigned *)ka, (long)p.m, (const unsigned *)kb, (long)p.n, tiles_a,
+                       (const unsigned *)hist, kouta, prow, koutb, pcol);
     if (bounds) {
       if (P != kPivots) return KS_ERR_INVALID_ARG;  // the bound kernels take all kPivots tiles
       float *pp = (float *)(base + o_pp), *pn = (float *)(base + o_pn);
