@@ -736,50 +736,49 @@ __global__ void mix_identity2_kernel(int *a, long m, int *b, long n) {
 // Writes point `pt` (coordinates x, or zeros when x is null) into the blocked
 // BF16 / FP32 layout, and with Bd non-null into the FP64 source layout too.
 template <int PREC>
-__device__ __forceinline__ void pack_point(const double *__restrict__ x, int k, int nc, long pt,
-                                           uint4 *pack, double4 *Bd) {
-  uint4 *blk = pack + (pt >> 4) * nc * 128;
-  const int p16 = (int)(pt & 15);
-  for (int c = 0; c < nc; ++c) {
-    for (int g = 0; g < 4; ++g) {
-      uint4 *o0 = blk + (c * 2) * 64 + g * 16 + p16;
-      uint4 *o1 = o0 + 64;
-      if constexpr (PREC == KS_PREC_MIXED_BF16X3) {
-        bf8 hi, lo;
+__device__ __forceinline__ void pack_chunk(const double *__restrict__ x, int k, int nc, long pt,
+                                           int c, int g, uint4 *pack) {
+  uint4 *o0 = pack + (pt >> 4) * nc * 128 + (c * 2) * 64 + g * 16 + (int)(pt & 15);
+  uint4 *o1 = o0 + 64;
+  if constexpr (PREC == KS_PREC_MIXED_BF16X3) {
+    bf8 hi, lo;
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          int idx = c * 32 + 8 * g + t;
-          double v = (x && idx < k) ? x[idx] : 0.0;
-          __bf16 h = (__bf16)(float)v;
-          hi[t] = h;
-          lo[t] = (__bf16)(float)(v - (double)(float)h);
-        }
-        *o0 = *reinterpret_cast<uint4 *>(&hi);
-        *o1 = *reinterpret_cast<uint4 *>(&lo);
-      } else {
-        float f[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          int idx = c * 32 + 8 * g + t;
-          f[t] = (x && idx < k) ? (float)x[idx] : 0.0f;
-        }
-        *o0 = *reinterpret_cast<uint4 *>(&f[0]);
-        *o1 = *reinterpret_cast<uint4 *>(&f[4]);
-      }
+    for (int t = 0; t < 8; ++t) {
+      const int idx = c * 32 + 8 * g + t;
+      const double v = (x && idx < k) ? x[idx] : 0.0;
+      const __bf16 h = (__bf16)(float)v;
+      hi[t] = h;
+      lo[t] = (__bf16)(float)(v - (double)(float)h);
     }
-  }
-  if (Bd) {
-    double4 *dblk = Bd + (pt >> 4) * (2 * nc) * 64;
-    for (int c = 0; c < 2 * nc; ++c)
-      for (int g = 0; g < 4; ++g) {
-        double v[4];
+    *o0 = *reinterpret_cast<uint4 *>(&hi);
+    *o1 = *reinterpret_cast<uint4 *>(&lo);
+  } else {
+    float f[8];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          int idx = c * 16 + 4 * g + t;
-          v[t] = (x && idx < k) ? x[idx] : 0.0;
-        }
-        dblk[c * 64 + g * 16 + p16] = make_double4(v[0], v[1], v[2], v[3]);
-      }
+    for (int t = 0; t < 8; ++t) {
+      const int idx = c * 32 + 8 * g + t;
+      f[t] = (x && idx < k) ? (float)x[idx] : 0.0f;
+    }
+    *o0 = *reinterpret_cast<uint4 *>(&f[0]);
+    *o1 = *reinterpret_cast<uint4 *>(&f[4]);
+  }
+}
+
+// The FP64 source operand's 16-coordinate chunks 2c and 2c + 1, lane group g:
+// coordinates 16 c2 + 4 g .. + 3 of point pt.
+__device__ __forceinline__ void pack_f64_chunk(const double *__restrict__ x, int k, int nc, long pt,
+                                               int c, int g, double4 *Bd) {
+  double4 *dblk = Bd + (pt >> 4) * (2 * nc) * 64 + g * 16 + (int)(pt & 15);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int c2 = 2 * c + h;
+    double v[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int idx = c2 * 16 + 4 * g + t;
+      v[t] = (x && idx < k) ? x[idx] : 0.0;
+    }
+    dblk[c2 * 64] = make_double4(v[0], v[1], v[2], v[3]);
   }
 }
 
@@ -806,45 +805,63 @@ __global__ void mix_rows_kernel(const Problem p, int kp, long mp, const int *__r
 // both in sorted order.
 template <int PREC, int TYPE>
 __global__ void mixed_pack_kernel(const Problem p, int nc, float xs, long mp, long np, MixPack pk) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < mp) {
+  // one thread per (point, 32-coordinate chunk c, lane group g): eight threads
+  // read a row of k = 64 together; the (0, 0) thread writes the point's factors
+  const int per = 4 * nc;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (mp + np) * per) return;
+  const long u = e / per;
+  const int cg = (int)(e - u * per), c = cg >> 2, g = cg & 3;
+  if (u < mp) {
+    const long t = u;
     if (t < p.m) {
       const int ai = p.amap[pk.prow[t]];
-      pack_point<PREC>(p.XA + (long)ai * p.k, p.k, nc, t, pk.Ap, nullptr);
-      const double aa = p.XA2[ai];
-      const double hi = (TYPE == VAR_BANDWIDTH) ? p.hi[ai] : 0.0;
-      pk.ra[t] = make_float2((float)(-0.5 * aa), (float)(hi * -0.5 * (double)kLog2e));
-      pk.rd[t] = make_double2(aa, -0.5 * hi);
+      pack_chunk<PREC>(p.XA + (long)ai * p.k, p.k, nc, t, c, g, pk.Ap);
+      if (cg == 0) {
+        const double aa = p.XA2[ai];
+        const double hi = (TYPE == VAR_BANDWIDTH) ? p.hi[ai] : 0.0;
+        pk.ra[t] = make_float2((float)(-0.5 * aa), (float)(hi * -0.5 * (double)kLog2e));
+        pk.rd[t] = make_double2(aa, -0.5 * hi);
+      }
     } else {  // padding target: its estimated exponent is -inf, so it is never near
-      pack_point<PREC>(nullptr, p.k, nc, t, pk.Ap, nullptr);
-      // Gaussian x2 = -2 xs acc + ...: acc = -inf gives -inf for xs < 0 (a
-      // decaying kernel); for xs > 0 the row starts at +inf instead
-      const float pad = (TYPE == GAUSSIAN && xs > 0.0f) ? INFINITY : -INFINITY;
-      pk.ra[t] = make_float2(pad, -1.0f);
-      pk.rd[t] = make_double2(0.0, 0.0);
+      pack_chunk<PREC>(nullptr, p.k, nc, t, c, g, pk.Ap);
+      if (cg == 0) {
+        // Gaussian x2 = -2 xs acc + ...: acc = -inf gives -inf for xs < 0 (a
+        // decaying kernel); for xs > 0 the row starts at +inf instead
+        const float pad = (TYPE == GAUSSIAN && xs > 0.0f) ? INFINITY : -INFINITY;
+        pk.ra[t] = make_float2(pad, -1.0f);
+        pk.rd[t] = make_double2(0.0, 0.0);
+      }
     }
-  } else if (t < mp + np) {
-    const long j = t - mp;
+  } else {
+    const long j = u - mp;
     if (j < p.n) {
       const int jj = pk.pcol[j];
       const int bj = p.bmap[jj];
-      pack_point<PREC>(p.XB + (long)bj * p.k, p.k, nc, j, pk.Bp, pk.Bd);
-      const double bb = p.XB2[bj];
-      const double w = p.w[p.wmap[jj]];
-      const double hj = (TYPE == VAR_BANDWIDTH) ? p.hj[bj] : 0.0;
-      const float nb = (float)sqrt(bb);
-      // Gaussian: {xs |b|^2, |b - p_own| (centred near tier), w, |b|}
-      if (TYPE == GAUSSIAN)
-        pk.cb[j] = make_float4((float)(bb * (double)xs), pk.nb ? pk.nb[j] : 0.0f, (float)w, nb);
-      else pk.cb[j] = make_float4((float)bb, (float)hj, (float)w, nb);
-      pk.cs[j] = make_float2((float)(bb * (double)xs), nb);
-      pk.cd[j] = make_double4(bb, w, hj, 0.0);
+      const double *x = p.XB + (long)bj * p.k;
+      pack_chunk<PREC>(x, p.k, nc, j, c, g, pk.Bp);
+      pack_f64_chunk(x, p.k, nc, j, c, g, pk.Bd);
+      if (cg == 0) {
+        const double bb = p.XB2[bj];
+        const double w = p.w[p.wmap[jj]];
+        const double hj = (TYPE == VAR_BANDWIDTH) ? p.hj[bj] : 0.0;
+        const float nb = (float)sqrt(bb);
+        // Gaussian: {xs |b|^2, |b - p_own| (centred near tier), w, |b|}
+        if (TYPE == GAUSSIAN)
+          pk.cb[j] = make_float4((float)(bb * (double)xs), pk.nb ? pk.nb[j] : 0.0f, (float)w, nb);
+        else pk.cb[j] = make_float4((float)bb, (float)hj, (float)w, nb);
+        pk.cs[j] = make_float2((float)(bb * (double)xs), nb);
+        pk.cd[j] = make_double4(bb, w, hj, 0.0);
+      }
     } else {  // padding source: exponent -inf, weight 0
-      pack_point<PREC>(nullptr, p.k, nc, j, pk.Bp, pk.Bd);
-      pk.cb[j] = (TYPE == GAUSSIAN) ? make_float4(-INFINITY, 0.0f, 0.0f, 0.0f)
-                                    : make_float4(INFINITY, 1.0f, 0.0f, 0.0f);
-      pk.cs[j] = make_float2(-INFINITY, 0.0f);
-      pk.cd[j] = make_double4(0.0, 0.0, 1.0, 0.0);
+      pack_chunk<PREC>(nullptr, p.k, nc, j, c, g, pk.Bp);
+      pack_f64_chunk(nullptr, p.k, nc, j, c, g, pk.Bd);
+      if (cg == 0) {
+        pk.cb[j] = (TYPE == GAUSSIAN) ? make_float4(-INFINITY, 0.0f, 0.0f, 0.0f)
+                                      : make_float4(INFINITY, 1.0f, 0.0f, 0.0f);
+        pk.cs[j] = make_float2(-INFINITY, 0.0f);
+        pk.cd[j] = make_double4(0.0, 0.0, 1.0, 0.0);
+      }
     }
   }
 }
@@ -1527,7 +1544,7 @@ static hipError_t mixed_launch(const Problem &p, const FusedPlan &plan, double *
   const float xs = (float)(p.P.scal * (double)kLog2e);
   const long mp = (long)plan.row_blocks * mix_rows_per_wg(PREC, p.type),
              np = ((long)p.n + 15) / 16 * 16;
-  dim3 pg((unsigned)((mp + np + 255) / 256));
+  dim3 pg((unsigned)(((mp + np) * 4 * nc + 255) / 256));
   const int kp = 32 * nc;
   hipLaunchKernelGGL(mix_rows_kernel, dim3((unsigned)((mp * (kp / 2) + 255) / 256)), dim3(256), 0,
                      s, p, kp, mp, pk.prow, reinterpret_cast<double2 *>(pk.Ad));
