@@ -54,7 +54,7 @@ CPU_THREADS_CAP = 16  # one GPU's share of the host's CPUs
 TRAFFIC = {
     ("mixture_k64", 0): (2.344e9, "profiles/mixture_k64_fp64_and_mixed.md: "
                                   "fused_f64_mfma_kernel FETCH_SIZE + WRITE_SIZE"),
-    ("mixture_k64", 2): (3.67e9, "profiles/mixture_k64_fp64_and_mixed.md: mixed_kernel + "
+    ("mixture_k64", 2): (3.54e9, "profiles/mixture_k64_fp64_and_mixed.md: mixed_kernel + "
                                  "pre-pass kernels FETCH_SIZE + WRITE_SIZE"),
 }
 # flops per kernel evaluation in the reference's model, test/test_dgsks.c:254-283
