@@ -505,23 +505,53 @@ __global__ __launch_bounds__(256, KP <= 64 ? 3 : 2) void mix_bounds_kernel(
 // Sort keys of the workgroups of each slice: slice in the high bits, then the
 // complement of the workgroup's needed-block count, so an ascending sort puts
 // every slice's heaviest row blocks first.
-__global__ __launch_bounds__(256) void mix_order_kernel(const unsigned long long *need, int ngw,
-                                                       int nsplit, int row_blocks, int waves,
-                                                       unsigned *key, int *val) {
+// The dispatch order of each slice's row blocks: by descending needed-block
+// count (work[split][rb], written by mix_need_kernel), equal counts in index
+// order, so the heaviest start first.  Up to kRankMaxRB row blocks per slice
+// each row block's position is the rank of its key c = count * 2^13 +
+// (8191 - rb) among its slice's keys (distinct keys, so the ranks are a
+// permutation and the order is that of a stable sort); one workgroup per 256
+// row blocks of a slice, the slice's keys in LDS.
+constexpr int kRankMaxRB = 8192;
+__global__ __launch_bounds__(256) void mix_rank_kernel(const unsigned *work, int row_blocks,
+                                                       int *order) {
+  __shared__ uint4 s_c[kRankMaxRB / 4];
+  unsigned *sc = (unsigned *)s_c;
+  const int split = blockIdx.y;
+  const unsigned *wk = work + (long)split * row_blocks;
+  const int nq = (row_blocks + 3) / 4;
+  for (int j = threadIdx.x; j < nq * 4; j += 256)
+    sc[j] = j < row_blocks ? (min(wk[j], 0x7FFFFu) << 13) | (unsigned)(kRankMaxRB - 1 - j) : 0u;
+  __syncthreads();
+  const int rb = blockIdx.x * 256 + threadIdx.x;
+  if (rb >= row_blocks) return;
+  const unsigned mine = sc[rb];
+  int r0 = 0, r1 = 0, r2 = 0, r3 = 0;  // padding keys (0) never exceed mine
+  for (int q = 0; q < nq; ++q) {
+    const uint4 c = s_c[q];
+    r0 += c.x > mine;
+    r1 += c.y > mine;
+    r2 += c.z > mine;
+    r3 += c.w > mine;
+  }
+  order[(long)split * row_blocks + (r0 + r1) + (r2 + r3)] = rb;
+}
+
+// More row blocks per slice: radix-sort keys (slice, -count) in place of the
+// counts, the row blocks as values.
+__global__ __launch_bounds__(256) void mix_order_kernel(unsigned *work_key, int nsplit,
+                                                        int row_blocks, int *val) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= nsplit * row_blocks) return;
-  const int split = t / row_blocks, rb = t - split * row_blocks;
-  unsigned work = 0;
-  for (int w = 0; w < waves; ++w)
-    for (int gi = 0; gi < ngw; ++gi)
-      work += __popcll(need[((long)(rb * waves + w) * nsplit + split) * ngw + gi]);
-  key[t] = ((unsigned)split << 20) | (0xFFFFFu - (work < 0xFFFFFu ? work : 0xFFFFFu));
-  val[t] = rb;
+  const int split = t / row_blocks;
+  const unsigned work = work_key[t];
+  work_key[t] = ((unsigned)split << 20) | (0xFFFFFu - (work < 0xFFFFFu ? work : 0xFFFFFu));
+  val[t] = t - split * row_blocks;
 }
 
 __global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ngrp, int nsplit,
                                                        int cps, int ngw, int n,
-                                                       unsigned long long *need) {
+                                                       unsigned long long *need, unsigned *work) {
   const int lane = threadIdx.x & 63;
   const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (w >= (long)ngrp * nsplit) return;  // wave-uniform
@@ -555,6 +585,7 @@ __global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ng
     const float x = __shfl(lo, B & 63), y = __shfl(hi, B & 63);
     return B < 64 ? x : y;
   };
+  unsigned cnt = 0;
   for (int gi = 0; gi < ngw; ++gi) {
     const int sb = gi * 64 + lane;
     const bool in = sb < nbs;
@@ -623,7 +654,11 @@ __global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ng
     }
     const unsigned long long msk = __ballot(in && nd);
     if (lane == 0) out[gi] = msk;
+    cnt += (unsigned)__popcll(msk);
   }
+  // the row group's needed-block count in this slice (one wave per workgroup
+  // in the skip-tier kernel: the group is its row block)
+  if (lane == 0) work[(long)split * ngrp + g] = cnt;
 }
 
 // ---- stable counting sort of the bucket keys (keys < kPivots = 2^kKeyBits),
@@ -1612,7 +1647,7 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
   const size_t mx = (size_t)(p.m > p.n ? p.m : p.n);
   // sort space for the dispatch order (radix sort of nsplit row_blocks 32-bit keys)
   size_t sort_bytes = 0;
-  if (bucket)
+  if (bucket && plan.row_blocks > kRankMaxRB)
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const unsigned *)nullptr,
                                              (unsigned *)nullptr, (const int *)nullptr,
                                              (int *)nullptr, plan.nsplit * plan.row_blocks, 0,
@@ -1784,20 +1819,28 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
       mb.thr_d2 = (float)(-(double)kSkipT2 / fabs((double)xs) * (1.0 + 0x1p-20));
       unsigned long long *need = (unsigned long long *)(base + o_need);
       const long nwaves = (long)ngrp * plan.nsplit;
+      // needed-block counts per slice and row block (row groups are row blocks:
+      // the skip-tier kernel runs one wave per workgroup)
+      static_assert(mix_waves(KS_PREC_MIXED_BF16X3, GAUSSIAN) == 1, "row group != row block");
+      unsigned *ok0 = (unsigned *)(base + o_ord), *ok1 = ok0 + nord;
+      int *ov0 = (int *)(ok1 + nord), *ov1 = ov0 + nord;
       hipLaunchKernelGGL(mix_need_kernel, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, s, mb,
-                         (int)ngrp, plan.nsplit, plan.cols_per_split, (int)ngw, p.n, need);
+                         (int)ngrp, plan.nsplit, plan.cols_per_split, (int)ngw, p.n, need, ok0);
       pk.need = need;
       // each slice's row blocks by descending needed-block count (stable: equal
       // counts keep their order), so the heaviest start first
-      unsigned *ok0 = (unsigned *)(base + o_ord), *ok1 = ok0 + nord;
-      int *ov0 = (int *)(ok1 + nord), *ov1 = ov0 + nord;
-      hipLaunchKernelGGL(mix_order_kernel, dim3((unsigned)((nord + 255) / 256)), dim3(256), 0, s,
-                         need, (int)ngw, plan.nsplit, plan.row_blocks,
-                         mix_waves(precision, p.type), ok0, ov0);
-      size_t obytes = sort_bytes;
-      if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, obytes, ok0, ok1, ov0, ov1, (int)nord, 0,
-                                             32, s) != hipSuccess)
-        return KS_ERR_DEVICE;
+      if (plan.row_blocks <= kRankMaxRB) {
+        hipLaunchKernelGGL(mix_rank_kernel,
+                           dim3((unsigned)((plan.row_blocks + 255) / 256), (unsigned)plan.nsplit),
+                           dim3(256), 0, s, ok0, plan.row_blocks, ov1);
+      } else {
+        hipLaunchKernelGGL(mix_order_kernel, dim3((unsigned)((nord + 255) / 256)), dim3(256), 0,
+                           s, ok0, plan.nsplit, plan.row_blocks, ov0);
+        size_t obytes = sort_bytes;
+        if (hipcub::DeviceRadixSort::SortPairs(base + o_sort, obytes, ok0, ok1, ov0, ov1,
+                                               (int)nord, 0, 32, s) != hipSuccess)
+          return KS_ERR_DEVICE;
+      }
       pk.order = ov1;
     }
   }

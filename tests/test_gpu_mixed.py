@@ -152,3 +152,28 @@ def test_unsupported_type_reports(torch_cuda):
         L.ks_set_error_mode(mode)
     torch.cuda.synchronize()
     assert torch.all(du == 7.0)
+
+
+def test_many_row_blocks(oracle, torch_cuda):
+    """270000 clustered targets (8438 row blocks of 32 per slice: the dispatch
+    order by radix sort instead of by rank in LDS), 20000 sources, k = 8,
+    BF16x3 Gaussian with the pruning bounds: 192 sampled rows against the FP64
+    oracle, and the same bits on a second call."""
+    rng = np.random.default_rng(33)
+    m, n, k = 270000, 20000, 8
+    C = rng.random((16, k))
+    X = C[rng.integers(0, 16, m + n)] + 0.05 * rng.standard_normal((m + n, k))
+    X2 = np.einsum("ij,ij->i", X, X)
+    w = rng.standard_normal(m + n)
+    amap = np.arange(m, dtype=np.int32)
+    bmap = np.arange(m, m + n, dtype=np.int32)
+    scal = -1.0 / (2 * 0.1 ** 2)
+    u = _run(torch_cuda, 0, 2, X, X2, amap, bmap, w, scal=scal)
+    u2 = _run(torch_cuda, 0, 2, X, X2, amap, bmap, w, scal=scal)
+    assert np.array_equal(u, u2)
+    rows = np.sort(rng.choice(m, 192, replace=False)).astype(np.int32)
+    ref = oracle.dgsks_rows(0, X, X2, amap, X, X2, bmap, w, bmap, np.zeros(m + n), amap,
+                            rows=rows, scal=scal)
+    err = oracle.rel_l2(u[rows], ref)
+    print(f"many row blocks m={m} n={n} k={k}: rel L2 = {err:.3e}")
+    assert err <= TOL
