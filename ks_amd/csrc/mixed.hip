@@ -502,43 +502,53 @@ __global__ __launch_bounds__(256, KP <= 64 ? 3 : 2) void mix_bounds_kernel(
 // registers (lane l holds entries l and 64 + l) and are read with lane
 // shuffles, so a group's only memory loads are the blocks' bucket ranges and
 // their Vs entries, independent of each other.
-// Sort keys of the workgroups of each slice: slice in the high bits, then the
-// complement of the workgroup's needed-block count, so an ascending sort puts
-// every slice's heaviest row blocks first.
 // The dispatch order of each slice's row blocks: by descending needed-block
 // count (work[split][rb], written by mix_need_kernel), equal counts in index
 // order, so the heaviest start first.  Up to kRankMaxRB row blocks per slice
 // each row block's position is the rank of its key c = count * 2^13 +
 // (8191 - rb) among its slice's keys (distinct keys, so the ranks are a
-// permutation and the order is that of a stable sort); one workgroup per 256
-// row blocks of a slice, the slice's keys in LDS.
+// permutation and the order is that of a stable sort).  One workgroup per 64
+// row blocks of a slice, the slice's keys in LDS, each of its kRankWaves waves
+// counting over its share of them.
 constexpr int kRankMaxRB = 8192;
-__global__ __launch_bounds__(256) void mix_rank_kernel(const unsigned *work, int row_blocks,
-                                                       int *order) {
+constexpr int kRankWaves = 16;
+__global__ __launch_bounds__(64 * kRankWaves) void mix_rank_kernel(const unsigned *work,
+                                                                   int row_blocks, int *order) {
   __shared__ uint4 s_c[kRankMaxRB / 4];
+  __shared__ int s_r[kRankWaves][64];
   unsigned *sc = (unsigned *)s_c;
   const int split = blockIdx.y;
   const unsigned *wk = work + (long)split * row_blocks;
   const int nq = (row_blocks + 3) / 4;
-  for (int j = threadIdx.x; j < nq * 4; j += 256)
+  for (int j = threadIdx.x; j < nq * 4; j += 64 * kRankWaves)
     sc[j] = j < row_blocks ? (min(wk[j], 0x7FFFFu) << 13) | (unsigned)(kRankMaxRB - 1 - j) : 0u;
   __syncthreads();
-  const int rb = blockIdx.x * 256 + threadIdx.x;
-  if (rb >= row_blocks) return;
-  const unsigned mine = sc[rb];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int rb = blockIdx.x * 64 + lane;
+  const unsigned mine = rb < row_blocks ? sc[rb] : 0xFFFFFFFFu;
+  const int q0 = wv * nq / kRankWaves, q1 = (wv + 1) * nq / kRankWaves;
   int r0 = 0, r1 = 0, r2 = 0, r3 = 0;  // padding keys (0) never exceed mine
-  for (int q = 0; q < nq; ++q) {
+#pragma unroll 4
+  for (int q = q0; q < q1; ++q) {
     const uint4 c = s_c[q];
     r0 += c.x > mine;
     r1 += c.y > mine;
     r2 += c.z > mine;
     r3 += c.w > mine;
   }
-  order[(long)split * row_blocks + (r0 + r1) + (r2 + r3)] = rb;
+  s_r[wv][lane] = (r0 + r1) + (r2 + r3);
+  __syncthreads();
+  if (wv == 0 && rb < row_blocks) {
+    int r = 0;
+#pragma unroll
+    for (int v = 0; v < kRankWaves; ++v) r += s_r[v][lane];
+    order[(long)split * row_blocks + r] = rb;
+  }
 }
 
-// More row blocks per slice: radix-sort keys (slice, -count) in place of the
-// counts, the row blocks as values.
+// More row blocks per slice: radix-sort keys in place of the counts -- slice in
+// the high bits, then the complement of the count, so an ascending sort puts
+// every slice's heaviest row blocks first -- and the row blocks as values.
 __global__ __launch_bounds__(256) void mix_order_kernel(unsigned *work_key, int nsplit,
                                                         int row_blocks, int *val) {
   const int t = blockIdx.x * 256 + threadIdx.x;
@@ -1831,8 +1841,8 @@ int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
       // counts keep their order), so the heaviest start first
       if (plan.row_blocks <= kRankMaxRB) {
         hipLaunchKernelGGL(mix_rank_kernel,
-                           dim3((unsigned)((plan.row_blocks + 255) / 256), (unsigned)plan.nsplit),
-                           dim3(256), 0, s, ok0, plan.row_blocks, ov1);
+                           dim3((unsigned)((plan.row_blocks + 63) / 64), (unsigned)plan.nsplit),
+                           dim3(64 * kRankWaves), 0, s, ok0, plan.row_blocks, ov1);
       } else {
         hipLaunchKernelGGL(mix_order_kernel, dim3((unsigned)((nord + 255) / 256)), dim3(256), 0,
                            s, ok0, plan.nsplit, plan.row_blocks, ov0);
