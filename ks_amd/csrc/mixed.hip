@@ -575,41 +575,50 @@ __global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ng
   //   d^2 >= min_{B in RB} V[T][B] + min_{A in RA} V[S][A] - max_{A, B} pp[A][B]
   // (the pivot-hyperplane bound of each pair's own pivots); a block pair is
   // kept when the bound of any of its part pairs fails to prove the cutoff.
+  // The two row blocks' rows of V (with the pp row of their first bucket) and
+  // V1, in this wave's LDS; entry kPivots repeats entry kPivots - 1, so entry
+  // b0 + 1 is the first bucket of a source block's part 1 (b0 + 1, or the last
+  // bucket when b0 is the last).  One wave writes and reads its own rows: its
+  // LDS operations complete in order.
+  __shared__ float2 s_tq[4][2][kPivots + 2];
+  __shared__ float s_t1[4][2][kPivots + 2];
+  const int wv = threadIdx.x >> 6;
   int a0[2], a1[2];
-  float vt_lo[2][2], vt_hi[2][2];  // rows V[T] and V1[T] of the two row blocks
-  float pp_lo[2], pp_hi[2];        // pp rows of their first buckets
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb) {
     const int T = 2 * g + rb;
     const unsigned rtv = b.rt[T];
     a0[rb] = (int)(rtv & 0xFFFFu);
     a1[rb] = (int)(rtv >> 16);
-    vt_lo[rb][0] = b.Vt[(long)T * kPivots + lane];
-    vt_hi[rb][0] = b.Vt[(long)T * kPivots + 64 + lane];
-    vt_lo[rb][1] = b.Vt1[(long)T * kPivots + lane];
-    vt_hi[rb][1] = b.Vt1[(long)T * kPivots + 64 + lane];
-    pp_lo[rb] = b.pp[a0[rb] * kPivots + lane];
-    pp_hi[rb] = b.pp[a0[rb] * kPivots + 64 + lane];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int B = h * 64 + lane;
+      const float2 v = make_float2(b.Vt[(long)T * kPivots + B], b.pp[a0[rb] * kPivots + B]);
+      const float v1 = b.Vt1[(long)T * kPivots + B];
+      s_tq[wv][rb][B] = v;
+      s_t1[wv][rb][B] = v1;
+      if (B == kPivots - 1) {
+        s_tq[wv][rb][kPivots] = v;
+        s_t1[wv][rb][kPivots] = v1;
+      }
+    }
   }
-  auto pick = [&](float lo, float hi, int B) {  // entry B of a row held as (lo, hi)
-    const float x = __shfl(lo, B & 63), y = __shfl(hi, B & 63);
-    return B < 64 ? x : y;
-  };
+  __builtin_amdgcn_wave_barrier();
   unsigned cnt = 0;
   for (int gi = 0; gi < ngw; ++gi) {
     const int sb = gi * 64 + lane;
     const bool in = sb < nbs;
-    const int S = (j0 >> 4) + (in ? sb : 0);
+    const int S = min((j0 >> 4) + (in ? sb : 0), b.nsb - 1);  // in range for an empty slice
     const unsigned rsv = b.rs[S];
     const int b0 = (int)(rsv & 0xFFFFu), b1 = (int)(rsv >> 16);
-    const int bn = b0 + 1 < kPivots ? b0 + 1 : kPivots - 1;  // first bucket of part 1
-    // the source block's two bound columns at the row blocks' first buckets,
-    // every load issued before the first use
+    // the source block's two bound columns at the row blocks' first buckets
+    // (V1 is +inf for a block of one bucket), every load issued before the
+    // first use
     float vs0[2], vs1[2];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
       vs0[rb] = b.Vs[(long)a0[rb] * b.nsb + S];
-      vs1[rb] = b1 > b0 ? b.Vs1[(long)a0[rb] * b.nsb + S] : INFINITY;
+      vs1[rb] = b.Vs1[(long)a0[rb] * b.nsb + S];
     }
     // one part pair: false where its bound proves every pair negligible
     auto part_keeps = [&](float tmin, float smin, float ppmax) {
@@ -621,46 +630,36 @@ __global__ __launch_bounds__(256) void mix_need_kernel(const MixBounds b, int ng
     bool nd = false;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
-      const int T = 2 * g + rb;
-      bool keep = b1 - b0 >= 4 || a1[rb] - a0[rb] >= 4;
-      const float q_b0 = pick(pp_lo[rb], pp_hi[rb], b0), q_bn = pick(pp_lo[rb], pp_hi[rb], bn);
-      const int npa = a1[rb] > a0[rb] ? 2 : 1;  // wave-uniform
-      for (int pa = 0; pa < npa; ++pa) {
-        const float *vt = pa ? b.Vt1 : b.Vt;
-        // shuffles first: every lane takes part
-        const float t_b0 = pick(vt_lo[rb][pa], vt_hi[rb][pa], b0);
-        const float t_bn = pick(vt_lo[rb][pa], vt_hi[rb][pa], bn);
-        if (pa == 0) {
-          // the row block's first bucket against the source's first bucket, and
-          // against its other buckets (one more bucket: registers; more: memory)
-          keep = keep || part_keeps(t_b0, vs0[rb], q_b0);
-          if (b1 == b0 + 1) {
-            keep = keep || part_keeps(t_bn, vs1[rb], q_bn);
-          } else if (b1 > b0 + 1) {
-            float tmin = t_bn, ppmax = q_bn;
-            for (int B = b0 + 2; B <= b1; ++B) {
-              tmin = fminf(tmin, vt[(long)T * kPivots + B]);
-              ppmax = fmaxf(ppmax, b.pp[a0[rb] * kPivots + B]);
-            }
-            keep = keep || part_keeps(tmin, vs1[rb], ppmax);
+      bool keep = (b1 - b0 >= 4) | (a1[rb] - a0[rb] >= 4);
+      // the row block's first bucket against the source's first bucket, and
+      // against its other buckets
+      const float2 q0 = s_tq[wv][rb][b0], q1 = s_tq[wv][rb][b0 + 1];
+      keep = keep | part_keeps(q0.x, vs0[rb], q0.y);
+      // (past b0 + 3 the block is kept anyway)
+      float tmin = q1.x, ppmax = q1.y;
+      const int bl = min(b1, b0 + 3);
+      for (int B = b0 + 2; B <= bl; ++B) {  // a source block over three buckets or more: rare
+        const float2 q = s_tq[wv][rb][B];
+        tmin = fminf(tmin, q.x);
+        ppmax = fmaxf(ppmax, q.y);
+      }
+      keep = keep | part_keeps(tmin, vs1[rb], ppmax);
+      if (a1[rb] > a0[rb] && a1[rb] - a0[rb] < 4) {  // wave-uniform: a row block over buckets (rare)
+        // the row block's other buckets, with the bounds of every source bucket
+        for (int pb = 0; pb < 2; ++pb) {
+          if (pb == 1 && b1 == b0) break;  // a source block of one bucket
+          const float *vs = pb ? b.Vs1 : b.Vs;
+          const int blo = pb ? b0 + 1 : b0, bhi = pb ? bl : b0;
+          float tm = s_t1[wv][rb][blo], smin = INFINITY, ppm = 0.0f;
+          for (int B = blo + 1; B <= bhi; ++B) tm = fminf(tm, s_t1[wv][rb][B]);
+          for (int A = a0[rb] + 1; A <= a1[rb]; ++A) {
+            smin = fminf(smin, vs[(long)A * b.nsb + S]);
+            for (int B = blo; B <= bhi; ++B) ppm = fmaxf(ppm, b.pp[A * kPivots + B]);
           }
-        } else {
-          // the row block's other buckets (a block over buckets: rare), from memory
-          for (int pb = 0; pb < 2; ++pb) {
-            if (pb == 1 && b1 == b0) break;  // a source block of one bucket
-            const float *vs = pb ? b.Vs1 : b.Vs;
-            const int blo = pb ? b0 + 1 : b0, bhi = pb ? b1 : b0;
-            float tmin = pb ? t_bn : t_b0, smin = INFINITY, ppmax = 0.0f;
-            for (int B = blo + 1; B <= bhi; ++B) tmin = fminf(tmin, vt[(long)T * kPivots + B]);
-            for (int A = a0[rb] + 1; A <= a1[rb]; ++A) {
-              smin = fminf(smin, vs[(long)A * b.nsb + S]);
-              for (int B = blo; B <= bhi; ++B) ppmax = fmaxf(ppmax, b.pp[A * kPivots + B]);
-            }
-            keep = keep || part_keeps(tmin, smin, ppmax);
-          }
+          keep = keep | part_keeps(tm, smin, ppm);
         }
       }
-      nd = nd || keep;
+      nd = nd | keep;
     }
     const unsigned long long msk = __ballot(in && nd);
     if (lane == 0) out[gi] = msk;
