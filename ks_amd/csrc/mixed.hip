@@ -66,6 +66,7 @@ constexpr float kLog2e = 1.4426950408889634f;
 constexpr int kPivots = 128;            // bucketing pivots (at most)
 constexpr int kSortMin = 16384;         // bucket only when m and n are both at least this
 constexpr int kXcds = 8;                // L2 domains; workgroups go to them round robin
+constexpr int kCUs = 256;               // compute units (32 per L2 domain)
 constexpr long kSliceHiBytes = 2l << 20;  // BF16 hi halves of one source slice: 2 MiB
                                         // (its sorts cost ~15 launches per call)
 constexpr int kMixRB = 2;               // 16-row blocks per wave
@@ -159,81 +160,122 @@ __global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv, flo
 
 // Bucket (nearest pivot) of every target, then every source: the pivot with the
 // least |p|^2 - 2 x.p in FP32 (|x - p|^2 less the point's own |x|^2), ties to
-// the lowest pivot.  One wave per 16 points (targets, then sources, in call
-// order): their dot products with the 128 pivots on the FP32 matrix cores
-// (v_mfma_f32_16x16x4f32, the points as the A operand, the pivots as B from
-// pivB, eight 16-pivot tiles), |p|^2 from the same fragments.  The buckets only
-// decide which points share a block.
+// the lowest pivot.  One wave per 16 points at a time (targets, then sources,
+// in call order): their dot products with the 128 pivots on the FP32 matrix
+// cores (v_mfma_f32_16x16x4f32, the points as the A operand, the pivots as B
+// from pivB, eight 16-pivot tiles), |p|^2 from the same fragments.  Up to 64
+// coordinates a wave keeps the pivot fragments and norms in registers (128
+// VGPRs) and takes blocks of 16 points in a grid-stride loop; beyond, it takes
+// one block and reads the fragments as it goes.  The buckets only decide which
+// points share a block.
 template <int KP>
 __global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p,
                                                          const float *__restrict__ pivB,
                                                          unsigned *key_a, unsigned *key_b) {
   constexpr int NS = KP / 4;
   constexpr int NT = kPivots / 16;
+  constexpr bool RES = KP <= 64;
   const int lane = threadIdx.x & 63, p16 = lane & 15, g = lane >> 4;
-  const long blk = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const long tot = (long)p.m + p.n;
-  if (blk * 16 >= tot) return;  // wave-uniform
+  const long nblk = (tot + 15) / 16;
+  const long nw = (long)gridDim.x * 4;
+  long blk = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (blk >= nblk) return;  // wave-uniform
+  float pb[RES ? NT : 1][RES ? NS : 1], nn[RES ? NT : 1];
+  if constexpr (RES) {
+#pragma unroll
+    for (int tt = 0; tt < NT; ++tt) {
+      float n2 = 0.0f;
+#pragma unroll
+      for (int st = 0; st < NS; ++st) {
+        pb[tt][st] = pivB[((long)tt * NS + st) * 64 + lane];
+        n2 = fmaf(pb[tt][st], pb[tt][st], n2);
+      }
+      n2 += __shfl_xor(n2, 16);
+      n2 += __shfl_xor(n2, 32);
+      nn[tt] = n2;  // |p|^2 of pivot 16 tt + p16
+    }
+  }
   // the A operand: point blk 16 + p16 supplies coordinates 4 st + g
-  const long t = blk * 16 + p16;
-  const double *x = nullptr;
-  if (t < p.m) x = p.XA + (long)p.amap[t] * p.k;
-  else if (t < tot) x = p.XB + (long)p.bmap[t - p.m] * p.k;
-  float a[NS];
-#pragma unroll
-  for (int st = 0; st < NS; ++st) {
-    const int q = 4 * st + g;
-    a[st] = (x && q < p.k) ? (float)x[q] : 0.0f;
-  }
-  float best[4];
-  int bi[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    best[v] = INFINITY;
-    bi[v] = 0;
-  }
-#pragma unroll
-  for (int tt = 0; tt < NT; ++tt) {
-    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    float n2 = 0.0f;
-    const float *pv = pivB + (long)tt * NS * 64 + lane;
+  auto load = [&](const long blk, float (&a)[NS]) {
+    const long t = blk * 16 + p16;
+    const double *x = nullptr;
+    if (t < p.m) x = p.XA + (long)p.amap[t] * p.k;
+    else if (t < tot) x = p.XB + (long)p.bmap[t - p.m] * p.k;
 #pragma unroll
     for (int st = 0; st < NS; ++st) {
-      const float b = pv[st * 64];
-      n2 = fmaf(b, b, n2);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], b, acc, 0, 0, 0);
+      const int q = 4 * st + g;
+      a[st] = (x && q < p.k) ? (float)x[q] : 0.0f;
     }
-    n2 += __shfl_xor(n2, 16);
-    n2 += __shfl_xor(n2, 32);  // |p|^2 of pivot 16 tt + p16
-    // acc[v]: point 4 g + v, pivot 16 tt + p16
+  };
+  auto block = [&](const long blk, const float (&a)[NS]) {
+    float best[4];
+    int bi[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const float d = fmaf(-2.0f, acc[v], n2);
-      if (d < best[v]) {
-        best[v] = d;
-        bi[v] = 16 * tt + p16;
+      best[v] = INFINITY;
+      bi[v] = 0;
+    }
+#pragma unroll
+    for (int tt = 0; tt < NT; ++tt) {
+      f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+      float n2 = 0.0f;
+      if constexpr (RES) {
+#pragma unroll
+        for (int st = 0; st < NS; ++st)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], pb[tt][st], acc, 0, 0, 0);
+        n2 = nn[tt];
+      } else {
+        const float *pv = pivB + (long)tt * NS * 64 + lane;
+#pragma unroll
+        for (int st = 0; st < NS; ++st) {
+          const float b = pv[st * 64];
+          n2 = fmaf(b, b, n2);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], b, acc, 0, 0, 0);
+        }
+        n2 += __shfl_xor(n2, 16);
+        n2 += __shfl_xor(n2, 32);  // |p|^2 of pivot 16 tt + p16
+      }
+      // acc[v]: point 4 g + v, pivot 16 tt + p16
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float d = fmaf(-2.0f, acc[v], n2);
+        if (d < best[v]) {
+          best[v] = d;
+          bi[v] = 16 * tt + p16;
+        }
       }
     }
-  }
-  // the least over the 16 lanes of the point's lane group, ties to the lower pivot
+    // the least over the 16 lanes of the point's lane group, ties to the lower pivot
 #pragma unroll
-  for (int o = 1; o < 16; o <<= 1)
+    for (int o = 1; o < 16; o <<= 1)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const float ob = __shfl_xor(best[v], o);
-      const int oi = __shfl_xor(bi[v], o);
-      if (ob < best[v] || (ob == best[v] && oi < bi[v])) {
-        best[v] = ob;
-        bi[v] = oi;
+      for (int v = 0; v < 4; ++v) {
+        const float ob = __shfl_xor(best[v], o);
+        const int oi = __shfl_xor(bi[v], o);
+        if (ob < best[v] || (ob == best[v] && oi < bi[v])) {
+          best[v] = ob;
+          bi[v] = oi;
+        }
+      }
+    if (p16 == 0) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long tp = blk * 16 + 4 * g + v;
+        if (tp < p.m) key_a[tp] = (unsigned)bi[v];
+        else if (tp < tot) key_b[tp - p.m] = (unsigned)bi[v];
       }
     }
-  if (p16 == 0) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const long tp = blk * 16 + 4 * g + v;
-      if (tp < p.m) key_a[tp] = (unsigned)bi[v];
-      else if (tp < tot) key_b[tp - p.m] = (unsigned)bi[v];
+  };
+  float a[NS];
+  if constexpr (RES) {
+    for (; blk < nblk; blk += nw) {
+      load(blk, a);
+      block(blk, a);
     }
+  } else {
+    load(blk, a);
+    block(blk, a);  // one block per wave: the grid covers them all
   }
 }
 
@@ -1616,8 +1658,11 @@ template <int KP>
 static void launch_bucket(const Problem &p, const float *pivB, unsigned *ka, unsigned *kb,
                           hipStream_t s) {
   const long blocks = ((long)p.m + p.n + 15) / 16;
-  hipLaunchKernelGGL((mix_bucket_kernel<KP>), dim3((unsigned)((blocks + 3) / 4)), dim3(256), 0, s,
-                     p, pivB, ka, kb);
+  // with the pivots resident (k <= 64), two workgroups per CU in a grid-stride loop
+  long wgs = (blocks + 3) / 4;
+  if (KP <= 64 && wgs > 2 * kCUs) wgs = 2 * kCUs;
+  hipLaunchKernelGGL((mix_bucket_kernel<KP>), dim3((unsigned)wgs), dim3(256), 0, s, p, pivB, ka,
+                     kb);
 }
 
 int enqueue_mixed(const Problem &p, int precision, hipStream_t s) {
