@@ -1039,7 +1039,6 @@ __global__ __launch_bounds__(64 * mix_waves(PREC, TYPE), NC <= 2 ? 2 : 1) void m
   // rows.
   constexpr bool SK = kSkipTier<PREC, TYPE>;
   constexpr int KS = 32 * NC + 2;
-  constexpr int NC4 = 2 * NC;  // float4 chunks of a centred operand fragment
   __shared__ __align__(16) double s_rows[WAVES][SK ? 2 : WROWS * KS];
   __shared__ __align__(16) uint4 s_ac[WAVES][SK ? RB * NC * 3 * 64 : 1];
   __shared__ double s_tot[WAVES][WROWS];
