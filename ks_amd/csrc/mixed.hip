@@ -361,7 +361,7 @@ struct MixTabs {
 // (+inf); rng[blk] = first | last << 16 bucket of the block's points.  With
 // o.Ex, also the centred tier's tables (MixTabs), zero for points past m / n.
 template <int KP>
-__global__ __launch_bounds__(256, KP <= 64 ? 3 : 2) void mix_bounds_kernel(
+__global__ __launch_bounds__(256, KP <= 64 ? 4 : 2) void mix_bounds_kernel(
     const Problem p, int ntb, int nsb, const double *__restrict__ pivA,
     const float *__restrict__ piv, const float *__restrict__ pn, const double *__restrict__ pn64,
     const double *__restrict__ ppd, const unsigned *__restrict__ keys_a,
@@ -419,12 +419,12 @@ __global__ __launch_bounds__(256, KP <= 64 ? 3 : 2) void mix_bounds_kernel(
   const bool tabs = o.Ex != nullptr;
   float *etab = tgt ? o.Ex : o.Qx;
   const long estride = tgt ? o.mp : o.np;
-  // the pivot tiles in four passes (fewer live accumulators: three waves per
-  // SIMD up to k = 64, where two passes held two: 0.30 -> 0.25 ms at
-  // mixture_k64)
-  constexpr int NH = NT / 4;
+  // the pivot tiles one per pass (few live accumulators: four waves per SIMD
+  // up to k = 64; two passes held two waves, four passes three: 0.30 -> 0.25
+  // -> 0.23 ms at mixture_k64)
+  constexpr int NH = NT / 8;
 #pragma unroll 1
-  for (int h = 0; h < 4; ++h) {
+  for (int h = 0; h < 8; ++h) {
     d4 acc[NH];
 #pragma unroll
     for (int t = 0; t < NH; ++t) {
