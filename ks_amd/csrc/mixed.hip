@@ -134,12 +134,20 @@ struct MixPack {
   const int *order;
 };
 
+// The coordinate that lane group g supplies at step st of the bounds kernel's
+// FP64 dot products: 32 (st / 8) + 8 g + st % 8.  A lane's coordinates are then
+// those of its lane in the BF16 split operands (lane 16 g + p holds 32 cc + 8 g
+// .. + 7), so the split reuses the values loaded for the products.
+__device__ __forceinline__ int bnd_coord(int st, int g) {
+  return 32 * (st >> 3) + 8 * g + (st & 7);
+}
+
 // FP32 copies of the pivots: pivot i is target (i * m) / P, zero past k; also
 // in the B-operand order of v_mfma_f32_16x16x4f32 for mix_bucket_kernel (pivB:
 // tile t, step st, lane 16 g + c holding coordinate 4 st + g of pivot 16 t + c).
 // With pivA, also as FP64 in the A-operand order of v_mfma_f64_16x16x4f64 for
-// mix_bounds_kernel: tile t (pivots 16 t ..), step st (coordinates 4 st ..),
-// lane 16 g + c holding coordinate 4 st + g of pivot 16 t + f64_a_row(c) -- the
+// mix_bounds_kernel: tile t (pivots 16 t ..), step st, lane 16 g + c holding
+// coordinate bnd_coord(st, g) of pivot 16 t + f64_a_row(c) -- the
 // row order that lands pivot 16 t + 4 g + v in accumulator slot v of lane group
 // g -- so that one fragment load is 512 contiguous bytes.  f64_a_row is its own
 // inverse (a 4 x 4 transpose of the row index).
@@ -154,7 +162,8 @@ __global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv, flo
   pivB[((long)(i >> 4) * (kp >> 2) + (q >> 2)) * 64 + (q & 3) * 16 + (i & 15)] = v;
   if (pivA) {
     const int c = 4 * ((i & 15) & 3) + ((i & 15) >> 2);
-    pivA[((long)(i >> 4) * (kp >> 2) + (q >> 2)) * 64 + (q & 3) * 16 + c] = (double)v;
+    const int st = 8 * (q >> 5) + (q & 7), g = (q >> 3) & 3;  // q = bnd_coord(st, g)
+    pivA[((long)(i >> 4) * (kp >> 2) + st) * 64 + g * 16 + c] = (double)v;
   }
 }
 
@@ -377,7 +386,7 @@ __global__ __launch_bounds__(256, KP <= 64 ? 4 : 2) void mix_bounds_kernel(
   const int cnt = tgt ? p.m : p.n;
   const unsigned *keys = tgt ? keys_a : keys_b;
   const int p16 = lane & 15, g = lane >> 4;
-  // the B operand: point p16 supplies coordinates 4 st + g
+  // the B operand: point p16 supplies coordinate bnd_coord(st, g) at step st
   const int pos = b0 * 16 + p16;
   const bool valid = pos < cnt;
   const double *x = nullptr;
@@ -396,7 +405,7 @@ __global__ __launch_bounds__(256, KP <= 64 ? 4 : 2) void mix_bounds_kernel(
   double n2 = 0.0;
 #pragma unroll
   for (int st = 0; st < NS; ++st) {
-    const int q = 4 * st + g;
+    const int q = bnd_coord(st, g);
     a[st] = (valid && q < p.k) ? x[q] : 0.0;
     n2 = fma(a[st], a[st], n2);
   }
@@ -406,7 +415,8 @@ __global__ __launch_bounds__(256, KP <= 64 ? 4 : 2) void mix_bounds_kernel(
   // coordinate groups summed)
   double dsel = 0.0;
 #pragma unroll
-  for (int st = 0; st < NS; ++st) dsel = fma(a[st], (double)piv[own * KP + 4 * st + g], dsel);
+  for (int st = 0; st < NS; ++st)
+    dsel = fma(a[st], (double)piv[own * KP + bnd_coord(st, g)], dsel);
   dsel += __shfl_xor(dsel, 16);
   dsel += __shfl_xor(dsel, 32);
   float pmx = fmaxf(pn[lane], pn[64 + lane]);
@@ -501,7 +511,7 @@ __global__ __launch_bounds__(256, KP <= 64 ? 4 : 2) void mix_bounds_kernel(
     double c2 = 0.0;
 #pragma unroll
     for (int st = 0; st < NS; ++st) {
-      const double d = valid ? a[st] - (double)piv[own * KP + 4 * st + g] : 0.0;
+      const double d = valid ? a[st] - (double)piv[own * KP + bnd_coord(st, g)] : 0.0;
       c2 = fma(d, d, c2);
     }
     c2 += __shfl_xor(c2, 16);
@@ -516,7 +526,8 @@ __global__ __launch_bounds__(256, KP <= 64 ? 4 : 2) void mix_bounds_kernel(
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         const int q = 32 * cc + 8 * g + t;
-        const double d = (valid && q < p.k) ? x[q] - (double)piv[own * KP + q] : 0.0;
+        // a[8 cc + t] holds coordinate q (bnd_coord)
+        const double d = (valid && q < p.k) ? a[8 * cc + t] - (double)piv[own * KP + q] : 0.0;
         const __bf16 h = (__bf16)(float)d;
         const double r1 = d - (double)(float)h;
         const __bf16 mm = (__bf16)(float)r1;
