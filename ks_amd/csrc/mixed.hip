@@ -402,13 +402,32 @@ __global__ __launch_bounds__(256, KP <= 64 ? 4 : 2) void mix_bounds_kernel(
   const int fbk = b0 * 16 < cnt ? (int)keys[b0 * 16] : 0;
   const bool part0 = own == fbk;
   double a[NS];
+  // a lane's coordinates come in runs of 8 (bnd_coord): 32-byte loads where
+  // the rows are 32-byte aligned (k a multiple of 4, table aligned)
+  const double *tab = tgt ? p.XA : p.XB;
+  const bool vec = (p.k & 3) == 0 && ((uintptr_t)tab & 31) == 0;  // wave-uniform
+#pragma unroll
+  for (int cc = 0; cc < KP / 32; ++cc) {
+    const int q0 = 32 * cc + 8 * g;
+    if (vec && valid && q0 + 8 <= p.k) {
+      const double4 *xv = reinterpret_cast<const double4 *>(x + q0);
+      const double4 u0 = xv[0], u1 = xv[1];
+      a[8 * cc + 0] = u0.x;
+      a[8 * cc + 1] = u0.y;
+      a[8 * cc + 2] = u0.z;
+      a[8 * cc + 3] = u0.w;
+      a[8 * cc + 4] = u1.x;
+      a[8 * cc + 5] = u1.y;
+      a[8 * cc + 6] = u1.z;
+      a[8 * cc + 7] = u1.w;
+    } else {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) a[8 * cc + t] = (valid && q0 + t < p.k) ? x[q0 + t] : 0.0;
+    }
+  }
   double n2 = 0.0;
 #pragma unroll
-  for (int st = 0; st < NS; ++st) {
-    const int q = bnd_coord(st, g);
-    a[st] = (valid && q < p.k) ? x[q] : 0.0;
-    n2 = fma(a[st], a[st], n2);
-  }
+  for (int st = 0; st < NS; ++st) n2 = fma(a[st], a[st], n2);
   n2 += __shfl_xor(n2, 16);
   n2 += __shfl_xor(n2, 32);  // |x|^2 of point p16 on its four lanes
   // x.p_own in FP64 from the point's own coordinates (lanes of the point's four
