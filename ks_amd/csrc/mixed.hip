@@ -144,7 +144,8 @@ __device__ __forceinline__ int bnd_coord(int st, int g) {
 
 // FP32 copies of the pivots: pivot i is target (i * m) / P, zero past k; also
 // in the B-operand order of v_mfma_f32_16x16x4f32 for mix_bucket_kernel (pivB:
-// tile t, step st, lane 16 g + c holding coordinate 4 st + g of pivot 16 t + c).
+// tile t, step st, lane 16 g + c holding coordinate bnd_coord(st, g) of pivot
+// 16 t + c).
 // With pivA, also as FP64 in the A-operand order of v_mfma_f64_16x16x4f64 for
 // mix_bounds_kernel: tile t (pivots 16 t ..), step st, lane 16 g + c holding
 // coordinate bnd_coord(st, g) of pivot 16 t + f64_a_row(c) -- the
@@ -159,10 +160,10 @@ __global__ void mix_pivot_kernel(const Problem p, int P, int kp, float *piv, flo
   const int r = (int)(((long)i * p.m) / P);
   const float v = q < p.k ? (float)p.XA[(long)p.amap[r] * p.k + q] : 0.0f;
   piv[t] = v;
-  pivB[((long)(i >> 4) * (kp >> 2) + (q >> 2)) * 64 + (q & 3) * 16 + (i & 15)] = v;
+  const int st = 8 * (q >> 5) + (q & 7), g = (q >> 3) & 3;  // q = bnd_coord(st, g)
+  pivB[((long)(i >> 4) * (kp >> 2) + st) * 64 + g * 16 + (i & 15)] = v;
   if (pivA) {
     const int c = 4 * ((i & 15) & 3) + ((i & 15) >> 2);
-    const int st = 8 * (q >> 5) + (q & 7), g = (q >> 3) & 3;  // q = bnd_coord(st, g)
     pivA[((long)(i >> 4) * (kp >> 2) + st) * 64 + g * 16 + c] = (double)v;
   }
 }
@@ -205,16 +206,40 @@ __global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p,
       nn[tt] = n2;  // |p|^2 of pivot 16 tt + p16
     }
   }
-  // the A operand: point blk 16 + p16 supplies coordinates 4 st + g
+  // the A operand: point blk 16 + p16 supplies coordinate bnd_coord(st, g) at
+  // step st, runs of 8 coordinates loaded as 32-byte vectors where the rows
+  // are 32-byte aligned
+  const bool vec_a = (p.k & 3) == 0 && ((uintptr_t)p.XA & 31) == 0;
+  const bool vec_b = (p.k & 3) == 0 && ((uintptr_t)p.XB & 31) == 0;
   auto load = [&](const long blk, float (&a)[NS]) {
     const long t = blk * 16 + p16;
     const double *x = nullptr;
-    if (t < p.m) x = p.XA + (long)p.amap[t] * p.k;
-    else if (t < tot) x = p.XB + (long)p.bmap[t - p.m] * p.k;
+    bool vec = false;
+    if (t < p.m) {
+      x = p.XA + (long)p.amap[t] * p.k;
+      vec = vec_a;
+    } else if (t < tot) {
+      x = p.XB + (long)p.bmap[t - p.m] * p.k;
+      vec = vec_b;
+    }
 #pragma unroll
-    for (int st = 0; st < NS; ++st) {
-      const int q = 4 * st + g;
-      a[st] = (x && q < p.k) ? (float)x[q] : 0.0f;
+    for (int cc = 0; cc < KP / 32; ++cc) {
+      const int q0 = 32 * cc + 8 * g;
+      if (vec && q0 + 8 <= p.k) {
+        const double4 *xv = reinterpret_cast<const double4 *>(x + q0);
+        const double4 u0 = xv[0], u1 = xv[1];
+        a[8 * cc + 0] = (float)u0.x;
+        a[8 * cc + 1] = (float)u0.y;
+        a[8 * cc + 2] = (float)u0.z;
+        a[8 * cc + 3] = (float)u0.w;
+        a[8 * cc + 4] = (float)u1.x;
+        a[8 * cc + 5] = (float)u1.y;
+        a[8 * cc + 6] = (float)u1.z;
+        a[8 * cc + 7] = (float)u1.w;
+      } else {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[8 * cc + u] = (x && q0 + u < p.k) ? (float)x[q0 + u] : 0.0f;
+      }
     }
   };
   auto block = [&](const long blk, const float (&a)[NS]) {
