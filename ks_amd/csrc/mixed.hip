@@ -134,6 +134,31 @@ struct MixPack {
   const int *order;
 };
 
+// Coordinates q0 .. q0 + 7 of row x (zero past k, or all zero for a null row),
+// as two 32-byte loads where vec says the rows are 32-byte aligned (k a
+// multiple of 4 and the table aligned).
+__device__ __forceinline__ void load_run8(const double *x, int k, int q0, bool vec,
+                                          double (&v)[8]) {
+  if (x && vec && q0 + 8 <= k) {
+    const double4 *xv = reinterpret_cast<const double4 *>(x + q0);
+    const double4 u0 = xv[0], u1 = xv[1];
+    v[0] = u0.x;
+    v[1] = u0.y;
+    v[2] = u0.z;
+    v[3] = u0.w;
+    v[4] = u1.x;
+    v[5] = u1.y;
+    v[6] = u1.z;
+    v[7] = u1.w;
+  } else {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = (x && q0 + t < k) ? x[q0 + t] : 0.0;
+  }
+}
+__device__ __forceinline__ bool rows_vec_aligned(const double *tab, int k) {
+  return (k & 3) == 0 && ((uintptr_t)tab & 31) == 0;
+}
+
 // The coordinate that lane group g supplies at step st of the bounds kernel's
 // FP64 dot products: 32 (st / 8) + 8 g + st % 8.  A lane's coordinates are then
 // those of its lane in the BF16 split operands (lane 16 g + p holds 32 cc + 8 g
@@ -209,8 +234,7 @@ __global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p,
   // the A operand: point blk 16 + p16 supplies coordinate bnd_coord(st, g) at
   // step st, runs of 8 coordinates loaded as 32-byte vectors where the rows
   // are 32-byte aligned
-  const bool vec_a = (p.k & 3) == 0 && ((uintptr_t)p.XA & 31) == 0;
-  const bool vec_b = (p.k & 3) == 0 && ((uintptr_t)p.XB & 31) == 0;
+  const bool vec_a = rows_vec_aligned(p.XA, p.k), vec_b = rows_vec_aligned(p.XB, p.k);
   auto load = [&](const long blk, float (&a)[NS]) {
     const long t = blk * 16 + p16;
     const double *x = nullptr;
@@ -224,22 +248,10 @@ __global__ __launch_bounds__(256) void mix_bucket_kernel(const Problem p,
     }
 #pragma unroll
     for (int cc = 0; cc < KP / 32; ++cc) {
-      const int q0 = 32 * cc + 8 * g;
-      if (vec && q0 + 8 <= p.k) {
-        const double4 *xv = reinterpret_cast<const double4 *>(x + q0);
-        const double4 u0 = xv[0], u1 = xv[1];
-        a[8 * cc + 0] = (float)u0.x;
-        a[8 * cc + 1] = (float)u0.y;
-        a[8 * cc + 2] = (float)u0.z;
-        a[8 * cc + 3] = (float)u0.w;
-        a[8 * cc + 4] = (float)u1.x;
-        a[8 * cc + 5] = (float)u1.y;
-        a[8 * cc + 6] = (float)u1.z;
-        a[8 * cc + 7] = (float)u1.w;
-      } else {
+      double v[8];
+      load_run8(x, p.k, 32 * cc + 8 * g, vec, v);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) a[8 * cc + u] = (x && q0 + u < p.k) ? (float)x[q0 + u] : 0.0f;
-      }
+      for (int u = 0; u < 8; ++u) a[8 * cc + u] = (float)v[u];
     }
   };
   auto block = [&](const long blk, const float (&a)[NS]) {
@@ -429,26 +441,13 @@ __global__ __launch_bounds__(256, KP <= 64 ? 4 : 2) void mix_bounds_kernel(
   double a[NS];
   // a lane's coordinates come in runs of 8 (bnd_coord): 32-byte loads where
   // the rows are 32-byte aligned (k a multiple of 4, table aligned)
-  const double *tab = tgt ? p.XA : p.XB;
-  const bool vec = (p.k & 3) == 0 && ((uintptr_t)tab & 31) == 0;  // wave-uniform
+  const bool vec = rows_vec_aligned(tgt ? p.XA : p.XB, p.k);  // wave-uniform
 #pragma unroll
   for (int cc = 0; cc < KP / 32; ++cc) {
-    const int q0 = 32 * cc + 8 * g;
-    if (vec && valid && q0 + 8 <= p.k) {
-      const double4 *xv = reinterpret_cast<const double4 *>(x + q0);
-      const double4 u0 = xv[0], u1 = xv[1];
-      a[8 * cc + 0] = u0.x;
-      a[8 * cc + 1] = u0.y;
-      a[8 * cc + 2] = u0.z;
-      a[8 * cc + 3] = u0.w;
-      a[8 * cc + 4] = u1.x;
-      a[8 * cc + 5] = u1.y;
-      a[8 * cc + 6] = u1.z;
-      a[8 * cc + 7] = u1.w;
-    } else {
+    double v[8];
+    load_run8(valid ? x : nullptr, p.k, 32 * cc + 8 * g, vec, v);
 #pragma unroll
-      for (int t = 0; t < 8; ++t) a[8 * cc + t] = (valid && q0 + t < p.k) ? x[q0 + t] : 0.0;
-    }
+    for (int t = 0; t < 8; ++t) a[8 * cc + t] = v[t];
   }
   double n2 = 0.0;
 #pragma unroll
@@ -878,28 +877,25 @@ __global__ void mix_identity2_kernel(int *a, long m, int *b, long n) {
 // BF16 / FP32 layout, and with Bd non-null into the FP64 source layout too.
 template <int PREC>
 __device__ __forceinline__ void pack_chunk(const double *__restrict__ x, int k, int nc, long pt,
-                                           int c, int g, uint4 *pack) {
+                                           int c, int g, bool vec, uint4 *pack) {
   uint4 *o0 = pack + (pt >> 4) * nc * 128 + (c * 2) * 64 + g * 16 + (int)(pt & 15);
   uint4 *o1 = o0 + 64;
+  double v[8];
+  load_run8(x, k, c * 32 + 8 * g, vec, v);
   if constexpr (PREC == KS_PREC_MIXED_BF16X3) {
     bf8 hi, lo;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const int idx = c * 32 + 8 * g + t;
-      const double v = (x && idx < k) ? x[idx] : 0.0;
-      const __bf16 h = (__bf16)(float)v;
+      const __bf16 h = (__bf16)(float)v[t];
       hi[t] = h;
-      lo[t] = (__bf16)(float)(v - (double)(float)h);
+      lo[t] = (__bf16)(float)(v[t] - (double)(float)h);
     }
     *o0 = *reinterpret_cast<uint4 *>(&hi);
     *o1 = *reinterpret_cast<uint4 *>(&lo);
   } else {
     float f[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int idx = c * 32 + 8 * g + t;
-      f[t] = (x && idx < k) ? (float)x[idx] : 0.0f;
-    }
+    for (int t = 0; t < 8; ++t) f[t] = (float)v[t];
     *o0 = *reinterpret_cast<uint4 *>(&f[0]);
     *o1 = *reinterpret_cast<uint4 *>(&f[4]);
   }
@@ -957,7 +953,7 @@ __global__ void mixed_pack_kernel(const Problem p, int nc, float xs, long mp, lo
     const long t = u;
     if (t < p.m) {
       const int ai = p.amap[pk.prow[t]];
-      pack_chunk<PREC>(p.XA + (long)ai * p.k, p.k, nc, t, c, g, pk.Ap);
+      pack_chunk<PREC>(p.XA + (long)ai * p.k, p.k, nc, t, c, g, rows_vec_aligned(p.XA, p.k), pk.Ap);
       if (cg == 0) {
         const double aa = p.XA2[ai];
         const double hi = (TYPE == VAR_BANDWIDTH) ? p.hi[ai] : 0.0;
@@ -965,7 +961,7 @@ __global__ void mixed_pack_kernel(const Problem p, int nc, float xs, long mp, lo
         pk.rd[t] = make_double2(aa, -0.5 * hi);
       }
     } else {  // padding target: its estimated exponent is -inf, so it is never near
-      pack_chunk<PREC>(nullptr, p.k, nc, t, c, g, pk.Ap);
+      pack_chunk<PREC>(nullptr, p.k, nc, t, c, g, false, pk.Ap);
       if (cg == 0) {
         // Gaussian x2 = -2 xs acc + ...: acc = -inf gives -inf for xs < 0 (a
         // decaying kernel); for xs > 0 the row starts at +inf instead
@@ -980,7 +976,7 @@ __global__ void mixed_pack_kernel(const Problem p, int nc, float xs, long mp, lo
       const int jj = pk.pcol[j];
       const int bj = p.bmap[jj];
       const double *x = p.XB + (long)bj * p.k;
-      pack_chunk<PREC>(x, p.k, nc, j, c, g, pk.Bp);
+      pack_chunk<PREC>(x, p.k, nc, j, c, g, rows_vec_aligned(p.XB, p.k), pk.Bp);
       pack_f64_chunk(x, p.k, nc, j, c, g, pk.Bd);
       if (cg == 0) {
         const double bb = p.XB2[bj];
@@ -995,7 +991,7 @@ __global__ void mixed_pack_kernel(const Problem p, int nc, float xs, long mp, lo
         pk.cd[j] = make_double4(bb, w, hj, 0.0);
       }
     } else {  // padding source: exponent -inf, weight 0
-      pack_chunk<PREC>(nullptr, p.k, nc, j, c, g, pk.Bp);
+      pack_chunk<PREC>(nullptr, p.k, nc, j, c, g, false, pk.Bp);
       pack_f64_chunk(nullptr, p.k, nc, j, c, g, pk.Bd);
       if (cg == 0) {
         pk.cb[j] = (TYPE == GAUSSIAN) ? make_float4(-INFINITY, 0.0f, 0.0f, 0.0f)
